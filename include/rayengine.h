@@ -1,0 +1,212 @@
+/*
+ * rayengine.h - C ABI of the MI355X (gfx950) ray-query library.
+ *
+ * Plain C: POD structs, raw pointers and sizes; no HIP or torch types cross
+ * this boundary.  Every call returns an re_status and never exits the
+ * process; re_last_error() describes the last failure.  A context is not
+ * thread-safe (one context per host thread, or external locking).  Each
+ * entry point is stream-ordered on the context's stream and returns after
+ * the work is complete unless RE_ASYNC is passed in flags.
+ *
+ * Each declaration names the reference interface it replaces
+ * (hadalhw17/RayEngine, paths relative to the repository root).
+ */
+#ifndef RAYENGINE_H
+#define RAYENGINE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RE_API __attribute__((visibility("default")))
+
+typedef enum re_status {
+  RE_OK = 0,
+  RE_ERR_INVALID = 1,     /* bad argument, shape or id */
+  RE_ERR_HIP = 2,         /* HIP runtime failure */
+  RE_ERR_OOM = 3,         /* device or host allocation failed */
+  RE_ERR_NOT_FOUND = 4,   /* unknown scene / volume / network id */
+  RE_ERR_UNSUPPORTED = 5, /* no gfx950 device, or a feature not built */
+  RE_ERR_IO = 6           /* file missing, short or malformed */
+} re_status;
+
+/* flags */
+#define RE_DEVICE_PTRS 0x1u /* ray and result pointers are device memory */
+#define RE_ASYNC 0x2u       /* return without synchronising the stream */
+
+typedef struct re_ctx re_ctx;
+typedef struct re_kdtree re_kdtree;
+
+/* ---------------------------------------------------------------- context */
+
+/* Replaces the reference's TU-global device state
+ * (RayEngine/src/cuda-src/CUDARayTracing.cuh:15-85). */
+RE_API re_status re_create(int device, re_ctx** out);
+/* Replaces free_memory (RayEngine/src/cuda-src/sphere_tracing.cuh:757-767). */
+RE_API void re_destroy(re_ctx* ctx);
+/* Replaces gpuErrchk/gpuAssert (RayEngine/src/RayEngine/RayEngine.h:56-63):
+ * errors are reported, not fatal.  ctx may be NULL (last error of a call that
+ * had no context). */
+RE_API const char* re_last_error(const re_ctx* ctx);
+/* Library version string. */
+RE_API const char* re_version(void);
+
+/* --------------------------------------------- host scene preparation */
+
+/* Subdivided icosphere, outward counter-clockwise triangles, as a soup of
+ * 9 floats per triangle (v0, v1, v2).  n_tris = 20 * 4^subdiv.  Called with
+ * out == NULL it only reports the count. */
+RE_API re_status re_gen_icosphere(int subdiv, float radius, float* out, int64_t* n_tris);
+
+/* n random triangles: centroid ~ U[lo,hi]^3, two edges of length
+ * ~ U[len_lo,len_hi] along random unit directions (PCG32, seed). */
+RE_API re_status re_gen_random_tris(uint64_t seed, int64_t n, float lo, float hi, float len_lo,
+                                    float len_hi, float* out);
+
+/* n triangles in n_clusters Gaussian clusters: cluster centre ~ U[lo,hi]^3,
+ * triangle centroid ~ centre + N(0, sigma^2 I), edge lengths
+ * ~ LogNormal(ln_mu, ln_sigma) along random unit directions (PCG32, seed). */
+RE_API re_status re_gen_cluster_soup(uint64_t seed, int64_t n, int n_clusters, float lo, float hi,
+                                     float sigma, float ln_mu, float ln_sigma, float* out);
+
+/* Pinhole camera, reference convention.  RCamera
+ * (RayEngine/src/Primitives/Camera.h:7-16) and generate_ray
+ * (RayEngine/src/cuda-src/ray_functions.cuh:13-55). */
+typedef struct re_camera {
+  float pos[3];
+  float view[3];    /* viewing direction (normalised by the ray generator) */
+  float camdown[3]; /* reference default (0,-1,0) */
+  float fov[2];     /* full field of view in degrees */
+  float focal_distance;
+} re_camera;
+
+/* width*height primary rays, pixel (x, y) at index y*width + x; origins and
+ * directions as 3 floats per ray. */
+RE_API re_status re_gen_camera_rays(const re_camera* cam, int width, int height, float* ray_o,
+                                    float* ray_d);
+
+/* One cosine-weighted diffuse bounce ray per hit (prim_id >= 0): origin =
+ * hit point + 1e-4 * n, n the geometric normal facing the incoming ray;
+ * direction sampled about n (PCG32 stream (seed, ray index)).  tri_v is the
+ * triangle soup the prim ids index.  Writes at most n rays and returns the
+ * count in n_out. */
+RE_API re_status re_gen_bounce_rays(const float* tri_v, int64_t n_tris, const float* ray_o,
+                                    const float* ray_d, const int32_t* prim_id, const float* t,
+                                    int64_t n, uint64_t seed, float* out_o, float* out_d,
+                                    int64_t* n_out);
+
+/* ------------------------------------------------------ kd-tree build */
+
+/* Replaces RKDTreeCPU (RayEngine/src/Primitives/KDTree.h:39-99,
+ * KDTree.cpp:117-834) and its flattening RKDThreeGPU
+ * (RayEngine/src/Primitives/KDThreeGPU.cpp:155-194): binned SAH build with
+ * ropes (Havran/Popov), split boxes on leaves, six ropes per leaf. */
+typedef struct re_kd_build_params {
+  int32_t max_leaf_tris;  /* leaf when a node holds <= this many (default 8) */
+  int32_t max_depth;      /* leaf at this depth (root depth 1; default 24) */
+  float cost_traversal;   /* SAH traversal cost (default 1.0) */
+  float cost_intersect;   /* SAH intersection cost (default 1.5) */
+  float empty_bonus;      /* SAH factor for a split with an empty side (default 0.8) */
+  int32_t n_bins;         /* bins per axis for large nodes (default 32) */
+} re_kd_build_params;
+
+typedef struct re_kd_info {
+  int64_t n_tris;    /* input triangles */
+  int64_t n_nodes;   /* 8-byte node entries */
+  int64_t n_leaves;  /* 64-byte leaf records */
+  int64_t n_trirecs; /* 40-byte triangle records (with straddler copies) */
+  int32_t depth;     /* deepest leaf, root = 1 */
+  int32_t max_leaf;  /* largest leaf population */
+  float root_min[3];
+  float root_max[3];
+} re_kd_info;
+
+/* Fills p with the defaults above. */
+RE_API void re_kd_default_params(re_kd_build_params* p);
+/* Builds a tree over the soup tri_v (9 floats per triangle; prim ids are
+ * 0..n_tris-1).  params may be NULL. */
+RE_API re_status re_kd_build(const float* tri_v, int64_t n_tris, const re_kd_build_params* params,
+                             re_kdtree** out);
+RE_API void re_kd_free(re_kdtree* tree);
+RE_API re_status re_kd_get_info(const re_kdtree* tree, re_kd_info* info);
+/* Copies the flattened arrays out (sizes from re_kd_get_info): nodes
+ * n_nodes*8 bytes, leaves n_leaves*64 bytes, tris n_trirecs*40 bytes.  Any
+ * pointer may be NULL to skip it. */
+RE_API re_status re_kd_export(const re_kdtree* tree, void* nodes, void* leaves, void* tris);
+
+/* ------------------------------------------------------ scene upload */
+
+/* One traced object: a tree plus its transform.  GPUSceneObject
+ * (RayEngine/src/cuda-src/gpu_structs.h:69-90); rotation enters as the
+ * sine and cosine of rotation.y, evaluated by the caller. */
+typedef struct re_object {
+  const re_kdtree* tree;
+  float location[3];
+  float sin_rot_y;
+  float cos_rot_y;
+  int32_t is_character;
+  int32_t prim_offset; /* added to the tree's prim ids in the results */
+} re_object;
+
+/* Replaces copy_memory (RayEngine/src/cuda-src/cuda_memory_functions.cuh:58-211)
+ * and update_objects (:50-56): concatenates the objects' trees and uploads
+ * them.  Host data is copied; the context owns the device copy. */
+RE_API re_status re_scene_upload(re_ctx* ctx, const re_object* objs, int32_t n_obj,
+                                 int32_t* scene_id);
+RE_API re_status re_scene_release(re_ctx* ctx, int32_t scene_id);
+/* Device footprint of a scene in bytes. */
+RE_API re_status re_scene_bytes(re_ctx* ctx, int32_t scene_id, int64_t* bytes);
+/* sizes[0..3] = node entries, leaf records, triangle records, objects. */
+RE_API re_status re_scene_info(re_ctx* ctx, int32_t scene_id, int64_t sizes[4]);
+/* Copies the resident scene back to host memory (sizes from re_scene_info;
+ * objects are 52-byte records {location[3], sin_y, cos_y, is_character,
+ * root entry, root_min[3], root_max[3]}).  Any pointer may be NULL.  The
+ * reference's tree dump, RKDThreeGPU::PrintCPUAndGPUTrees
+ * (RayEngine/src/Primitives/KDThreeGPU.cpp:137-153), read the same data. */
+RE_API re_status re_scene_export(re_ctx* ctx, int32_t scene_id, void* nodes, void* leaves,
+                                 void* tris, void* objs);
+
+/* ------------------------------------------------------ closest hit */
+
+typedef struct re_trace_params {
+  int32_t max_leaf_visits; /* reference cap 50 (ray_functions.cuh:284); <= 0 = unlimited */
+  int32_t one_sided;       /* 1: reference one-sided test (ray_functions.cuh:182) */
+  int32_t secondary;       /* is_secondary of stackless_kdtree_traversal */
+  int32_t block;           /* threads per block, 0 = tuned default */
+} re_trace_params;
+
+RE_API void re_trace_default_params(re_trace_params* p);
+
+/* Closest hit for n rays (origins/directions 3 floats per ray).  Replaces
+ * trace_primary_rays (RayEngine/src/cuda-src/renderer.cu:389-410) over
+ * trace_scene / stackless_kdtree_traversal
+ * (RayEngine/src/cuda-src/ray_functions.cuh:230-396).  Misses give
+ * prim_id = -1 and t = 1e20; obj_id may be NULL. */
+RE_API re_status re_trace_closest(re_ctx* ctx, int32_t scene_id, const float* ray_o,
+                                  const float* ray_d, int64_t n, const re_trace_params* params,
+                                  int32_t* prim_id, float* t, int32_t* obj_id, uint32_t flags);
+
+/* Same traversal, also summing its work counters: counts[0] node entries
+ * read, counts[1] leaf records read, counts[2] triangle records tested,
+ * counts[3] rays with a hit.  These give the algorithmic bytes of a launch:
+ * 24 + 8 per ray, 8 per entry, 56 per leaf, 40 per triangle. */
+RE_API re_status re_trace_counts(re_ctx* ctx, int32_t scene_id, const float* ray_o,
+                                 const float* ray_d, int64_t n, const re_trace_params* params,
+                                 int32_t* prim_id, float* t, uint64_t counts[4], uint32_t flags);
+
+/* Elapsed milliseconds of the last kernel launch of the context, measured
+ * with HIP events recorded around the launch on its stream (waits for it).
+ * Replaces the reference's FPS counter (RayEngine/src/Engine/Window.cpp:46-56). */
+RE_API re_status re_last_kernel_ms(re_ctx* ctx, float* ms);
+/* Durations of the last min(max, 256) kernel launches, oldest first (waits
+ * for them); *count gets the number written. */
+RE_API re_status re_kernel_times(re_ctx* ctx, float* ms, int32_t max, int32_t* count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RAYENGINE_H */

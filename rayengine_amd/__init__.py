@@ -1,0 +1,289 @@
+"""rayengine_amd - MI355X-native ray-query path of RayEngine.
+
+Python host mirror of the C ABI in include/rayengine.h.  The classes follow
+the reference's host-side objects:
+
+* KDTree        - RKDTreeCPU + RKDThreeGPU (RayEngine/src/Primitives/KDTree.h:39-99,
+                  KDThreeGPU.h:9): build over a triangle soup, flattened for the GPU.
+* SceneObject   - GPUSceneObject (RayEngine/src/cuda-src/gpu_structs.h:69-90).
+* Context.upload_scene - copy_memory (RayEngine/src/cuda-src/cuda_memory_functions.cuh:58-211).
+* Scene.trace_closest  - trace_primary_rays / trace_scene
+                  (RayEngine/src/cuda-src/renderer.cu:389-410, ray_functions.cuh:376-396).
+
+All compute runs in librayengine.so (HIP kernels for gfx950); this module
+only moves arrays across the boundary.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from ._lib import (LEAF_DT, NODE_DT, OBJ_DT, RE_ASYNC, RE_DEVICE_PTRS, TRI_DT, ReError, check, lib, ptr)
+
+__all__ = ["KDTree", "SceneObject", "Context", "Scene", "Camera", "ReError", "icosphere", "random_tris",
+           "cluster_soup", "camera_rays", "bounce_rays", "NODE_DT", "LEAF_DT", "TRI_DT", "OBJ_DT",
+           "RE_DEVICE_PTRS", "RE_ASYNC", "K_EPSILON", "K_INFINITY"]
+
+K_EPSILON = np.float32(1e-4)   # RayEngine/src/RayEngine/RayEngine.h:24
+K_INFINITY = np.float32(1e20)  # RayEngine/src/RayEngine/RayEngine.h:25
+
+
+def _f32c(a, shape_last=None) -> np.ndarray:
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if shape_last is not None and a.size % shape_last:
+        raise ValueError(f"array size {a.size} is not a multiple of {shape_last}")
+    return a
+
+
+# ------------------------------------------------------------------ generators
+
+def icosphere(subdiv: int, radius: float = 1.0) -> np.ndarray:
+    """Subdivided icosphere as an (n, 9) float32 soup, outward CCW faces."""
+    n = C.c_int64()
+    check(lib().re_gen_icosphere(subdiv, radius, None, C.byref(n)))
+    out = np.empty((n.value, 9), np.float32)
+    check(lib().re_gen_icosphere(subdiv, radius, ptr(out), C.byref(n)))
+    return out
+
+
+def random_tris(seed: int, n: int, lo=-2.0, hi=2.0, len_lo=0.02, len_hi=0.2) -> np.ndarray:
+    out = np.empty((n, 9), np.float32)
+    check(lib().re_gen_random_tris(seed, n, lo, hi, len_lo, len_hi, ptr(out)))
+    return out
+
+
+def cluster_soup(seed: int, n: int, n_clusters=64, lo=-10.0, hi=10.0, sigma=0.3, ln_mu=-3.0,
+                 ln_sigma=0.5) -> np.ndarray:
+    out = np.empty((n, 9), np.float32)
+    check(lib().re_gen_cluster_soup(seed, n, n_clusters, lo, hi, sigma, ln_mu, ln_sigma, ptr(out)))
+    return out
+
+
+@dataclass
+class Camera:
+    """RCamera (RayEngine/src/Primitives/Camera.h:7-16); defaults of RMovableCamera
+    (RayEngine/src/Objects/MovableCamera.cpp:4-14,101-118)."""
+    pos: tuple = (0.0, 0.0, 5.0)
+    view: tuple = (0.0, 0.0, -1.0)
+    camdown: tuple = (0.0, -1.0, 0.0)
+    fov: tuple = (60.0, 60.0)
+    focal_distance: float = 4.0
+
+    @staticmethod
+    def look_at(pos, target, **kw) -> "Camera":
+        v = np.asarray(target, np.float64) - np.asarray(pos, np.float64)
+        return Camera(pos=tuple(map(float, pos)), view=tuple(map(float, v)), **kw)
+
+    def _c(self) -> _lib.re_camera:
+        c = _lib.re_camera()
+        c.pos[:] = self.pos
+        c.view[:] = self.view
+        c.camdown[:] = self.camdown
+        c.fov[:] = self.fov
+        c.focal_distance = self.focal_distance
+        return c
+
+
+def camera_rays(cam: Camera, width: int, height: int) -> tuple[np.ndarray, np.ndarray]:
+    """Primary rays of generate_ray (RayEngine/src/cuda-src/ray_functions.cuh:13-55)."""
+    o = np.empty((width * height, 3), np.float32)
+    d = np.empty((width * height, 3), np.float32)
+    c = cam._c()
+    check(lib().re_gen_camera_rays(C.byref(c), width, height, ptr(o), ptr(d)))
+    return o, d
+
+
+def bounce_rays(tri_v, o, d, prim, t, seed: int) -> tuple[np.ndarray, np.ndarray]:
+    """One cosine-weighted diffuse bounce per hit (prim >= 0)."""
+    tri_v = _f32c(tri_v, 9)
+    o, d = _f32c(o, 3), _f32c(d, 3)
+    prim = np.ascontiguousarray(prim, np.int32)
+    t = _f32c(t)
+    n = prim.size
+    oo = np.empty((n, 3), np.float32)
+    od = np.empty((n, 3), np.float32)
+    m = C.c_int64()
+    check(lib().re_gen_bounce_rays(ptr(tri_v), tri_v.size // 9, ptr(o), ptr(d), ptr(prim), ptr(t), n, seed,
+                                   ptr(oo), ptr(od), C.byref(m)))
+    return oo[:m.value].copy(), od[:m.value].copy()
+
+
+# ------------------------------------------------------------------ kd-tree
+
+class KDTree:
+    """Rope kd-tree over a triangle soup ((n, 9) float32: v0, v1, v2)."""
+
+    def __init__(self, tri_v, max_leaf_tris: int = 8, max_depth: int = 24, cost_traversal: float = 1.0,
+                 cost_intersect: float = 1.5, empty_bonus: float = 0.8, n_bins: int = 32):
+        self.tri_v = _f32c(tri_v, 9).reshape(-1, 9)
+        p = _lib.re_kd_build_params(max_leaf_tris, max_depth, cost_traversal, cost_intersect, empty_bonus, n_bins)
+        h = C.c_void_p()
+        check(lib().re_kd_build(ptr(self.tri_v), self.tri_v.shape[0], C.byref(p), C.byref(h)))
+        self._h = h
+        info = _lib.re_kd_info()
+        check(lib().re_kd_get_info(h, C.byref(info)))
+        self.info = {"n_tris": info.n_tris, "n_nodes": info.n_nodes, "n_leaves": info.n_leaves,
+                     "n_trirecs": info.n_trirecs, "depth": info.depth, "max_leaf": info.max_leaf,
+                     "root_min": np.array(info.root_min[:], np.float32),
+                     "root_max": np.array(info.root_max[:], np.float32)}
+
+    @property
+    def handle(self):
+        return self._h
+
+    def export(self):
+        """(nodes, leaves, tris) as structured numpy arrays (NODE_DT, LEAF_DT, TRI_DT)."""
+        nodes = np.zeros(self.info["n_nodes"], NODE_DT)
+        leaves = np.zeros(self.info["n_leaves"], LEAF_DT)
+        tris = np.zeros(self.info["n_trirecs"], TRI_DT)
+        check(lib().re_kd_export(self._h, ptr(nodes), ptr(leaves), ptr(tris)))
+        return nodes, leaves, tris
+
+    def identity_object(self) -> np.ndarray:
+        """The object record of this tree alone at identity transform (OBJ_DT)."""
+        ob = np.zeros(1, OBJ_DT)
+        ob["cos_y"] = 1.0
+        ob["root"] = 0
+        ob["root_min"] = self.info["root_min"]
+        ob["root_max"] = self.info["root_max"]
+        return ob
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _lib._lib is not None:
+            _lib._lib.re_kd_free(h)
+            self._h = None
+
+
+@dataclass
+class SceneObject:
+    """GPUSceneObject: a tree, a location and a rotation about Y (radians)."""
+    tree: KDTree
+    location: tuple = (0.0, 0.0, 0.0)
+    rotation_y: float = 0.0
+    is_character: bool = False
+    prim_offset: int = 0
+
+
+# ------------------------------------------------------------------ device
+
+class Context:
+    """Owns a HIP stream and the device-resident scenes (re_ctx)."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        check(lib().re_create(device, C.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def last_kernel_ms(self) -> float:
+        v = C.c_float()
+        check(lib().re_last_kernel_ms(self._h, C.byref(v)), self._h)
+        return v.value
+
+    def kernel_times(self, max_n: int = 256) -> np.ndarray:
+        """Durations (ms) of the last launches, oldest first (HIP events)."""
+        buf = np.zeros(max_n, np.float32)
+        k = C.c_int32()
+        check(lib().re_kernel_times(self._h, buf.ctypes.data_as(_lib._F), max_n, C.byref(k)), self._h)
+        return buf[:k.value].copy()
+
+    def upload_scene(self, objects: list[SceneObject]) -> "Scene":
+        arr = (_lib.re_object * len(objects))()
+        for i, ob in enumerate(objects):
+            a = arr[i]
+            a.tree = ob.tree.handle
+            a.location[:] = ob.location
+            a.sin_rot_y = float(np.float32(math.sin(ob.rotation_y)))
+            a.cos_rot_y = float(np.float32(math.cos(ob.rotation_y)))
+            a.is_character = int(ob.is_character)
+            a.prim_offset = int(ob.prim_offset)
+        sid = C.c_int32()
+        check(lib().re_scene_upload(self._h, arr, len(objects), C.byref(sid)), self._h)
+        return Scene(self, sid.value, list(objects))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().re_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _trace_params(max_leaf_visits=50, one_sided=True, secondary=False, block=0):
+    return _lib.re_trace_params(int(max_leaf_visits), int(bool(one_sided)), int(bool(secondary)), int(block))
+
+
+class Scene:
+    def __init__(self, ctx: Context, sid: int, objects):
+        self.ctx, self.id, self.objects = ctx, sid, objects
+
+    def bytes(self) -> int:
+        v = C.c_int64()
+        check(lib().re_scene_bytes(self.ctx.handle, self.id, C.byref(v)), self.ctx.handle)
+        return v.value
+
+    def export(self):
+        sizes = (C.c_int64 * 4)()
+        check(lib().re_scene_info(self.ctx.handle, self.id, sizes), self.ctx.handle)
+        nodes = np.zeros(sizes[0], NODE_DT)
+        leaves = np.zeros(sizes[1], LEAF_DT)
+        tris = np.zeros(sizes[2], TRI_DT)
+        objs = np.zeros(sizes[3], OBJ_DT)
+        check(lib().re_scene_export(self.ctx.handle, self.id, ptr(nodes), ptr(leaves), ptr(tris), ptr(objs)),
+              self.ctx.handle)
+        return nodes, leaves, tris, objs
+
+    def trace_closest(self, o, d, with_obj=False, **params):
+        """Closest hit of each ray: (prim_id int32, t float32[, obj_id])."""
+        o, d = _f32c(o, 3), _f32c(d, 3)
+        n = o.size // 3
+        prim = np.empty(n, np.int32)
+        t = np.empty(n, np.float32)
+        obj = np.empty(n, np.int32) if with_obj else None
+        p = _trace_params(**params)
+        check(lib().re_trace_closest(self.ctx.handle, self.id, ptr(o), ptr(d), n, C.byref(p), ptr(prim), ptr(t),
+                                     ptr(obj), 0), self.ctx.handle)
+        return (prim, t, obj) if with_obj else (prim, t)
+
+    def trace_closest_device(self, o_ptr: int, d_ptr: int, n: int, prim_ptr: int, t_ptr: int, sync=True, **params):
+        """Same on device-resident buffers (raw device pointers)."""
+        p = _trace_params(**params)
+        flags = RE_DEVICE_PTRS | (0 if sync else RE_ASYNC)
+        check(lib().re_trace_closest(self.ctx.handle, self.id, o_ptr, d_ptr, n, C.byref(p), prim_ptr, t_ptr, None,
+                                     flags), self.ctx.handle)
+
+    def trace_counts(self, o, d, **params):
+        """(prim, t, counts) with counts = [entries, leaves, tris, rays_hit] summed over rays."""
+        o, d = _f32c(o, 3), _f32c(d, 3)
+        n = o.size // 3
+        prim = np.empty(n, np.int32)
+        t = np.empty(n, np.float32)
+        cnt = (C.c_uint64 * 4)()
+        p = _trace_params(**params)
+        check(lib().re_trace_counts(self.ctx.handle, self.id, ptr(o), ptr(d), n, C.byref(p), ptr(prim), ptr(t),
+                                    cnt, 0), self.ctx.handle)
+        return prim, t, list(cnt)
+
+    def trace_counts_device(self, o_ptr: int, d_ptr: int, n: int, prim_ptr: int, t_ptr: int, **params):
+        cnt = (C.c_uint64 * 4)()
+        p = _trace_params(**params)
+        check(lib().re_trace_counts(self.ctx.handle, self.id, o_ptr, d_ptr, n, C.byref(p), prim_ptr, t_ptr, cnt,
+                                    RE_DEVICE_PTRS), self.ctx.handle)
+        return list(cnt)
+
+    def release(self):
+        if self.id is not None:
+            check(lib().re_scene_release(self.ctx.handle, self.id), self.ctx.handle)
+            self.id = None

@@ -1,0 +1,133 @@
+"""ctypes binding of librayengine.so (the C ABI declared in include/rayengine.h).
+
+The shared library is built in-tree by rayengine_amd.build.  Loading fails
+loudly when it is missing: there is no fallback implementation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "librayengine.so"
+
+RE_OK = 0
+RE_DEVICE_PTRS = 0x1
+RE_ASYNC = 0x2
+
+STATUS_NAMES = {0: "RE_OK", 1: "RE_ERR_INVALID", 2: "RE_ERR_HIP", 3: "RE_ERR_OOM", 4: "RE_ERR_NOT_FOUND",
+                5: "RE_ERR_UNSUPPORTED", 6: "RE_ERR_IO"}
+
+# Record layouts of the flattened scene (see csrc/re_math.hpp).
+NODE_DT = np.dtype([("split", "<f4"), ("meta", "<u4")])
+LEAF_DT = np.dtype([("bmin", "<f4", 3), ("bmax", "<f4", 3), ("rope", "<i4", 6), ("first", "<u4"),
+                    ("count", "<u4"), ("pad", "<u4", 2)])
+TRI_DT = np.dtype([("v0", "<f4", 3), ("e1", "<f4", 3), ("e2", "<f4", 3), ("prim", "<i4")])
+OBJ_DT = np.dtype([("loc", "<f4", 3), ("sin_y", "<f4"), ("cos_y", "<f4"), ("is_character", "<i4"),
+                   ("root", "<i4"), ("root_min", "<f4", 3), ("root_max", "<f4", 3)])
+assert NODE_DT.itemsize == 8 and LEAF_DT.itemsize == 64 and TRI_DT.itemsize == 40 and OBJ_DT.itemsize == 52
+
+
+class ReError(RuntimeError):
+    """A C-ABI call returned a status other than RE_OK."""
+
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {msg}")
+        self.status = status
+
+
+class re_camera(C.Structure):
+    _fields_ = [("pos", C.c_float * 3), ("view", C.c_float * 3), ("camdown", C.c_float * 3),
+                ("fov", C.c_float * 2), ("focal_distance", C.c_float)]
+
+
+class re_kd_build_params(C.Structure):
+    _fields_ = [("max_leaf_tris", C.c_int32), ("max_depth", C.c_int32), ("cost_traversal", C.c_float),
+                ("cost_intersect", C.c_float), ("empty_bonus", C.c_float), ("n_bins", C.c_int32)]
+
+
+class re_kd_info(C.Structure):
+    _fields_ = [("n_tris", C.c_int64), ("n_nodes", C.c_int64), ("n_leaves", C.c_int64),
+                ("n_trirecs", C.c_int64), ("depth", C.c_int32), ("max_leaf", C.c_int32),
+                ("root_min", C.c_float * 3), ("root_max", C.c_float * 3)]
+
+
+class re_object(C.Structure):
+    _fields_ = [("tree", C.c_void_p), ("location", C.c_float * 3), ("sin_rot_y", C.c_float),
+                ("cos_rot_y", C.c_float), ("is_character", C.c_int32), ("prim_offset", C.c_int32)]
+
+
+class re_trace_params(C.Structure):
+    _fields_ = [("max_leaf_visits", C.c_int32), ("one_sided", C.c_int32), ("secondary", C.c_int32),
+                ("block", C.c_int32)]
+
+
+_P = C.c_void_p
+_F = C.POINTER(C.c_float)
+_I32 = C.POINTER(C.c_int32)
+_I64 = C.POINTER(C.c_int64)
+_U64 = C.POINTER(C.c_uint64)
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "re_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    "re_destroy": (None, [_P]),
+    "re_last_error": (C.c_char_p, [_P]),
+    "re_version": (C.c_char_p, []),
+    "re_gen_icosphere": (C.c_int, [C.c_int, C.c_float, _P, _I64]),
+    "re_gen_random_tris": (C.c_int, [C.c_uint64, C.c_int64, C.c_float, C.c_float, C.c_float, C.c_float, _P]),
+    "re_gen_cluster_soup": (C.c_int, [C.c_uint64, C.c_int64, C.c_int, C.c_float, C.c_float, C.c_float,
+                                      C.c_float, C.c_float, _P]),
+    "re_gen_camera_rays": (C.c_int, [C.POINTER(re_camera), C.c_int, C.c_int, _P, _P]),
+    "re_gen_bounce_rays": (C.c_int, [_P, C.c_int64, _P, _P, _P, _P, C.c_int64, C.c_uint64, _P, _P, _I64]),
+    "re_kd_default_params": (None, [C.POINTER(re_kd_build_params)]),
+    "re_kd_build": (C.c_int, [_P, C.c_int64, C.POINTER(re_kd_build_params), C.POINTER(C.c_void_p)]),
+    "re_kd_free": (None, [_P]),
+    "re_kd_get_info": (C.c_int, [_P, C.POINTER(re_kd_info)]),
+    "re_kd_export": (C.c_int, [_P, _P, _P, _P]),
+    "re_scene_upload": (C.c_int, [_P, C.POINTER(re_object), C.c_int32, _I32]),
+    "re_scene_release": (C.c_int, [_P, C.c_int32]),
+    "re_scene_bytes": (C.c_int, [_P, C.c_int32, _I64]),
+    "re_scene_info": (C.c_int, [_P, C.c_int32, _I64]),
+    "re_scene_export": (C.c_int, [_P, C.c_int32, _P, _P, _P, _P]),
+    "re_trace_default_params": (None, [C.POINTER(re_trace_params)]),
+    "re_trace_closest": (C.c_int, [_P, C.c_int32, _P, _P, C.c_int64, C.POINTER(re_trace_params), _P, _P, _P,
+                                   C.c_uint32]),
+    "re_trace_counts": (C.c_int, [_P, C.c_int32, _P, _P, C.c_int64, C.POINTER(re_trace_params), _P, _P, _U64,
+                                  C.c_uint32]),
+    "re_last_kernel_ms": (C.c_int, [_P, _F]),
+    "re_kernel_times": (C.c_int, [_P, _F, C.c_int32, _I32]),
+}
+
+_lib: C.CDLL | None = None
+
+
+def lib() -> C.CDLL:
+    """Loads librayengine.so once; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise ImportError(f"{LIB_PATH} is missing: build it with `python -m rayengine_amd.build`")
+        L = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(status: int, ctx=None) -> None:
+    if status != RE_OK:
+        msg = lib().re_last_error(ctx)
+        raise ReError(status, msg.decode() if msg else "")
+
+
+def ptr(a) -> int:
+    """Address of a numpy array's data (or an int device pointer)."""
+    if a is None:
+        return None
+    if isinstance(a, int):
+        return a
+    return a.ctypes.data

@@ -1,0 +1,367 @@
+// C-ABI implementation: context, device residency of scenes, and the
+// closest-hit entry point.  All device work is ordered on the context's
+// stream; errors come back as re_status with a message, never as exit().
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <new>
+
+#include "host/errors.hpp"
+#include "internal.hpp"
+
+using namespace re;
+
+namespace {
+
+re_status set_err(re_ctx* ctx, re_status st, const char* msg) {
+  fail(st, "%s", msg);
+  if (ctx) ctx->err = msg;
+  return st;
+}
+
+re_status hip_err(re_ctx* ctx, hipError_t e, const char* where) {
+  char buf[400];
+  snprintf(buf, sizeof(buf), "%s: %s", where, hipGetErrorString(e));
+  return set_err(ctx, e == hipErrorOutOfMemory ? RE_ERR_OOM : RE_ERR_HIP, buf);
+}
+
+#define HIP_TRY(ctx, expr, where)               \
+  do {                                          \
+    hipError_t _e = (expr);                     \
+    if (_e != hipSuccess) return hip_err(ctx, _e, where); \
+  } while (0)
+
+void free_scene(DevScene& s) {
+  (void)hipFree(s.nodes);
+  (void)hipFree(s.leaves);
+  (void)hipFree(s.tris);
+  (void)hipFree(s.objs);
+  s = DevScene{};
+}
+
+// Grows the context's staging buffer to at least `bytes`.
+re_status ensure_stage(re_ctx* ctx, size_t bytes) {
+  if (ctx->stage_bytes >= bytes) return RE_OK;
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream), "stage sync");
+  if (ctx->stage) (void)hipFree(ctx->stage);
+  ctx->stage = nullptr;
+  ctx->stage_bytes = 0;
+  size_t want = bytes + bytes / 4;
+  HIP_TRY(ctx, hipMalloc(&ctx->stage, want), "stage hipMalloc");
+  ctx->stage_bytes = want;
+  return RE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+RE_API const char* re_version(void) { return "rayengine-mi355x 0.1 (gfx950)"; }
+
+RE_API re_status re_create(int device, re_ctx** out) {
+  if (!out) return fail(RE_ERR_INVALID, "re_create: out is null");
+  *out = nullptr;
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n <= 0) return fail(RE_ERR_UNSUPPORTED, "re_create: no HIP device (%s)", hipGetErrorString(e));
+  if (device < 0 || device >= n) return fail(RE_ERR_INVALID, "re_create: device %d of %d", device, n);
+  hipDeviceProp_t prop;
+  e = hipGetDeviceProperties(&prop, device);
+  if (e != hipSuccess) return fail(RE_ERR_HIP, "re_create: %s", hipGetErrorString(e));
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return fail(RE_ERR_UNSUPPORTED, "re_create: device is %s, library is built for gfx950", prop.gcnArchName);
+  re_ctx* ctx = new (std::nothrow) re_ctx();
+  if (!ctx) return fail(RE_ERR_OOM, "re_create: out of host memory");
+  ctx->device = device;
+  if ((e = hipSetDevice(device)) == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+  for (int i = 0; e == hipSuccess && i < 2 * re_ctx::kRing; ++i) e = hipEventCreate(&ctx->ring[i]);
+  if (e == hipSuccess) e = hipMalloc(&ctx->counts_buf, 256);
+  if (e != hipSuccess) {
+    re_status st = fail(RE_ERR_HIP, "re_create: %s", hipGetErrorString(e));
+    re_destroy(ctx);
+    return st;
+  }
+  *out = ctx;
+  return RE_OK;
+}
+
+RE_API void re_destroy(re_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (auto& kv : ctx->scenes) free_scene(kv.second);
+  if (ctx->stage) (void)hipFree(ctx->stage);
+  if (ctx->counts_buf) (void)hipFree(ctx->counts_buf);
+  for (int i = 0; i < 2 * re_ctx::kRing; ++i)
+    if (ctx->ring[i]) (void)hipEventDestroy(ctx->ring[i]);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+RE_API const char* re_last_error(const re_ctx* ctx) {
+  if (ctx && !ctx->err.empty()) return ctx->err.c_str();
+  return g_last_error;
+}
+
+RE_API re_status re_scene_upload(re_ctx* ctx, const re_object* objs, int32_t n_obj, int32_t* scene_id) {
+  if (!ctx || !objs || n_obj <= 0 || !scene_id) return set_err(ctx, RE_ERR_INVALID, "re_scene_upload: bad arguments");
+  for (int32_t k = 0; k < n_obj; ++k)
+    if (!objs[k].tree) return set_err(ctx, RE_ERR_INVALID, "re_scene_upload: object without a tree");
+  HIP_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+  // Concatenate, offsetting entry indices, leaf ids, triangle indices and
+  // prim ids per object (RayEngine/src/World/RTScene.cpp:263-279).
+  std::vector<NodeEntry> nodes;
+  std::vector<LeafRec> leaves;
+  std::vector<TriRec> tris;
+  std::vector<ObjXform> xf(n_obj);
+  for (int32_t k = 0; k < n_obj; ++k) {
+    const re_object& ob = objs[k];
+    const auto& tn = kd_nodes(ob.tree);
+    const auto& tl = kd_leaves(ob.tree);
+    const auto& tt = kd_tris(ob.tree);
+    re_kd_info info;
+    re_kd_get_info(ob.tree, &info);
+    const uint32_t nb = (uint32_t)nodes.size(), lb = (uint32_t)leaves.size(), tb = (uint32_t)tris.size();
+    if ((uint64_t)nb + tn.size() >= 0x3fffffffull || (uint64_t)lb + tl.size() >= 0x3fffffffull)
+      return set_err(ctx, RE_ERR_INVALID, "re_scene_upload: scene too large for 30-bit node indices");
+    for (NodeEntry e : tn) {
+      if ((e.meta >> 30) == 3u) e.meta = kLeafMeta | ((e.meta & 0x3fffffffu) + lb);
+      else e.meta = (e.meta & 0xc0000000u) | ((e.meta & 0x3fffffffu) + nb);
+      nodes.push_back(e);
+    }
+    for (LeafRec l : tl) {
+      for (int f = 0; f < 6; ++f) {
+        int32_t r = l.rope[f];
+        if (r == kNoRope) continue;
+        l.rope[f] = ref_is_leaf(r) ? leaf_ref(ref_leaf_id(r) + lb) : r + (int32_t)nb;
+      }
+      l.first += tb;
+      leaves.push_back(l);
+    }
+    for (TriRec t : tt) {
+      t.prim += ob.prim_offset;
+      tris.push_back(t);
+    }
+    ObjXform& x = xf[k];
+    for (int a = 0; a < 3; ++a) {
+      x.loc[a] = ob.location[a];
+      x.root_min[a] = info.root_min[a];
+      x.root_max[a] = info.root_max[a];
+    }
+    x.sin_y = ob.sin_rot_y;
+    x.cos_y = ob.cos_rot_y;
+    x.is_character = ob.is_character;
+    x.root = (int32_t)nb;
+  }
+  DevScene s;
+  s.n_obj = n_obj;
+  s.n_nodes = (int64_t)nodes.size();
+  s.n_leaves = (int64_t)leaves.size();
+  s.n_trirecs = (int64_t)tris.size();
+  auto up = [&](void** dst, const void* src, size_t bytes, const char* what) -> re_status {
+    if (bytes == 0) bytes = 64;  // keep every pointer valid
+    hipError_t e = hipMalloc(dst, bytes);
+    if (e != hipSuccess) return hip_err(ctx, e, what);
+    if (src) {
+      e = hipMemcpyAsync(*dst, src, bytes, hipMemcpyHostToDevice, ctx->stream);
+      if (e != hipSuccess) return hip_err(ctx, e, what);
+    }
+    s.bytes += (int64_t)bytes;
+    return RE_OK;
+  };
+  re_status st;
+  if ((st = up((void**)&s.nodes, nodes.empty() ? nullptr : nodes.data(), nodes.size() * sizeof(NodeEntry), "upload nodes")) ||
+      (st = up((void**)&s.leaves, leaves.empty() ? nullptr : leaves.data(), leaves.size() * sizeof(LeafRec), "upload leaves")) ||
+      (st = up((void**)&s.tris, tris.empty() ? nullptr : tris.data(), tris.size() * sizeof(TriRec), "upload tris")) ||
+      (st = up((void**)&s.objs, xf.data(), xf.size() * sizeof(ObjXform), "upload objects"))) {
+    (void)hipStreamSynchronize(ctx->stream);
+    free_scene(s);
+    return st;
+  }
+  hipError_t e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) {
+    free_scene(s);
+    return hip_err(ctx, e, "re_scene_upload sync");
+  }
+  int32_t id = ctx->next_id++;
+  ctx->scenes[id] = s;
+  *scene_id = id;
+  return RE_OK;
+}
+
+RE_API re_status re_scene_release(re_ctx* ctx, int32_t scene_id) {
+  if (!ctx) return fail(RE_ERR_INVALID, "re_scene_release: null ctx");
+  auto it = ctx->scenes.find(scene_id);
+  if (it == ctx->scenes.end()) return set_err(ctx, RE_ERR_NOT_FOUND, "re_scene_release: unknown scene");
+  (void)hipStreamSynchronize(ctx->stream);
+  free_scene(it->second);
+  ctx->scenes.erase(it);
+  return RE_OK;
+}
+
+RE_API re_status re_scene_bytes(re_ctx* ctx, int32_t scene_id, int64_t* bytes) {
+  if (!ctx || !bytes) return fail(RE_ERR_INVALID, "re_scene_bytes: null argument");
+  auto it = ctx->scenes.find(scene_id);
+  if (it == ctx->scenes.end()) return set_err(ctx, RE_ERR_NOT_FOUND, "re_scene_bytes: unknown scene");
+  *bytes = it->second.bytes;
+  return RE_OK;
+}
+
+RE_API re_status re_scene_info(re_ctx* ctx, int32_t scene_id, int64_t sizes[4]) {
+  if (!ctx || !sizes) return fail(RE_ERR_INVALID, "re_scene_info: null argument");
+  auto it = ctx->scenes.find(scene_id);
+  if (it == ctx->scenes.end()) return set_err(ctx, RE_ERR_NOT_FOUND, "re_scene_info: unknown scene");
+  sizes[0] = it->second.n_nodes;
+  sizes[1] = it->second.n_leaves;
+  sizes[2] = it->second.n_trirecs;
+  sizes[3] = it->second.n_obj;
+  return RE_OK;
+}
+
+RE_API re_status re_scene_export(re_ctx* ctx, int32_t scene_id, void* nodes, void* leaves, void* tris, void* objs) {
+  if (!ctx) return fail(RE_ERR_INVALID, "re_scene_export: null ctx");
+  auto it = ctx->scenes.find(scene_id);
+  if (it == ctx->scenes.end()) return set_err(ctx, RE_ERR_NOT_FOUND, "re_scene_export: unknown scene");
+  const DevScene& s = it->second;
+  HIP_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+  if (nodes && s.n_nodes)
+    HIP_TRY(ctx, hipMemcpyAsync(nodes, s.nodes, s.n_nodes * sizeof(NodeEntry), hipMemcpyDeviceToHost, ctx->stream), "export nodes");
+  if (leaves && s.n_leaves)
+    HIP_TRY(ctx, hipMemcpyAsync(leaves, s.leaves, s.n_leaves * sizeof(LeafRec), hipMemcpyDeviceToHost, ctx->stream), "export leaves");
+  if (tris && s.n_trirecs)
+    HIP_TRY(ctx, hipMemcpyAsync(tris, s.tris, s.n_trirecs * sizeof(TriRec), hipMemcpyDeviceToHost, ctx->stream), "export tris");
+  if (objs)
+    HIP_TRY(ctx, hipMemcpyAsync(objs, s.objs, s.n_obj * sizeof(ObjXform), hipMemcpyDeviceToHost, ctx->stream), "export objs");
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream), "export sync");
+  return RE_OK;
+}
+
+RE_API void re_trace_default_params(re_trace_params* p) {
+  if (!p) return;
+  p->max_leaf_visits = 50;
+  p->one_sided = 1;
+  p->secondary = 0;
+  p->block = 0;
+}
+
+RE_API re_status re_kernel_times(re_ctx* ctx, float* ms, int32_t max, int32_t* count) {
+  if (!ctx || !count || max < 0 || (max > 0 && !ms)) return fail(RE_ERR_INVALID, "re_kernel_times: bad arguments");
+  int64_t avail = ctx->n_launch < re_ctx::kRing ? ctx->n_launch : re_ctx::kRing;
+  int32_t k = (int32_t)(avail < max ? avail : max);
+  for (int32_t i = 0; i < k; ++i) {
+    int64_t idx = (ctx->n_launch - k + i) % re_ctx::kRing;
+    HIP_TRY(ctx, hipEventSynchronize(ctx->ring[2 * idx + 1]), "re_kernel_times sync");
+    HIP_TRY(ctx, hipEventElapsedTime(&ms[i], ctx->ring[2 * idx], ctx->ring[2 * idx + 1]), "re_kernel_times");
+  }
+  *count = k;
+  return RE_OK;
+}
+
+RE_API re_status re_last_kernel_ms(re_ctx* ctx, float* ms) {
+  if (!ctx || !ms) return fail(RE_ERR_INVALID, "re_last_kernel_ms: null argument");
+  if (ctx->n_launch == 0) return set_err(ctx, RE_ERR_INVALID, "re_last_kernel_ms: no launch yet");
+  int32_t k = 0;
+  return re_kernel_times(ctx, ms, 1, &k);
+}
+
+}  // extern "C"
+
+namespace re {
+
+// Shared body of the closest-hit entry points.  counts (host pointer, may be
+// null) receives the summed work counters.
+re_status trace_common(re_ctx* ctx, int32_t scene_id, const float* ray_o, const float* ray_d, int64_t n,
+                       const re_trace_params* params, int32_t* prim_id, float* t, int32_t* obj_id,
+                       uint32_t flags, TraceCounts* counts) {
+  if (!ctx) return fail(RE_ERR_INVALID, "re_trace_closest: null ctx");
+  if (n < 0 || (n > 0 && (!ray_o || !ray_d || !prim_id || !t)))
+    return set_err(ctx, RE_ERR_INVALID, "re_trace_closest: bad arguments");
+  auto it = ctx->scenes.find(scene_id);
+  if (it == ctx->scenes.end()) return set_err(ctx, RE_ERR_NOT_FOUND, "re_trace_closest: unknown scene");
+  re_trace_params p;
+  re_trace_default_params(&p);
+  if (params) p = *params;
+  if (!(p.block == 0 || p.block == 64 || p.block == 128 || p.block == 256))
+    return set_err(ctx, RE_ERR_INVALID, "re_trace_closest: block must be 0, 64, 128 or 256");
+  if (n == 0) return RE_OK;
+  HIP_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+  TraceLaunch L;
+  L.scene = &it->second;
+  L.n = n;
+  L.cap = p.max_leaf_visits;
+  L.one_sided = p.one_sided;
+  L.secondary = p.secondary;
+  L.block = p.block;
+  L.counts = nullptr;
+  const bool dev = (flags & RE_DEVICE_PTRS) != 0;
+  const size_t ray_bytes = (size_t)n * 12;
+  char* base = nullptr;
+  if (!dev) {
+    size_t need = 2 * ray_bytes + (size_t)n * 12 + 256 + 64;
+    re_status st = ensure_stage(ctx, need);
+    if (st) return st;
+    base = (char*)ctx->stage;
+    float* d_o = (float*)base;
+    float* d_d = (float*)(base + ray_bytes);
+    int32_t* d_p = (int32_t*)(base + 2 * ray_bytes);
+    float* d_t = (float*)(base + 2 * ray_bytes + (size_t)n * 4);
+    int32_t* d_ob = (int32_t*)(base + 2 * ray_bytes + (size_t)n * 8);
+    HIP_TRY(ctx, hipMemcpyAsync(d_o, ray_o, ray_bytes, hipMemcpyHostToDevice, ctx->stream), "copy ray_o");
+    HIP_TRY(ctx, hipMemcpyAsync(d_d, ray_d, ray_bytes, hipMemcpyHostToDevice, ctx->stream), "copy ray_d");
+    L.ray_o = d_o;
+    L.ray_d = d_d;
+    L.prim = d_p;
+    L.t = d_t;
+    L.obj = obj_id ? d_ob : nullptr;
+  } else {
+    L.ray_o = ray_o;
+    L.ray_d = ray_d;
+    L.prim = prim_id;
+    L.t = t;
+    L.obj = obj_id;
+  }
+  TraceCounts* d_counts = nullptr;
+  if (counts) {
+    d_counts = (TraceCounts*)ctx->counts_buf;
+    HIP_TRY(ctx, hipMemsetAsync(d_counts, 0, sizeof(TraceCounts), ctx->stream), "counts clear");
+    L.counts = d_counts;
+  }
+  HIP_TRY(ctx, ctx->launch_begin(), "event");
+  HIP_TRY(ctx, launch_trace_closest(L, ctx->stream), "trace_closest launch");
+  HIP_TRY(ctx, ctx->launch_end(), "event");
+  if (!dev) {
+    HIP_TRY(ctx, hipMemcpyAsync(prim_id, L.prim, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream), "copy prim");
+    HIP_TRY(ctx, hipMemcpyAsync(t, L.t, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream), "copy t");
+    if (obj_id) HIP_TRY(ctx, hipMemcpyAsync(obj_id, L.obj, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream), "copy obj");
+  }
+  if (counts) {
+    HIP_TRY(ctx, hipMemcpyAsync(counts, d_counts, sizeof(TraceCounts), hipMemcpyDeviceToHost, ctx->stream), "copy counts");
+  }
+  if (!(flags & RE_ASYNC) || !dev || counts) HIP_TRY(ctx, hipStreamSynchronize(ctx->stream), "trace sync");
+  return RE_OK;
+}
+
+}  // namespace re
+
+extern "C" {
+
+RE_API re_status re_trace_closest(re_ctx* ctx, int32_t scene_id, const float* ray_o, const float* ray_d, int64_t n,
+                                  const re_trace_params* params, int32_t* prim_id, float* t, int32_t* obj_id,
+                                  uint32_t flags) {
+  return trace_common(ctx, scene_id, ray_o, ray_d, n, params, prim_id, t, obj_id, flags, nullptr);
+}
+
+RE_API re_status re_trace_counts(re_ctx* ctx, int32_t scene_id, const float* ray_o, const float* ray_d, int64_t n,
+                                 const re_trace_params* params, int32_t* prim_id, float* t, uint64_t counts[4],
+                                 uint32_t flags) {
+  if (!counts) return set_err(ctx, RE_ERR_INVALID, "re_trace_counts: counts is null");
+  TraceCounts c{0, 0, 0, 0};
+  re_status st = trace_common(ctx, scene_id, ray_o, ray_d, n, params, prim_id, t, nullptr, flags, &c);
+  counts[0] = c.entries;
+  counts[1] = c.leaves;
+  counts[2] = c.tris;
+  counts[3] = c.rays_hit;
+  return st;
+}
+
+}  // extern "C"

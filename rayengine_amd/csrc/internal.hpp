@@ -1,0 +1,80 @@
+// Internal declarations shared between the C-ABI layer (capi.cpp) and the
+// kernel launchers (kernels/*.hip).  Not part of the public interface.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/rayengine.h"
+#include "re_math.hpp"
+
+struct re_kdtree;
+
+namespace re {
+
+const std::vector<NodeEntry>& kd_nodes(const re_kdtree* t);
+const std::vector<LeafRec>& kd_leaves(const re_kdtree* t);
+const std::vector<TriRec>& kd_tris(const re_kdtree* t);
+
+// A scene resident in HBM: the concatenated trees of its objects.
+struct DevScene {
+  NodeEntry* nodes = nullptr;
+  LeafRec* leaves = nullptr;
+  TriRec* tris = nullptr;
+  ObjXform* objs = nullptr;
+  int32_t n_obj = 0;
+  int64_t n_nodes = 0, n_leaves = 0, n_trirecs = 0;
+  int64_t bytes = 0;
+};
+
+// Per-ray work counters of a traversal: node entries read, leaf records read,
+// triangle records tested.
+struct TraceCounts {
+  unsigned long long entries, leaves, tris, rays_hit;
+};
+
+struct TraceLaunch {
+  const DevScene* scene;
+  const float* ray_o;
+  const float* ray_d;
+  int64_t n;
+  int32_t cap;
+  int32_t one_sided;
+  int32_t secondary;
+  int32_t block;
+  int32_t* prim;
+  float* t;
+  int32_t* obj;
+  TraceCounts* counts;  // device pointer or null
+};
+
+hipError_t launch_trace_closest(const TraceLaunch& L, hipStream_t s);
+
+}  // namespace re
+
+struct re_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  // A pair of HIP events is recorded around every kernel launch into a ring,
+  // so the durations of asynchronous launches can be read back later.
+  static constexpr int kRing = 256;
+  hipEvent_t ring[2 * kRing] = {};
+  int64_t n_launch = 0;
+  int32_t next_id = 1;
+  std::map<int32_t, re::DevScene> scenes;
+  // Staging for host-pointer calls, grown on demand.
+  void* counts_buf = nullptr;  // 256 bytes for work counters
+  void* stage = nullptr;
+  size_t stage_bytes = 0;
+  std::string err;
+
+  hipError_t launch_begin() { return hipEventRecord(ring[2 * (n_launch % kRing)], stream); }
+  hipError_t launch_end() {
+    hipError_t e = hipEventRecord(ring[2 * (n_launch % kRing) + 1], stream);
+    ++n_launch;
+    return e;
+  }
+};

@@ -1,0 +1,43 @@
+"""Seeded synthetic workloads of the ray-query path.
+
+* small_scene      - icosphere (subdivided 4x, 5120 triangles) + 1024 random
+                     triangles (centroids U[-2,2]^3, edges U[0.02,0.2], seed 0),
+                     256x256 pinhole rays from (0,0,5) towards the origin, FOV 60.
+* cluster_scene    - 2^20 triangles in 64 Gaussian clusters (sigma 0.3, centres
+                     U[-10,10]^3, edge lengths LogNormal(-3, 0.5), seed 1);
+                     1920x1080 primary rays from (0,0,30) towards the origin plus
+                     one cosine-weighted diffuse bounce ray per primary hit.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import Camera, KDTree, bounce_rays, camera_rays, cluster_soup, icosphere, random_tris
+
+SMALL_CAMERA = Camera(pos=(0.0, 0.0, 5.0), view=(0.0, 0.0, -1.0))
+CLUSTER_CAMERA = Camera(pos=(0.0, 0.0, 30.0), view=(0.0, 0.0, -1.0))
+CLUSTER_TRIS = 1 << 20
+BOUNCE_SEED = 1
+
+
+def small_soup() -> np.ndarray:
+    return np.concatenate([icosphere(4, 1.0), random_tris(0, 1024, -2.0, 2.0, 0.02, 0.2)])
+
+
+def small_scene(width: int = 256, height: int = 256):
+    """(tri_v, tree, ray_o, ray_d) of the small icosphere workload."""
+    tri_v = small_soup()
+    tree = KDTree(tri_v, max_leaf_tris=8, max_depth=24)
+    o, d = camera_rays(SMALL_CAMERA, width, height)
+    return tri_v, tree, o, d
+
+
+def cluster_soup_default(n: int = CLUSTER_TRIS, seed: int = 1) -> np.ndarray:
+    return cluster_soup(seed, n, 64, -10.0, 10.0, 0.3, -3.0, 0.5)
+
+
+def cluster_rays(tri_v, primary_prim, primary_t, width=1920, height=1080):
+    """Primary rays plus one bounce ray per primary hit, concatenated."""
+    o, d = camera_rays(CLUSTER_CAMERA, width, height)
+    bo, bd = bounce_rays(tri_v, o, d, primary_prim, primary_t, BOUNCE_SEED)
+    return np.concatenate([o, bo]), np.concatenate([d, bd])

@@ -1,0 +1,175 @@
+"""GPU parity of the rope kd-tree closest-hit kernel against the CPU oracle.
+
+Bar: primitive ids equal and hit t bit-identical (compared as uint32) on the
+same seeded inputs.  Calls go through the C ABI (librayengine.so)."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+import rayengine_amd as re
+from rayengine_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _parity(p, t, po, to):
+    assert np.array_equal(p, po), f"{int((p != po).sum())} prim ids differ"
+    assert np.array_equal(t.view(np.uint32), to.view(np.uint32)), f"{int((t.view(np.uint32) != to.view(np.uint32)).sum())} t differ"
+
+
+@pytest.mark.parametrize("cap", [50, 1, 4096])
+def test_small_scene_bit_exact(gpu_ctx, small, cap):
+    say(f"small scene, cap {cap}")
+    tri_v, tree, o, d = small
+    scene = gpu_ctx.upload_scene([re.SceneObject(tree)])
+    p, t = scene.trace_closest(o, d, max_leaf_visits=cap)
+    po, to = oracle.trace(*tree.export(), tree.identity_object(), o, d, cap=cap)
+    _parity(p, t, po, to)
+    assert (p >= 0).sum() > 8000
+    scene.release()
+
+
+def test_small_scene_golden(gpu_ctx, small):
+    from pathlib import Path
+    g = np.load(Path(__file__).resolve().parent / "golden" / "small_scene.npz")
+    tri_v, tree, o, d = small
+    scene = gpu_ctx.upload_scene([re.SceneObject(tree)])
+    p, t = scene.trace_closest(o, d)
+    assert np.array_equal(p, g["prim"]) and np.array_equal(t.view(np.uint32), g["t_bits"])
+
+
+def test_two_sided_and_block_shapes(gpu_ctx, small):
+    tri_v, tree, o, d = small
+    scene = gpu_ctx.upload_scene([re.SceneObject(tree)])
+    po, to = oracle.trace(*tree.export(), tree.identity_object(), o, d, two_sided=True)
+    for block in (64, 128, 256):
+        say(f"two-sided, block {block}")
+        p, t = scene.trace_closest(o, d, one_sided=False, block=block)
+        _parity(p, t, po, to)
+
+
+def test_counters_match_oracle(gpu_ctx, small):
+    tri_v, tree, o, d = small
+    scene = gpu_ctx.upload_scene([re.SceneObject(tree)])
+    p, t, cnt = scene.trace_counts(o, d)
+    po, to, co = oracle.trace(*tree.export(), tree.identity_object(), o, d, counts=True)
+    _parity(p, t, po, to)
+    assert cnt[:3] == [int(x) for x in co.sum(0)]
+    assert cnt[3] == int((po >= 0).sum())
+
+
+def test_multi_object_transforms(gpu_ctx):
+    """Several objects with Y rotation, translation, a character object, and
+    both primary and secondary transforms (ray_functions.cuh:238-267,376-396)."""
+    a = re.icosphere(3, 1.0)
+    b = re.random_tris(5, 3000, -1.5, 1.5, 0.05, 0.3)
+    ta, tb = re.KDTree(a), re.KDTree(b)
+    objs = [re.SceneObject(ta, location=(0.5, -0.2, 0.1), rotation_y=0.7),
+            re.SceneObject(tb, location=(-0.3, 0.4, 0.0), rotation_y=-1.3, prim_offset=len(a)),
+            re.SceneObject(ta, location=(0.2, 0.1, -0.4), rotation_y=2.1, is_character=True,
+                           prim_offset=len(a) + len(b))]
+    scene = gpu_ctx.upload_scene(objs)
+    nodes, leaves, tris, obs = scene.export()
+    o, d = re.camera_rays(re.Camera(pos=(0.3, 0.5, 6.0), view=(0.0, -0.1, -1.0)), 160, 120)
+    for secondary in (False, True):
+        say(f"multi-object, secondary={secondary}")
+        p, t, ob = scene.trace_closest(o, d, with_obj=True, secondary=secondary)
+        po, to = oracle.trace(nodes, leaves, tris, obs, o, d, secondary=secondary)
+        _parity(p, t, po, to)
+        assert set(np.unique(ob[p >= 0])) <= {0, 1, 2}
+    assert (p >= 0).sum() > 1000
+
+
+def test_degenerate_rays(gpu_ctx, small):
+    """NaN, zero and axis-aligned directions, origins on leaf planes."""
+    tri_v, tree, _, _ = small
+    scene = gpu_ctx.upload_scene([re.SceneObject(tree)])
+    nodes, leaves, tris = tree.export()
+    rng = np.random.default_rng(3)
+    n = 4096
+    o = rng.uniform(-3, 3, (n, 3)).astype(np.float32)
+    d = np.zeros((n, 3), np.float32)
+    ax = rng.integers(0, 3, n)
+    d[np.arange(n), ax] = rng.choice([-1.0, 1.0], n)
+    d[:64] = 0.0
+    d[64:128, 0] = np.nan
+    o[128:512, 0] = leaves["bmin"][rng.integers(0, len(leaves), 384), 0]
+    d[512:1024] = -0.0
+    d[512:1024, 2] = -1.0
+    p, t = scene.trace_closest(o, d)
+    po, to = oracle.trace(nodes, leaves, tris, tree.identity_object(), o, d)
+    _parity(p, t, po, to)
+
+
+def test_empty_batch_and_empty_scene(gpu_ctx):
+    tree = re.KDTree(np.zeros((0, 9), np.float32))
+    scene = gpu_ctx.upload_scene([re.SceneObject(tree)])
+    p, t = scene.trace_closest(np.zeros((0, 3), np.float32), np.zeros((0, 3), np.float32))
+    assert p.size == 0
+    o, d = re.camera_rays(W.SMALL_CAMERA, 8, 8)
+    p, t = scene.trace_closest(o, d)
+    assert (p == -1).all() and (t == np.float32(1e20)).all()
+
+
+def test_bad_arguments_report_status(gpu_ctx, small):
+    tri_v, tree, o, d = small
+    scene = gpu_ctx.upload_scene([re.SceneObject(tree)])
+    with pytest.raises(re.ReError):
+        scene.trace_closest(o, d, block=96)
+    bogus = re.Scene(gpu_ctx, 987654, [])
+    with pytest.raises(re.ReError):
+        bogus.trace_closest(o[:4], d[:4])
+
+
+def test_device_pointer_path(gpu_ctx, small):
+    import torch
+    tri_v, tree, o, d = small
+    scene = gpu_ctx.upload_scene([re.SceneObject(tree)])
+    to_ = torch.from_numpy(o).cuda()
+    td = torch.from_numpy(d).cuda()
+    n = o.shape[0]
+    tp = torch.empty(n, dtype=torch.int32, device="cuda")
+    tt = torch.empty(n, dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    scene.trace_closest_device(to_.data_ptr(), td.data_ptr(), n, tp.data_ptr(), tt.data_ptr())
+    p, t = scene.trace_closest(o, d)
+    assert np.array_equal(tp.cpu().numpy(), p) and np.array_equal(tt.cpu().numpy().view(np.uint32), t.view(np.uint32))
+
+
+@pytest.fixture(scope="module")
+def cluster():
+    say("building the 2^20-triangle cluster soup")
+    tri_v = W.cluster_soup_default()
+    tree = re.KDTree(tri_v)
+    return tri_v, tree
+
+
+def test_cluster_scene_subset_bit_exact(gpu_ctx, cluster):
+    """Full-size cluster workload on the GPU; a seeded subset of 65536 rays
+    (primary and bounce) checked against the oracle."""
+    tri_v, tree = cluster
+    scene = gpu_ctx.upload_scene([re.SceneObject(tree)])
+    o, d = re.camera_rays(W.CLUSTER_CAMERA, 1920, 1080)
+    say("cluster primary")
+    p, t = scene.trace_closest(o, d)
+    ro, rd = W.cluster_rays(tri_v, p, t)
+    say(f"cluster all rays: {len(ro)}")
+    pa, ta = scene.trace_closest(ro, rd)
+    assert np.array_equal(pa[:len(o)], p) and np.array_equal(ta[:len(o)].view(np.uint32), t.view(np.uint32))
+    rng = np.random.default_rng(11)
+    sel = np.concatenate([rng.choice(len(o), 32768, replace=False),
+                          len(o) + rng.choice(len(ro) - len(o), 32768, replace=False)])
+    nodes, leaves, tris = tree.export()
+    po, to = oracle.trace(nodes, leaves, tris, tree.identity_object(), ro[sel], rd[sel])
+    _parity(pa[sel], ta[sel], po, to)
+    # size-independent properties over all rays: hits are in front, misses are exact
+    hit = pa >= 0
+    assert (ta[hit] > 1e-4).all() and (ta[~hit] == np.float32(1e20)).all()
+    assert hit[len(o):].mean() > 0.3
