@@ -197,6 +197,63 @@ RE_API re_status re_trace_counts(re_ctx* ctx, int32_t scene_id, const float* ray
                                  const float* ray_d, int64_t n, const re_trace_params* params,
                                  int32_t* prim_id, float* t, uint64_t counts[4], uint32_t flags);
 
+/* ------------------------------------------------------ terrain volume */
+
+/* Doubled permutation table (512 entries) of RPerlinNoise(seed)
+ * (RayEngine/src/World/PerlinNoise.h:16-52; reference default seed 2016). */
+RE_API re_status re_perlin_table(uint32_t seed, uint32_t* perm);
+
+/* Terrain field f(p) = (p.y - y_offset) - sum_{o<octaves} gain^o *
+ * noise(p * base_freq * lacunarity^o) sampled at voxel (x,y,z) -> p =
+ * (x,y,z) * spacing.  noise is eval (RayEngine/src/cuda-src/sdf_functions.cuh:83-135)
+ * with hash P[P[P[x]+y]+z]. */
+typedef struct re_noise_params {
+  int32_t dim[3];     /* default 256^3 */
+  float spacing[3];   /* default 1/256 (extent 1) */
+  uint32_t seed;      /* permutation seed, default 2016 */
+  int32_t octaves;    /* default 5 */
+  float lacunarity;   /* default 2.0 */
+  float gain;         /* default 0.5 */
+  float base_freq;    /* default 4.0 (= 4 / extent) */
+  float y_offset;     /* default 0.4 */
+} re_noise_params;
+
+RE_API void re_noise_default_params(re_noise_params* p);
+/* Generates a volume on the device (bricked 8^3, Morton order inside a
+ * brick).  Replaces generate_noise / cuda_update_chunk_gen
+ * (RayEngine/src/cuda-src/renderer.cu:105-126,812-836,982-1005). */
+RE_API re_status re_noise_generate(re_ctx* ctx, const re_noise_params* params, int32_t* vol_id);
+/* Uploads a linear x-fastest grid (index x + dim.x*(y + dim.y*z)).
+ * Replaces initialize_volume_render's grid upload
+ * (RayEngine/src/cuda-src/sphere_tracing.cuh:617-755). */
+RE_API re_status re_volume_upload(re_ctx* ctx, const int32_t* dim, const float* spacing,
+                                  const float* linear, int32_t* vol_id);
+/* Reads a volume back as a linear x-fastest grid (save_map's D2H copy,
+ * RayEngine/src/cuda-src/renderer.cu:838-870). */
+RE_API re_status re_volume_read(re_ctx* ctx, int32_t vol_id, float* linear);
+RE_API re_status re_volume_info(re_ctx* ctx, int32_t vol_id, int32_t* dim, float* spacing,
+                                int64_t* bytes);
+RE_API re_status re_volume_release(re_ctx* ctx, int32_t vol_id);
+
+typedef struct re_sdf_params {
+  int32_t max_steps; /* default 256 */
+  float eps;         /* hit when d <= eps * t; default 1e-3 (renderer.cu:210-211) */
+  float step_scale;  /* t += d * step_scale; default 0.8 */
+  int32_t step_rule; /* 0: step_scale; 1: reference clamp(0.6 t, 0.45, 0.8) */
+  int32_t block;     /* threads per block, 0 = default */
+} re_sdf_params;
+
+RE_API void re_sdf_default_params(re_sdf_params* p);
+/* Sphere traces n rays through the volume (clipped to its box [0, dim *
+ * spacing]), trilinear samples.  t = post-step distance of the hit (1e20 on
+ * a miss), hit = 0/1; steps (nullable) = samples taken; normal (nullable, 3
+ * floats per ray) = forward-difference normal at the hit.  Replaces the
+ * trace part of render_sphere_trace / single_ray_sphere_trace
+ * (RayEngine/src/cuda-src/renderer.cu:142-235, sphere_tracing.cuh:182-216). */
+RE_API re_status re_sdf_trace(re_ctx* ctx, int32_t vol_id, const re_sdf_params* params,
+                              const float* ray_o, const float* ray_d, int64_t n, float* t,
+                              uint8_t* hit, int32_t* steps, float* normal, uint32_t flags);
+
 /* Elapsed milliseconds of the last kernel launch of the context, measured
  * with HIP events recorded around the launch on its stream (waits for it).
  * Replaces the reference's FPS counter (RayEngine/src/Engine/Window.cpp:46-56). */

@@ -28,7 +28,22 @@ _SIGS = {
     "orc_neighbour_face": (C.c_int, [_P, _P, _P]),
     "orc_perlin_table": (None, [C.c_uint32, _P, _P]),
     "orc_mt19937_nth": (C.c_uint32, [C.c_uint32, C.c_int64]),
+    "orc_grad_dot": (C.c_float, [C.c_uint32, C.c_float, C.c_float, C.c_float]),
+    "orc_noise_eval": (C.c_float, [_P, C.c_float, C.c_float, C.c_float]),
+    "orc_noise_eval_f64": (C.c_double, [_P, C.c_double, C.c_double, C.c_double]),
+    "orc_noise_fill": (None, [_P, _P, _P, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, _P, C.c_int]),
+    "orc_noise_fill_f64": (None, [_P, _P, _P, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, _P, C.c_int]),
+    "orc_trilinear": (C.c_float, [_P, _P, _P, C.c_float, C.c_float, C.c_float]),
+    "orc_sdf_trace": (None, [_P, _P, _P, _P, _P, C.c_int64, C.c_int, C.c_float, C.c_float, C.c_int, _P, _P, _P, _P,
+                             C.c_int]),
+    "orc_sdf_trace_f64": (None, [_P, _P, _P, _P, _P, C.c_int64, C.c_int, C.c_float, C.c_float, C.c_int, _P, _P,
+                                 _P, C.c_int]),
 }
+
+# Terrain defaults: 256^3 over extent 1, 5 octaves, lacunarity 2, gain 0.5,
+# base frequency 4, surface offset 0.4, permutation seed 2016.
+TERRAIN = dict(dim=(256, 256, 256), spacing=(1 / 256, 1 / 256, 1 / 256), octaves=5, lacunarity=2.0, gain=0.5,
+               base_freq=4.0, y_offset=0.4)
 
 _lib = None
 
@@ -120,3 +135,82 @@ def perlin_table(seed=2016):
 
 def mt19937_nth(seed, k):
     return int(lib().orc_mt19937_nth(seed, k))
+
+
+# ------------------------------------------------------------------ terrain
+
+def grad_dot(h, x, y, z):
+    return lib().orc_grad_dot(h, x, y, z)
+
+
+def noise_eval(perm, x, y, z):
+    perm = np.ascontiguousarray(perm, np.uint32)
+    return np.float32(lib().orc_noise_eval(_p(perm), x, y, z))
+
+
+def noise_eval_f64(perm, x, y, z):
+    perm = np.ascontiguousarray(perm, np.uint32)
+    return lib().orc_noise_eval_f64(_p(perm), x, y, z)
+
+
+def _terrain_args(perm, kw):
+    p = dict(TERRAIN)
+    p.update(kw)
+    dim = np.array(p["dim"], np.int32)
+    sp = np.array(p["spacing"], np.float32)
+    perm = np.ascontiguousarray(perm, np.uint32)
+    return perm, dim, sp, p
+
+
+def noise_fill(perm, threads=0, **kw):
+    """FP32 terrain grid as (dz, dy, dx)."""
+    perm, dim, sp, p = _terrain_args(perm, kw)
+    out = np.empty((dim[2], dim[1], dim[0]), np.float32)
+    lib().orc_noise_fill(_p(perm), _p(dim), _p(sp), p["octaves"], p["lacunarity"], p["gain"], p["base_freq"],
+                         p["y_offset"], _p(out), threads)
+    return out
+
+
+def noise_fill_f64(perm, threads=0, **kw):
+    perm, dim, sp, p = _terrain_args(perm, kw)
+    out = np.empty((dim[2], dim[1], dim[0]), np.float64)
+    lib().orc_noise_fill_f64(_p(perm), _p(dim), _p(sp), p["octaves"], p["lacunarity"], p["gain"], p["base_freq"],
+                             p["y_offset"], _p(out), threads)
+    return out
+
+
+def trilinear(grid, spacing, x, y, z):
+    g = np.ascontiguousarray(grid, np.float32)
+    dim = np.array([g.shape[2], g.shape[1], g.shape[0]], np.int32)
+    sp = np.array(spacing, np.float32)
+    return np.float32(lib().orc_trilinear(_p(g), _p(dim), _p(sp), x, y, z))
+
+
+def sdf_trace(grid, spacing, o, d, max_steps=256, eps=1e-3, step_scale=0.8, step_rule=0, normals=False, threads=0):
+    """FP32 sphere trace: (t, hit, steps[, normal])."""
+    g = np.ascontiguousarray(grid, np.float32)
+    dim = np.array([g.shape[2], g.shape[1], g.shape[0]], np.int32)
+    sp = np.array(spacing, np.float32)
+    o, d = _f(o), _f(d)
+    n = o.size // 3
+    t = np.empty(n, np.float32)
+    hit = np.empty(n, np.uint8)
+    steps = np.empty(n, np.int32)
+    nrm = np.empty((n, 3), np.float32) if normals else None
+    lib().orc_sdf_trace(_p(g), _p(dim), _p(sp), _p(o), _p(d), n, max_steps, eps, step_scale, step_rule, _p(t),
+                        _p(hit), _p(steps), _p(nrm), threads)
+    return (t, hit, steps, nrm) if normals else (t, hit, steps)
+
+
+def sdf_trace_f64(grid64, spacing, o, d, max_steps=256, eps=1e-3, step_scale=0.8, step_rule=0, threads=0):
+    g = np.ascontiguousarray(grid64, np.float64)
+    dim = np.array([g.shape[2], g.shape[1], g.shape[0]], np.int32)
+    sp = np.array(spacing, np.float32)
+    o, d = _f(o), _f(d)
+    n = o.size // 3
+    t = np.empty(n, np.float64)
+    hit = np.empty(n, np.uint8)
+    steps = np.empty(n, np.int32)
+    lib().orc_sdf_trace_f64(_p(g), _p(dim), _p(sp), _p(o), _p(d), n, max_steps, eps, step_scale, step_rule, _p(t),
+                            _p(hit), _p(steps), threads)
+    return t, hit, steps

@@ -24,7 +24,8 @@ import numpy as np
 from . import _lib
 from ._lib import (LEAF_DT, NODE_DT, OBJ_DT, RE_ASYNC, RE_DEVICE_PTRS, TRI_DT, ReError, check, lib, ptr)
 
-__all__ = ["KDTree", "SceneObject", "Context", "Scene", "Camera", "ReError", "icosphere", "random_tris",
+__all__ = ["KDTree", "SceneObject", "Context", "Scene", "Volume", "Camera", "ReError", "perlin_table",
+           "noise_params", "sdf_params", "icosphere", "random_tris",
            "cluster_soup", "camera_rays", "bounce_rays", "NODE_DT", "LEAF_DT", "TRI_DT", "OBJ_DT",
            "RE_DEVICE_PTRS", "RE_ASYNC", "K_EPSILON", "K_INFINITY"]
 
@@ -195,6 +196,25 @@ class Context:
         check(lib().re_kernel_times(self._h, buf.ctypes.data_as(_lib._F), max_n, C.byref(k)), self._h)
         return buf[:k.value].copy()
 
+    # ------------------------------------------------------------ terrain
+
+    def noise_volume(self, **kw) -> "Volume":
+        """Generates the fBm terrain volume on the device (re_noise_generate).
+        Keywords: dim, spacing, seed, octaves, lacunarity, gain, base_freq, y_offset."""
+        p = noise_params(**kw)
+        vid = C.c_int32()
+        check(lib().re_noise_generate(self._h, C.byref(p), C.byref(vid)), self._h)
+        return Volume(self, vid.value)
+
+    def upload_volume(self, grid: np.ndarray, spacing) -> "Volume":
+        """Uploads a (dz, dy, dx) float32 grid (x fastest) with the given spacing."""
+        g = np.ascontiguousarray(grid, np.float32)
+        dim = np.array([g.shape[2], g.shape[1], g.shape[0]], np.int32)
+        sp = np.array(spacing, np.float32).reshape(3)
+        vid = C.c_int32()
+        check(lib().re_volume_upload(self._h, ptr(dim), ptr(sp), ptr(g), C.byref(vid)), self._h)
+        return Volume(self, vid.value)
+
     def upload_scene(self, objects: list[SceneObject]) -> "Scene":
         arr = (_lib.re_object * len(objects))()
         for i, ob in enumerate(objects):
@@ -219,6 +239,76 @@ class Context:
             self.close()
         except Exception:
             pass
+
+
+def perlin_table(seed: int = 2016) -> np.ndarray:
+    """Doubled permutation table of RPerlinNoise(seed) (512 uint32)."""
+    out = np.zeros(512, np.uint32)
+    check(lib().re_perlin_table(seed, ptr(out)))
+    return out
+
+
+def noise_params(**kw) -> _lib.re_noise_params:
+    p = _lib.re_noise_params()
+    lib().re_noise_default_params(C.byref(p))
+    for k, v in kw.items():
+        if k == "dim":
+            p.dim[:] = [int(x) for x in (v if np.ndim(v) else (v, v, v))]
+        elif k == "spacing":
+            p.spacing[:] = [float(x) for x in (v if np.ndim(v) else (v, v, v))]
+        else:
+            setattr(p, k, v)
+    return p
+
+
+def sdf_params(**kw) -> _lib.re_sdf_params:
+    p = _lib.re_sdf_params()
+    lib().re_sdf_default_params(C.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+class Volume:
+    """A device-resident terrain volume (bricked FP32)."""
+
+    def __init__(self, ctx: Context, vid: int):
+        self.ctx, self.id = ctx, vid
+        dim = np.zeros(3, np.int32)
+        sp = np.zeros(3, np.float32)
+        nb = C.c_int64()
+        check(lib().re_volume_info(ctx.handle, vid, ptr(dim), ptr(sp), C.byref(nb)), ctx.handle)
+        self.dim, self.spacing, self.bytes = dim, sp, nb.value
+
+    def read(self) -> np.ndarray:
+        """The grid as a (dz, dy, dx) float32 array."""
+        out = np.empty((self.dim[2], self.dim[1], self.dim[0]), np.float32)
+        check(lib().re_volume_read(self.ctx.handle, self.id, ptr(out)), self.ctx.handle)
+        return out
+
+    def trace(self, o, d, normals=False, **params):
+        """Sphere trace: (t, hit, steps[, normal])."""
+        o, d = _f32c(o, 3), _f32c(d, 3)
+        n = o.size // 3
+        t = np.empty(n, np.float32)
+        hit = np.empty(n, np.uint8)
+        steps = np.empty(n, np.int32)
+        nrm = np.empty((n, 3), np.float32) if normals else None
+        p = sdf_params(**params)
+        check(lib().re_sdf_trace(self.ctx.handle, self.id, C.byref(p), ptr(o), ptr(d), n, ptr(t), ptr(hit),
+                                 ptr(steps), ptr(nrm), 0), self.ctx.handle)
+        return (t, hit, steps, nrm) if normals else (t, hit, steps)
+
+    def trace_device(self, o_ptr, d_ptr, n, t_ptr, hit_ptr, steps_ptr=None, nrm_ptr=None, sync=True, **params):
+        p = sdf_params(**params)
+        flags = RE_DEVICE_PTRS | (0 if sync else RE_ASYNC)
+        check(lib().re_sdf_trace(self.ctx.handle, self.id, C.byref(p), o_ptr, d_ptr, n, t_ptr, hit_ptr, steps_ptr,
+                                 nrm_ptr, flags), self.ctx.handle)
+
+    def release(self):
+        if self.id is not None:
+            check(lib().re_volume_release(self.ctx.handle, self.id), self.ctx.handle)
+            self.id = None
 
 
 def _trace_params(max_leaf_visits=50, one_sided=True, secondary=False, block=0):

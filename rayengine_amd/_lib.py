@@ -63,6 +63,16 @@ class re_trace_params(C.Structure):
                 ("block", C.c_int32)]
 
 
+class re_noise_params(C.Structure):
+    _fields_ = [("dim", C.c_int32 * 3), ("spacing", C.c_float * 3), ("seed", C.c_uint32), ("octaves", C.c_int32),
+                ("lacunarity", C.c_float), ("gain", C.c_float), ("base_freq", C.c_float), ("y_offset", C.c_float)]
+
+
+class re_sdf_params(C.Structure):
+    _fields_ = [("max_steps", C.c_int32), ("eps", C.c_float), ("step_scale", C.c_float), ("step_rule", C.c_int32),
+                ("block", C.c_int32)]
+
+
 _P = C.c_void_p
 _F = C.POINTER(C.c_float)
 _I32 = C.POINTER(C.c_int32)
@@ -96,6 +106,16 @@ SIGNATURES = {
                                    C.c_uint32]),
     "re_trace_counts": (C.c_int, [_P, C.c_int32, _P, _P, C.c_int64, C.POINTER(re_trace_params), _P, _P, _U64,
                                   C.c_uint32]),
+    "re_perlin_table": (C.c_int, [C.c_uint32, _P]),
+    "re_noise_default_params": (None, [C.POINTER(re_noise_params)]),
+    "re_noise_generate": (C.c_int, [_P, C.POINTER(re_noise_params), _I32]),
+    "re_volume_upload": (C.c_int, [_P, _P, _P, _P, _I32]),
+    "re_volume_read": (C.c_int, [_P, C.c_int32, _P]),
+    "re_volume_info": (C.c_int, [_P, C.c_int32, _P, _P, _I64]),
+    "re_volume_release": (C.c_int, [_P, C.c_int32]),
+    "re_sdf_default_params": (None, [C.POINTER(re_sdf_params)]),
+    "re_sdf_trace": (C.c_int, [_P, C.c_int32, C.POINTER(re_sdf_params), _P, _P, C.c_int64, _P, _P, _P, _P,
+                               C.c_uint32]),
     "re_last_kernel_ms": (C.c_int, [_P, _F]),
     "re_kernel_times": (C.c_int, [_P, _F, C.c_int32, _I32]),
 }

@@ -25,7 +25,7 @@ OBJ = PKG / "_build"
 ROCM = Path(os.environ.get("ROCM_PATH", "/opt/rocm"))
 
 HOST_SOURCES = sorted((CSRC / "host").glob("*.cpp"))
-HIP_SOURCES = [CSRC / "capi.cpp"] + sorted((CSRC / "kernels").glob("*.hip"))
+HIP_SOURCES = sorted(CSRC.glob("capi*.cpp")) + sorted((CSRC / "kernels").glob("*.hip"))
 
 COMMON = ["-O3", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-std=c++17", "-Wall",
           "-Wno-unused-function", f"-I{PKG.parent / 'include'}"]

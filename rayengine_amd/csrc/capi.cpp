@@ -6,30 +6,11 @@
 #include <cstring>
 #include <new>
 
-#include "host/errors.hpp"
-#include "internal.hpp"
+#include "capi_util.hpp"
 
 using namespace re;
 
 namespace {
-
-re_status set_err(re_ctx* ctx, re_status st, const char* msg) {
-  fail(st, "%s", msg);
-  if (ctx) ctx->err = msg;
-  return st;
-}
-
-re_status hip_err(re_ctx* ctx, hipError_t e, const char* where) {
-  char buf[400];
-  snprintf(buf, sizeof(buf), "%s: %s", where, hipGetErrorString(e));
-  return set_err(ctx, e == hipErrorOutOfMemory ? RE_ERR_OOM : RE_ERR_HIP, buf);
-}
-
-#define HIP_TRY(ctx, expr, where)               \
-  do {                                          \
-    hipError_t _e = (expr);                     \
-    if (_e != hipSuccess) return hip_err(ctx, _e, where); \
-  } while (0)
 
 void free_scene(DevScene& s) {
   (void)hipFree(s.nodes);
@@ -37,19 +18,6 @@ void free_scene(DevScene& s) {
   (void)hipFree(s.tris);
   (void)hipFree(s.objs);
   s = DevScene{};
-}
-
-// Grows the context's staging buffer to at least `bytes`.
-re_status ensure_stage(re_ctx* ctx, size_t bytes) {
-  if (ctx->stage_bytes >= bytes) return RE_OK;
-  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream), "stage sync");
-  if (ctx->stage) (void)hipFree(ctx->stage);
-  ctx->stage = nullptr;
-  ctx->stage_bytes = 0;
-  size_t want = bytes + bytes / 4;
-  HIP_TRY(ctx, hipMalloc(&ctx->stage, want), "stage hipMalloc");
-  ctx->stage_bytes = want;
-  return RE_OK;
 }
 
 }  // namespace
@@ -90,6 +58,7 @@ RE_API void re_destroy(re_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   for (auto& kv : ctx->scenes) free_scene(kv.second);
+  for (auto& kv : ctx->volumes) (void)hipFree(kv.second.data);
   if (ctx->stage) (void)hipFree(ctx->stage);
   if (ctx->counts_buf) (void)hipFree(ctx->counts_buf);
   for (int i = 0; i < 2 * re_ctx::kRing; ++i)

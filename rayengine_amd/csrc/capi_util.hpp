@@ -1,0 +1,64 @@
+// Helpers shared by the C-ABI translation units: status/error recording,
+// HIP error mapping and the context's staging buffer for host pointers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+
+#include "host/errors.hpp"
+#include "internal.hpp"
+
+namespace re {
+
+inline re_status set_err(re_ctx* ctx, re_status st, const char* msg) {
+  fail(st, "%s", msg);
+  if (ctx) ctx->err = msg;
+  return st;
+}
+
+inline re_status hip_err(re_ctx* ctx, hipError_t e, const char* where) {
+  char buf[400];
+  snprintf(buf, sizeof(buf), "%s: %s", where, hipGetErrorString(e));
+  return set_err(ctx, e == hipErrorOutOfMemory ? RE_ERR_OOM : RE_ERR_HIP, buf);
+}
+
+#define HIP_TRY(ctx, expr, where)                          \
+  do {                                                     \
+    hipError_t _e = (expr);                                \
+    if (_e != hipSuccess) return re::hip_err(ctx, _e, where); \
+  } while (0)
+
+// Grows the context's staging buffer to at least `bytes`.
+inline re_status ensure_stage(re_ctx* ctx, size_t bytes) {
+  if (ctx->stage_bytes >= bytes) return RE_OK;
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream), "stage sync");
+  if (ctx->stage) (void)hipFree(ctx->stage);
+  ctx->stage = nullptr;
+  ctx->stage_bytes = 0;
+  size_t want = bytes + bytes / 4;
+  HIP_TRY(ctx, hipMalloc(&ctx->stage, want), "stage hipMalloc");
+  ctx->stage_bytes = want;
+  return RE_OK;
+}
+
+// Carves aligned sub-buffers out of the staging buffer.
+struct StageCursor {
+  char* base;
+  size_t off = 0;
+  template <typename T>
+  T* take(size_t count) {
+    off = (off + 255) & ~(size_t)255;
+    T* p = (T*)(base + off);
+    off += count * sizeof(T);
+    return p;
+  }
+};
+
+inline size_t stage_size(std::initializer_list<size_t> parts) {
+  size_t s = 0;
+  for (size_t p : parts) s += ((p + 255) & ~(size_t)255);
+  return s + 256;
+}
+
+}  // namespace re

@@ -10,6 +10,7 @@
 
 #include "../../include/rayengine.h"
 #include "re_math.hpp"
+#include "terrain_math.hpp"
 
 struct re_kdtree;
 
@@ -53,6 +54,58 @@ struct TraceLaunch {
 
 hipError_t launch_trace_closest(const TraceLaunch& L, hipStream_t s);
 
+// A volume resident in HBM: 8x8x8 Morton-ordered bricks.
+struct DevVolume {
+  float* data = nullptr;
+  int32_t dim[3] = {0, 0, 0};
+  int32_t bdim[3] = {0, 0, 0};
+  float spacing[3] = {0, 0, 0};
+  float box_min[3] = {0, 0, 0};
+  float box_max[3] = {0, 0, 0};
+  int64_t bytes = 0;
+  VolView view() const {
+    VolView v;
+    v.data = data;
+    for (int a = 0; a < 3; ++a) {
+      v.dim[a] = dim[a];
+      v.bdim[a] = bdim[a];
+      v.spacing[a] = spacing[a];
+    }
+    return v;
+  }
+};
+
+struct NoiseLaunch {
+  float* out;
+  const uint32_t* perm;
+  int32_t dim[3];
+  int32_t bdim[3];
+  float spacing[3];
+  FbmParams fbm;
+};
+
+struct SdfLaunch {
+  VolView vol;
+  float box_min[3], box_max[3];
+  const float* ray_o;
+  const float* ray_d;
+  int64_t n;
+  int32_t max_steps;
+  float eps, step_scale;
+  int32_t step_rule;
+  int32_t block;
+  float* t;
+  uint8_t* hit;
+  int32_t* steps;
+  float* nrm;
+};
+
+hipError_t launch_noise_fill(const NoiseLaunch& L, hipStream_t s);
+hipError_t launch_sdf_trace(const SdfLaunch& L, hipStream_t s);
+
+// Doubled Perlin permutation table of RPerlinNoise(seed).
+void perlin_table(uint32_t seed, uint32_t* perm512);
+
 }  // namespace re
 
 struct re_ctx {
@@ -65,6 +118,7 @@ struct re_ctx {
   int64_t n_launch = 0;
   int32_t next_id = 1;
   std::map<int32_t, re::DevScene> scenes;
+  std::map<int32_t, re::DevVolume> volumes;
   // Staging for host-pointer calls, grown on demand.
   void* counts_buf = nullptr;  // 256 bytes for work counters
   void* stage = nullptr;

@@ -1,0 +1,121 @@
+// Noise-volume generation and sphere tracing of the terrain field on gfx950.
+//
+// noise_fill_kernel: one work-group per 8x8x8 brick (256 lanes, 2 voxels
+// each), the 512-entry permutation table staged in LDS, every lane writing
+// its voxel's field value at its Morton slot so a wave's stores are one
+// contiguous 256-byte run.  Replaces generate_noise
+// (RayEngine/src/cuda-src/renderer.cu:105-126), which used 8-thread blocks
+// and a linear float2 grid.
+//
+// sdf_trace_kernel: one ray per lane.  Clip to the volume box, then the
+// reference march (single_ray_sphere_trace,
+// RayEngine/src/cuda-src/sphere_tracing.cuh:182-216): sample, step
+// t += d * scale, then test d <= eps * t on the stepped t, at most max_steps
+// samples.  Normals by forward differences of the sampled field with
+// delta = min(0.001 t, 0.002) (compute_sdf_normal,
+// RayEngine/src/cuda-src/sdf_functions.cuh:506-516).
+#include "../internal.hpp"
+#include "../terrain_math.hpp"
+
+namespace re {
+namespace {
+
+struct LdsTable {
+  const uint16_t* t;
+  __device__ uint32_t operator[](uint32_t i) const { return t[i]; }
+};
+
+__global__ void __launch_bounds__(256) noise_fill_kernel(float* __restrict__ out, const uint32_t* __restrict__ perm,
+                                                         int32_t dx, int32_t dy, int32_t dz, int32_t bx, int32_t by,
+                                                         float sx, float sy, float sz, FbmParams fp) {
+  __shared__ uint16_t P[512];
+  for (int i = threadIdx.x; i < 512; i += blockDim.x) P[i] = (uint16_t)perm[i];
+  __syncthreads();
+  const int64_t brick = blockIdx.x;
+  const int gx = (int)(brick % bx), gy = (int)((brick / bx) % by), gz = (int)(brick / ((int64_t)bx * by));
+  LdsTable T{P};
+  for (int m = threadIdx.x; m < kBrickVox; m += blockDim.x) {
+    const int x = gx * kBrick + (int)compact3((uint32_t)m);
+    const int y = gy * kBrick + (int)compact3((uint32_t)m >> 1);
+    const int z = gz * kBrick + (int)compact3((uint32_t)m >> 2);
+    float v = 0.0f;
+    if (x < dx && y < dy && z < dz) {
+      const f3 p{(float)x * sx, (float)y * sy, (float)z * sz};
+      v = terrain_field(T, fp, p);
+    }
+    out[brick * kBrickVox + m] = v;
+  }
+}
+
+template <bool NORMALS>
+__global__ void __launch_bounds__(256) sdf_trace_kernel(VolView V, float bx0, float by0, float bz0, float bx1,
+                                                        float by1, float bz1, const float* __restrict__ ray_o,
+                                                        const float* __restrict__ ray_d, int64_t n, int32_t max_steps,
+                                                        float eps, float step_scale, int32_t step_rule,
+                                                        float* __restrict__ t_out, uint8_t* __restrict__ hit_out,
+                                                        int32_t* __restrict__ steps_out, float* __restrict__ nrm_out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const f3 o{ray_o[3 * i], ray_o[3 * i + 1], ray_o[3 * i + 2]};
+  const f3 d{ray_d[3 * i], ray_d[3 * i + 1], ray_d[3 * i + 2]};
+  const float bmin[3] = {bx0, by0, bz0}, bmax[3] = {bx1, by1, bz1};
+  float t_near, t_far;
+  bool hit = false;
+  int32_t steps = 0;
+  float t = kInf;
+  if (ray_box(bmin, bmax, o, d, t_near, t_far)) {
+    t = t_near;
+    while (t < t_far && steps < max_steps) {
+      const f3 p = add(o, smul(t, d));
+      const float dist = trilinear(V, p);
+      const float k = step_rule == 0 ? step_scale : fmaxf(0.45f, fminf(t * 0.6f, 0.8f));
+      t += dist * k;
+      ++steps;
+      if (dist <= eps * t) {
+        hit = true;
+        break;
+      }
+    }
+  }
+  t_out[i] = hit ? t : kInf;
+  hit_out[i] = hit ? 1 : 0;
+  if (steps_out) steps_out[i] = steps;
+  if (NORMALS) {
+    f3 nrm{0.0f, 0.0f, 0.0f};
+    if (hit) {
+      const float delta = fminf((float)(0.001 * (double)t), 0.002f);
+      const f3 p = add(o, smul(t, d));
+      const float c = trilinear(V, p);
+      nrm = normalize(f3{trilinear(V, add(p, f3{delta, 0.0f, 0.0f})) - c,
+                         trilinear(V, add(p, f3{0.0f, delta, 0.0f})) - c,
+                         trilinear(V, add(p, f3{0.0f, 0.0f, delta})) - c});
+    }
+    nrm_out[3 * i] = nrm.x;
+    nrm_out[3 * i + 1] = nrm.y;
+    nrm_out[3 * i + 2] = nrm.z;
+  }
+}
+
+}  // namespace
+
+hipError_t launch_noise_fill(const NoiseLaunch& L, hipStream_t s) {
+  const int64_t bricks = (int64_t)L.bdim[0] * L.bdim[1] * L.bdim[2];
+  hipLaunchKernelGGL(noise_fill_kernel, dim3((unsigned)bricks), dim3(256), 0, s, L.out, L.perm, L.dim[0], L.dim[1],
+                     L.dim[2], L.bdim[0], L.bdim[1], L.spacing[0], L.spacing[1], L.spacing[2], L.fbm);
+  return hipGetLastError();
+}
+
+hipError_t launch_sdf_trace(const SdfLaunch& L, hipStream_t s) {
+  if (L.n <= 0) return hipSuccess;
+  const int block = L.block > 0 ? L.block : 256;
+  const int64_t grid = (L.n + block - 1) / block;
+#define RE_LAUNCH(NRM)                                                                                        \
+  hipLaunchKernelGGL((sdf_trace_kernel<NRM>), dim3((unsigned)grid), dim3(block), 0, s, L.vol, L.box_min[0],  \
+                     L.box_min[1], L.box_min[2], L.box_max[0], L.box_max[1], L.box_max[2], L.ray_o, L.ray_d, \
+                     L.n, L.max_steps, L.eps, L.step_scale, L.step_rule, L.t, L.hit, L.steps, L.nrm)
+  if (L.nrm) RE_LAUNCH(true); else RE_LAUNCH(false);
+#undef RE_LAUNCH
+  return hipGetLastError();
+}
+
+}  // namespace re

@@ -1,0 +1,142 @@
+// Gradient noise, fBm terrain field, bricked volume addressing and trilinear
+// sampling, shared by the noise-fill and sphere-tracing kernels.
+//
+// The noise primitive follows eval / gradientDotV / quintic
+// (RayEngine/src/cuda-src/sdf_functions.cuh:25-135) with its exact operation
+// order.  The reference hashes through a 2D texture whose contents are
+// undefined (a 2 KB table copied as 256 rows of 1 KB); this build defines the
+// hash as P[P[P[x] + y] + z] on the doubled 512-entry permutation table that
+// RPerlinNoise prepares (RayEngine/src/World/PerlinNoise.h:48-51).
+#pragma once
+
+#include "re_math.hpp"
+
+namespace re {
+
+// Brick of 8x8x8 voxels, Morton-ordered inside: one 128-byte cache line holds
+// a 4x4x2 block, so the 8 corners of a trilinear cell usually share 1-2 lines
+// and a ray marching through a brick stays in 2 KB.
+constexpr int kBrick = 8;
+constexpr int kBrickVox = kBrick * kBrick * kBrick;
+
+RE_HD uint32_t spread3(uint32_t v) { return (v & 1u) | ((v & 2u) << 2) | ((v & 4u) << 4); }
+RE_HD uint32_t compact3(uint32_t m) { return (m & 1u) | ((m >> 2) & 2u) | ((m >> 4) & 4u); }
+
+struct VolView {
+  const float* data;
+  int32_t dim[3];   // voxels per axis
+  int32_t bdim[3];  // bricks per axis
+  float spacing[3];
+};
+
+RE_HD int64_t vox_index(const int32_t* bdim, int x, int y, int z) {
+  int64_t b = ((int64_t)(z >> 3) * bdim[1] + (y >> 3)) * bdim[0] + (x >> 3);
+  uint32_t m = spread3((uint32_t)x & 7u) | (spread3((uint32_t)y & 7u) << 1) | (spread3((uint32_t)z & 7u) << 2);
+  return b * kBrickVox + m;
+}
+
+// 16-case gradient dot product (sdf_functions.cuh:25-48) as s1*A + s2*B with
+// A, B in {x, y, z} and s1, s2 = +-1, which rounds identically to each case
+// (x - y == x + (-y) and multiplying by +-1 is exact).
+RE_HD float grad_dot(uint32_t h, float x, float y, float z) {
+  switch (h & 15u) {
+    case 0: return x + y;
+    case 1: return -x + y;
+    case 2: return x - y;
+    case 3: return -x - y;
+    case 4: return x + z;
+    case 5: return -x + z;
+    case 6: return x - z;
+    case 7: return -x - z;
+    case 8: return y + z;
+    case 9: return -y + z;
+    case 10: return y - z;
+    case 11: return -y - z;
+    case 12: return y + x;
+    case 13: return -x + y;
+    case 14: return -y + z;
+    default: return -y - z;
+  }
+}
+
+RE_HD float quintic(float t) { return t * t * t * (t * (t * 6 - 15) + 10); }
+
+// P is the doubled table (512 entries, values 0..255).
+template <typename Table>
+RE_HD float noise_eval(const Table& P, f3 p) {
+  int fx = (int)floorf(p.x), fy = (int)floorf(p.y), fz = (int)floorf(p.z);
+  int xi0 = fx & 255, yi0 = fy & 255, zi0 = fz & 255;
+  int xi1 = (xi0 + 1) & 255, yi1 = (yi0 + 1) & 255, zi1 = (zi0 + 1) & 255;
+  float tx = p.x - (float)fx, ty = p.y - (float)fy, tz = p.z - (float)fz;
+  float u = quintic(tx), v = quintic(ty), w = quintic(tz);
+  float x0 = tx, x1 = tx - 1, y0 = ty, y1 = ty - 1, z0 = tz, z1 = tz - 1;
+  uint32_t px0 = P[xi0], px1 = P[xi1];
+  uint32_t p00 = P[px0 + yi0], p10 = P[px1 + yi0], p01 = P[px0 + yi1], p11 = P[px1 + yi1];
+  float a = grad_dot(P[p00 + zi0], x0, y0, z0);
+  float b = grad_dot(P[p10 + zi0], x1, y0, z0);
+  float c = grad_dot(P[p01 + zi0], x0, y1, z0);
+  float d = grad_dot(P[p11 + zi0], x1, y1, z0);
+  float e = grad_dot(P[p00 + zi1], x0, y0, z1);
+  float f = grad_dot(P[p10 + zi1], x1, y0, z1);
+  float g = grad_dot(P[p01 + zi1], x0, y1, z1);
+  float h = grad_dot(P[p11 + zi1], x1, y1, z1);
+  float k0 = a;
+  float k1 = (b - a);
+  float k2 = (c - a);
+  float k3 = (e - a);
+  float k4 = (a + d - b - c);
+  float k5 = (a + f - b - e);
+  float k6 = (a + g - c - e);
+  float k7 = (b + c + e + h - a - d - f - g);
+  return k0 + k1 * u + k2 * v + k3 * w + k4 * u * v + k5 * u * w + k6 * v * w + k7 * u * v * w;
+}
+
+struct FbmParams {
+  int32_t octaves;
+  float lacunarity, gain, base_freq, y_offset;
+};
+
+// Terrain field value: (p.y - y_offset) - sum_o gain^o * noise(p * base_freq *
+// lacunarity^o), i.e. minus the density fBm(p) - (p.y - y_offset).  Positive
+// above the surface, like the reference's terrain distance
+// (RayEngine/src/cuda-src/sdf_functions.cuh:476-503).
+template <typename Table>
+RE_HD float terrain_field(const Table& P, const FbmParams& fp, f3 p) {
+  float sum = 0.0f, amp = 1.0f, freq = fp.base_freq;
+  for (int o = 0; o < fp.octaves; ++o) {
+    sum = sum + amp * noise_eval(P, f3{p.x * freq, p.y * freq, p.z * freq});
+    freq = freq * fp.lacunarity;
+    amp = amp * fp.gain;
+  }
+  return (p.y - fp.y_offset) - sum;
+}
+
+// Trilinear sample at world point p: grid coordinate c = p / spacing, cell
+// index clamped to [0, dim-2], lerps in x then y then z
+// (interpolate / bilinear, RayEngine/src/cuda-src/sdf_functions.cuh:209-265).
+RE_HD float trilinear(const VolView& V, f3 p) {
+  float gx = p.x / V.spacing[0], gy = p.y / V.spacing[1], gz = p.z / V.spacing[2];
+  int ix = (int)gx, iy = (int)gy, iz = (int)gz;
+  ix = ix < 0 ? 0 : (ix > V.dim[0] - 2 ? V.dim[0] - 2 : ix);
+  iy = iy < 0 ? 0 : (iy > V.dim[1] - 2 ? V.dim[1] - 2 : iy);
+  iz = iz < 0 ? 0 : (iz > V.dim[2] - 2 ? V.dim[2] - 2 : iz);
+  float tx = gx - (float)ix, ty = gy - (float)iy, tz = gz - (float)iz;
+  const float* D = V.data;
+  float c000 = D[vox_index(V.bdim, ix, iy, iz)];
+  float c100 = D[vox_index(V.bdim, ix + 1, iy, iz)];
+  float c010 = D[vox_index(V.bdim, ix, iy + 1, iz)];
+  float c110 = D[vox_index(V.bdim, ix + 1, iy + 1, iz)];
+  float c001 = D[vox_index(V.bdim, ix, iy, iz + 1)];
+  float c101 = D[vox_index(V.bdim, ix + 1, iy, iz + 1)];
+  float c011 = D[vox_index(V.bdim, ix, iy + 1, iz + 1)];
+  float c111 = D[vox_index(V.bdim, ix + 1, iy + 1, iz + 1)];
+  float a0 = c000 * (1.f - tx) + c100 * tx;
+  float b0 = c010 * (1.f - tx) + c110 * tx;
+  float e = a0 * (1.f - ty) + b0 * ty;
+  float a1 = c001 * (1.f - tx) + c101 * tx;
+  float b1 = c011 * (1.f - tx) + c111 * tx;
+  float f = a1 * (1.f - ty) + b1 * ty;
+  return e * (1 - tz) + f * tz;
+}
+
+}  // namespace re

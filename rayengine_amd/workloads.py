@@ -41,3 +41,13 @@ def cluster_rays(tri_v, primary_prim, primary_t, width=1920, height=1080):
     o, d = camera_rays(CLUSTER_CAMERA, width, height)
     bo, bd = bounce_rays(tri_v, o, d, primary_prim, primary_t, BOUNCE_SEED)
     return np.concatenate([o, bo]), np.concatenate([d, bd])
+
+
+# Terrain workload: 256^3 FP32 field over the unit cube (seed 2016), traced
+# with 1920x1080 rays from an elevated camera looking across the terrain.
+TERRAIN_CAMERA = Camera(pos=(0.5, 0.95, -0.35), view=(0.0, -0.55, 1.0))
+TERRAIN_SPACING = (1.0 / 256, 1.0 / 256, 1.0 / 256)
+
+
+def terrain_rays(width: int = 1920, height: int = 1080):
+    return camera_rays(TERRAIN_CAMERA, width, height)

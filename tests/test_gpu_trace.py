@@ -32,7 +32,8 @@ def test_small_scene_bit_exact(gpu_ctx, small, cap):
     p, t = scene.trace_closest(o, d, max_leaf_visits=cap)
     po, to = oracle.trace(*tree.export(), tree.identity_object(), o, d, cap=cap)
     _parity(p, t, po, to)
-    assert (p >= 0).sum() > 8000
+    if cap >= 50:
+        assert (p >= 0).sum() > 8000
     scene.release()
 
 
