@@ -176,6 +176,8 @@ typedef struct re_trace_params {
   int32_t one_sided;       /* 1: reference one-sided test (ray_functions.cuh:182) */
   int32_t secondary;       /* is_secondary of stackless_kdtree_traversal */
   int32_t block;           /* threads per block, 0 = tuned default */
+  int32_t variant;         /* 0: persistent waves with per-lane ray refill (default);
+                              1: one ray per thread.  Results are identical. */
 } re_trace_params;
 
 RE_API void re_trace_default_params(re_trace_params* p);
@@ -253,6 +255,86 @@ RE_API void re_sdf_default_params(re_sdf_params* p);
 RE_API re_status re_sdf_trace(re_ctx* ctx, int32_t vol_id, const re_sdf_params* params,
                               const float* ray_o, const float* ray_d, int64_t n, float* t,
                               uint8_t* hit, int32_t* steps, float* normal, uint32_t flags);
+
+/* ------------------------------------------------------ SDF surrogate
+ *
+ * A learned stand-in for the terrain field with no counterpart in the
+ * reference: 16-frequency Fourier features of p, 4 hidden SiLU layers of 64,
+ * scalar output, BF16 MFMA with FP32 accumulation.  Parameters are one flat
+ * float vector of re_mlp_param_count() values:
+ *   W0[64][99] b0[64] W1[64][64] b1[64] W2[64][64] b2[64] W3[64][64] b3[64] W4[64] b4
+ * It is trained on the signed Euclidean distance to the field's zero set. */
+RE_API int32_t re_mlp_param_count(void);
+/* New network with seeded uniform initialisation (variance 1/fan_in). */
+RE_API re_status re_mlp_create(re_ctx* ctx, uint64_t seed, int32_t* mlp_id);
+RE_API re_status re_mlp_release(re_ctx* ctx, int32_t mlp_id);
+/* Replaces the parameters (and resets the Adam state). */
+RE_API re_status re_mlp_set_params(re_ctx* ctx, int32_t mlp_id, const float* params);
+/* Reads the parameters; adam_m, adam_v, step may be NULL. */
+RE_API re_status re_mlp_get_params(re_ctx* ctx, int32_t mlp_id, float* params, float* adam_m,
+                                   float* adam_v, int32_t* step);
+/* Network output at n points (3 floats each). */
+RE_API re_status re_mlp_forward(re_ctx* ctx, int32_t mlp_id, const float* pts, int64_t n,
+                                float* out, uint32_t flags);
+/* Loss mean_i w_i (out_i - target_i)^2 (w_i = over_weight when out_i >
+ * target_i, else 1) over n samples (x, y, z, target), n a multiple of 64:
+ * grad (nullable) gets its gradient, loss (nullable) its value; with adam
+ * != 0 one Adam step (beta 0.9/0.999, eps 1e-8) at rate lr is applied. */
+RE_API re_status re_mlp_batch_step(re_ctx* ctx, int32_t mlp_id, const float* samples, int64_t n,
+                                   float over_weight, int32_t adam, float lr, float* grad,
+                                   float* loss);
+/* Signed Euclidean distance (world units, positive where the field is
+ * positive) from each voxel to the field's zero set: exact separable
+ * transform of sign-change seeds; a new volume. */
+RE_API re_status re_sdf_distance_volume(re_ctx* ctx, int32_t vol_id, int32_t* dist_vol_id);
+/* n samples (x, y, z, signed distance) from a distance volume: even indices
+ * uniform in the volume, odd indices within near_band of the surface. */
+RE_API re_status re_mlp_sample(re_ctx* ctx, int32_t dist_vol_id, int64_t n, uint64_t seed,
+                               float near_band, float* out, uint32_t flags);
+
+typedef struct re_mlp_train_params {
+  int64_t n_samples;      /* default 2^22 */
+  int32_t batch;          /* default 65536 */
+  int32_t steps;          /* default 20000 */
+  float lr;               /* default 1e-3 */
+  float beta1, beta2;     /* default 0.9, 0.999 */
+  float eps;              /* default 1e-8 */
+  float over_weight;      /* loss weight of over-estimates, default 4 */
+  float near_band_voxels; /* near-surface band, default 2 voxels */
+  uint64_t seed;
+} re_mlp_train_params;
+
+RE_API void re_mlp_default_train_params(re_mlp_train_params* p);
+/* Adam training on the GPU on samples drawn from the distance volume;
+ * loss_history (nullable) receives one loss per step. */
+RE_API re_status re_mlp_train(re_ctx* ctx, int32_t mlp_id, int32_t dist_vol_id,
+                              const re_mlp_train_params* params, float* loss_history);
+
+/* Hybrid terrain tracer.  Surrogate phase: while the network's distance s
+ * exceeds tau, t += s - margin.  Exact phase: march the trilinear field with
+ * step max(f * k_exact, h_min); at the first sample with f <= 0, bisect the
+ * crossing n_bisect times; after rejoin of travel without a crossing the
+ * surrogate resumes.  With mlp_id < 0 the exact phase runs from the volume
+ * entry: the exact-field tracer at the same accuracy.  Distances are in
+ * voxels of the volume. */
+typedef struct re_hybrid_params {
+  float tau_voxels;    /* default 2 */
+  float margin_voxels; /* default 1 */
+  float k_exact;       /* default 0.05 */
+  float h_min_voxels;  /* default 0.5 */
+  float rejoin_voxels; /* default 4 */
+  int32_t n_bisect;    /* default 8 */
+  int32_t max_iters;   /* default 4096 */
+  int32_t block;       /* reserved, 0 */
+} re_hybrid_params;
+
+RE_API void re_hybrid_default_params(re_hybrid_params* p);
+/* t = hit distance (1e20 on a miss), hit = 0/1; evals (nullable) = network
+ * evaluations per ray; samples (nullable) = exact-field samples per ray. */
+RE_API re_status re_hybrid_trace(re_ctx* ctx, int32_t vol_id, int32_t mlp_id,
+                                 const re_hybrid_params* params, const float* ray_o,
+                                 const float* ray_d, int64_t n, float* t, uint8_t* hit,
+                                 int32_t* evals, int32_t* samples, uint32_t flags);
 
 /* Elapsed milliseconds of the last kernel launch of the context, measured
  * with HIP events recorded around the launch on its stream (waits for it).

@@ -214,3 +214,72 @@ def sdf_trace_f64(grid64, spacing, o, d, max_steps=256, eps=1e-3, step_scale=0.8
     lib().orc_sdf_trace_f64(_p(g), _p(dim), _p(sp), _p(o), _p(d), n, max_steps, eps, step_scale, step_rule, _p(t),
                             _p(hit), _p(steps), threads)
     return t, hit, steps
+
+
+# ------------------------------------------------------------------ surrogate MLP
+
+def _mlp_sigs():
+    L = lib()
+    for name, res, args in [
+        ("orc_mlp_n_params", C.c_int, []),
+        ("orc_mlp_features", None, [_P, C.c_int64, _P]),
+        ("orc_mlp_forward", None, [_P, _P, C.c_int64, _P]),
+        ("orc_mlp_loss_grad", C.c_double, [_P, _P, _P, C.c_int64, C.c_float, _P]),
+        ("orc_adam", None, [_P, _P, _P, _P, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float]),
+    ]:
+        f = getattr(L, name)
+        f.restype, f.argtypes = res, args
+    return L
+
+
+def mlp_n_params():
+    return _mlp_sigs().orc_mlp_n_params()
+
+
+def mlp_features(pts):
+    pts = _f(pts)
+    n = pts.size // 3
+    out = np.empty((n, 99), np.float64)
+    _mlp_sigs().orc_mlp_features(_p(pts), n, _p(out))
+    return out
+
+
+def mlp_forward(params, pts):
+    params, pts = _f(params), _f(pts)
+    n = pts.size // 3
+    out = np.empty(n, np.float64)
+    _mlp_sigs().orc_mlp_forward(_p(params), _p(pts), n, _p(out))
+    return out
+
+
+def mlp_loss_grad(params, pts, target, over_weight=1.0):
+    params, pts, target = _f(params), _f(pts), _f(target)
+    n = pts.size // 3
+    g = np.empty(params.size, np.float64)
+    loss = _mlp_sigs().orc_mlp_loss_grad(_p(params), _p(pts), _p(target), n, over_weight, _p(g))
+    return loss, g
+
+
+def adam(params, m, v, grad, step, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
+    """In-place FP64 Adam step on float32 arrays."""
+    for a in (params, m, v):
+        assert a.dtype == np.float32 and a.flags.c_contiguous
+    grad = _f(grad)
+    _mlp_sigs().orc_adam(_p(params), _p(m), _p(v), _p(grad), params.size, step, lr, b1, b2, eps)
+
+
+def truth_trace(grid64, spacing, o, d, h_voxels=0.125, threads=0):
+    """FP64 first zero crossing per ray (fine march + bisection): (t, hit)."""
+    L = lib()
+    f = L.orc_truth_trace
+    f.restype = None
+    f.argtypes = [_P, _P, _P, _P, _P, C.c_int64, C.c_double, _P, _P, C.c_int]
+    g = np.ascontiguousarray(grid64, np.float64)
+    dim = np.array([g.shape[2], g.shape[1], g.shape[0]], np.int32)
+    sp = np.array(spacing, np.float32)
+    o, d = _f(o), _f(d)
+    n = o.size // 3
+    t = np.empty(n, np.float64)
+    hit = np.empty(n, np.uint8)
+    f(_p(g), _p(dim), _p(sp), _p(o), _p(d), n, h_voxels, _p(t), _p(hit), threads)
+    return t, hit

@@ -262,3 +262,46 @@ ORC_API void orc_sdf_trace_f64(const double* grid, const int32_t* dim, const flo
     march<double>(G, bmin, bmax, o, d, max_steps, (double)eps, (double)scale, rule, t[i], hit[i], steps[i], nullptr);
   }
 }
+
+// Ground truth for the accuracy of a terrain tracer: the first crossing of
+// the zero set along each ray, in FP64 (double grid, double ray), found by
+// marching fixed steps of h_voxels voxels from the volume entry and
+// bisecting the first sign change 40 times.  A ray that starts inside
+// (f <= 0 at the entry) hits at the entry; misses return 1e20.
+ORC_API void orc_truth_trace(const double* grid, const int32_t* dim, const float* spacing, const float* ray_o,
+                             const float* ray_d, int64_t n, double h_voxels, double* t_out, uint8_t* hit_out,
+                             int threads) {
+  Grid<double> G{grid, dim[0], dim[1], dim[2], spacing[0], spacing[1], spacing[2]};
+  double bmin[3] = {0, 0, 0};
+  double bmax[3] = {(double)dim[0] * spacing[0], (double)dim[1] * spacing[1], (double)dim[2] * spacing[2]};
+  const double h = h_voxels * (double)spacing[0];
+#ifdef _OPENMP
+  if (threads <= 0) threads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 16) num_threads(threads)
+#endif
+  for (int64_t i = 0; i < n; ++i) {
+    double o[3] = {ray_o[3 * i], ray_o[3 * i + 1], ray_o[3 * i + 2]};
+    double d[3] = {ray_d[3 * i], ray_d[3 * i + 1], ray_d[3 * i + 2]};
+    double tn, tf;
+    t_out[i] = 1e20;
+    hit_out[i] = 0;
+    if (!slab<double>(bmin, bmax, o, d, tn, tf)) continue;
+    auto f = [&](double t) { return trilerp<double>(G, o[0] + t * d[0], o[1] + t * d[1], o[2] + t * d[2]); };
+    double t = tn, fp = f(t);
+    if (fp <= 0) { t_out[i] = t; hit_out[i] = 1; continue; }
+    while (t < tf) {
+      double tq = t + h, fq = f(tq);
+      if (fq <= 0) {
+        double a = t, b = tq;
+        for (int k = 0; k < 40; ++k) {
+          double m = 0.5 * (a + b);
+          if (f(m) > 0) a = m; else b = m;
+        }
+        t_out[i] = 0.5 * (a + b);
+        hit_out[i] = 1;
+        break;
+      }
+      t = tq;
+    }
+  }
+}

@@ -154,10 +154,13 @@ class KDTree:
         return ob
 
     def __del__(self):
-        h = getattr(self, "_h", None)
-        if h and _lib._lib is not None:
-            _lib._lib.re_kd_free(h)
-            self._h = None
+        try:
+            h = getattr(self, "_h", None)
+            if h and _lib._lib is not None:
+                _lib._lib.re_kd_free(h)
+                self._h = None
+        except Exception:
+            pass  # interpreter shutdown
 
 
 @dataclass
@@ -311,8 +314,109 @@ class Volume:
             self.id = None
 
 
-def _trace_params(max_leaf_visits=50, one_sided=True, secondary=False, block=0):
-    return _lib.re_trace_params(int(max_leaf_visits), int(bool(one_sided)), int(bool(secondary)), int(block))
+class Surrogate:
+    """The learned SDF surrogate network (re_mlp_*)."""
+
+    N_PARAMS = 18945
+
+    def __init__(self, ctx: Context, seed: int = 0):
+        self.ctx = ctx
+        mid = C.c_int32()
+        check(lib().re_mlp_create(ctx.handle, seed, C.byref(mid)), ctx.handle)
+        self.id = mid.value
+
+    def params(self):
+        p = np.empty(self.N_PARAMS, np.float32)
+        m = np.empty(self.N_PARAMS, np.float32)
+        v = np.empty(self.N_PARAMS, np.float32)
+        st = C.c_int32()
+        check(lib().re_mlp_get_params(self.ctx.handle, self.id, ptr(p), ptr(m), ptr(v), C.byref(st)), self.ctx.handle)
+        return p, m, v, st.value
+
+    def set_params(self, p):
+        p = _f32c(p)
+        assert p.size == self.N_PARAMS
+        check(lib().re_mlp_set_params(self.ctx.handle, self.id, ptr(p)), self.ctx.handle)
+
+    def forward(self, pts) -> np.ndarray:
+        pts = _f32c(pts, 3)
+        n = pts.size // 3
+        out = np.empty(n, np.float32)
+        check(lib().re_mlp_forward(self.ctx.handle, self.id, ptr(pts), n, ptr(out), 0), self.ctx.handle)
+        return out
+
+    def batch_step(self, samples, over_weight=4.0, adam=False, lr=1e-3):
+        """(loss, grad) of one batch of (x, y, z, target) samples; optionally one Adam step."""
+        s = _f32c(samples, 4)
+        n = s.size // 4
+        g = np.empty(self.N_PARAMS, np.float32)
+        loss = np.zeros(1, np.float32)
+        check(lib().re_mlp_batch_step(self.ctx.handle, self.id, ptr(s), n, over_weight, int(adam), lr, ptr(g),
+                                      ptr(loss)), self.ctx.handle)
+        return float(loss[0]), g
+
+    def train(self, dist_volume: "Volume", **kw) -> np.ndarray:
+        p = _lib.re_mlp_train_params()
+        lib().re_mlp_default_train_params(C.byref(p))
+        for k, v in kw.items():
+            setattr(p, k, v)
+        hist = np.zeros(max(p.steps, 1), np.float32)
+        check(lib().re_mlp_train(self.ctx.handle, self.id, dist_volume.id, C.byref(p), ptr(hist)), self.ctx.handle)
+        return hist[:p.steps]
+
+    def release(self):
+        if self.id is not None:
+            check(lib().re_mlp_release(self.ctx.handle, self.id), self.ctx.handle)
+            self.id = None
+
+
+def hybrid_params(**kw) -> _lib.re_hybrid_params:
+    p = _lib.re_hybrid_params()
+    lib().re_hybrid_default_params(C.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def hybrid_trace(vol: "Volume", net: "Surrogate | None", o, d, **kw):
+    """Hybrid trace (net given) or the exact-field march alone (net None):
+    (t, hit, evals, samples)."""
+    o, d = _f32c(o, 3), _f32c(d, 3)
+    n = o.size // 3
+    t = np.empty(n, np.float32)
+    hit = np.empty(n, np.uint8)
+    ev = np.empty(n, np.int32)
+    sm = np.empty(n, np.int32)
+    p = hybrid_params(**kw)
+    check(lib().re_hybrid_trace(vol.ctx.handle, vol.id, net.id if net is not None else -1, C.byref(p), ptr(o),
+                                ptr(d), n, ptr(t), ptr(hit), ptr(ev), ptr(sm), 0), vol.ctx.handle)
+    return t, hit, ev, sm
+
+
+def hybrid_trace_device(vol: "Volume", net, o_ptr, d_ptr, n, t_ptr, hit_ptr, sync=True, **kw):
+    p = hybrid_params(**kw)
+    flags = RE_DEVICE_PTRS | (0 if sync else RE_ASYNC)
+    check(lib().re_hybrid_trace(vol.ctx.handle, vol.id, net.id if net is not None else -1, C.byref(p), o_ptr,
+                                d_ptr, n, t_ptr, hit_ptr, None, None, flags), vol.ctx.handle)
+
+
+def distance_volume(vol: "Volume") -> "Volume":
+    """Signed Euclidean distance to the zero set of a field volume (re_sdf_distance_volume)."""
+    vid = C.c_int32()
+    check(lib().re_sdf_distance_volume(vol.ctx.handle, vol.id, C.byref(vid)), vol.ctx.handle)
+    return Volume(vol.ctx, vid.value)
+
+
+def sample_points(dist_vol: "Volume", n: int, seed: int = 1, near_band: float = 2 / 256) -> np.ndarray:
+    out = np.empty((n, 4), np.float32)
+    check(lib().re_mlp_sample(dist_vol.ctx.handle, dist_vol.id, n, seed, near_band, ptr(out), 0),
+          dist_vol.ctx.handle)
+    return out
+
+
+def _trace_params(max_leaf_visits=50, one_sided=True, secondary=False, block=0, variant=0):
+    return _lib.re_trace_params(int(max_leaf_visits), int(bool(one_sided)), int(bool(secondary)), int(block),
+                                int(variant))
 
 
 class Scene:

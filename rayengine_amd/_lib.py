@@ -60,7 +60,7 @@ class re_object(C.Structure):
 
 class re_trace_params(C.Structure):
     _fields_ = [("max_leaf_visits", C.c_int32), ("one_sided", C.c_int32), ("secondary", C.c_int32),
-                ("block", C.c_int32)]
+                ("block", C.c_int32), ("variant", C.c_int32)]
 
 
 class re_noise_params(C.Structure):
@@ -71,6 +71,18 @@ class re_noise_params(C.Structure):
 class re_sdf_params(C.Structure):
     _fields_ = [("max_steps", C.c_int32), ("eps", C.c_float), ("step_scale", C.c_float), ("step_rule", C.c_int32),
                 ("block", C.c_int32)]
+
+
+class re_mlp_train_params(C.Structure):
+    _fields_ = [("n_samples", C.c_int64), ("batch", C.c_int32), ("steps", C.c_int32), ("lr", C.c_float),
+                ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float), ("over_weight", C.c_float),
+                ("near_band_voxels", C.c_float), ("seed", C.c_uint64)]
+
+
+class re_hybrid_params(C.Structure):
+    _fields_ = [("tau_voxels", C.c_float), ("margin_voxels", C.c_float), ("k_exact", C.c_float),
+                ("h_min_voxels", C.c_float), ("rejoin_voxels", C.c_float), ("n_bisect", C.c_int32),
+                ("max_iters", C.c_int32), ("block", C.c_int32)]
 
 
 _P = C.c_void_p
@@ -116,6 +128,20 @@ SIGNATURES = {
     "re_sdf_default_params": (None, [C.POINTER(re_sdf_params)]),
     "re_sdf_trace": (C.c_int, [_P, C.c_int32, C.POINTER(re_sdf_params), _P, _P, C.c_int64, _P, _P, _P, _P,
                                C.c_uint32]),
+    "re_mlp_param_count": (C.c_int32, []),
+    "re_mlp_create": (C.c_int, [_P, C.c_uint64, _I32]),
+    "re_mlp_release": (C.c_int, [_P, C.c_int32]),
+    "re_mlp_set_params": (C.c_int, [_P, C.c_int32, _P]),
+    "re_mlp_get_params": (C.c_int, [_P, C.c_int32, _P, _P, _P, _I32]),
+    "re_mlp_forward": (C.c_int, [_P, C.c_int32, _P, C.c_int64, _P, C.c_uint32]),
+    "re_mlp_batch_step": (C.c_int, [_P, C.c_int32, _P, C.c_int64, C.c_float, C.c_int32, C.c_float, _P, _P]),
+    "re_sdf_distance_volume": (C.c_int, [_P, C.c_int32, _I32]),
+    "re_mlp_sample": (C.c_int, [_P, C.c_int32, C.c_int64, C.c_uint64, C.c_float, _P, C.c_uint32]),
+    "re_mlp_default_train_params": (None, [C.POINTER(re_mlp_train_params)]),
+    "re_mlp_train": (C.c_int, [_P, C.c_int32, C.c_int32, C.POINTER(re_mlp_train_params), _P]),
+    "re_hybrid_default_params": (None, [C.POINTER(re_hybrid_params)]),
+    "re_hybrid_trace": (C.c_int, [_P, C.c_int32, C.c_int32, C.POINTER(re_hybrid_params), _P, _P, C.c_int64, _P, _P,
+                                  _P, _P, C.c_uint32]),
     "re_last_kernel_ms": (C.c_int, [_P, _F]),
     "re_kernel_times": (C.c_int, [_P, _F, C.c_int32, _I32]),
 }

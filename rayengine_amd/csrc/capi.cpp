@@ -211,6 +211,7 @@ RE_API void re_trace_default_params(re_trace_params* p) {
   p->one_sided = 1;
   p->secondary = 0;
   p->block = 0;
+  p->variant = 0;
 }
 
 RE_API re_status re_kernel_times(re_ctx* ctx, float* ms, int32_t max, int32_t* count) {
@@ -252,6 +253,7 @@ re_status trace_common(re_ctx* ctx, int32_t scene_id, const float* ray_o, const 
   if (params) p = *params;
   if (!(p.block == 0 || p.block == 64 || p.block == 128 || p.block == 256))
     return set_err(ctx, RE_ERR_INVALID, "re_trace_closest: block must be 0, 64, 128 or 256");
+  if (p.variant != 0 && p.variant != 1) return set_err(ctx, RE_ERR_INVALID, "re_trace_closest: variant must be 0 or 1");
   if (n == 0) return RE_OK;
   HIP_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
   TraceLaunch L;
@@ -262,6 +264,14 @@ re_status trace_common(re_ctx* ctx, int32_t scene_id, const float* ray_o, const 
   L.secondary = p.secondary;
   L.block = p.block;
   L.counts = nullptr;
+  L.variant = p.variant;
+  L.queue = (unsigned long long*)((char*)ctx->counts_buf + 64);
+  {
+    const int blk = p.block > 0 ? p.block : 256;
+    int64_t& g = ctx->persistent_grid[blk == 64 ? 0 : (blk == 128 ? 1 : 2)];
+    if (g <= 0) g = trace_persistent_grid(blk);
+    L.grid_blocks = g;
+  }
   const bool dev = (flags & RE_DEVICE_PTRS) != 0;
   const size_t ray_bytes = (size_t)n * 12;
   char* base = nullptr;
@@ -295,6 +305,7 @@ re_status trace_common(re_ctx* ctx, int32_t scene_id, const float* ray_o, const 
     HIP_TRY(ctx, hipMemsetAsync(d_counts, 0, sizeof(TraceCounts), ctx->stream), "counts clear");
     L.counts = d_counts;
   }
+  if (L.variant == 0) HIP_TRY(ctx, hipMemsetAsync(L.queue, 0, sizeof(unsigned long long), ctx->stream), "queue clear");
   HIP_TRY(ctx, ctx->launch_begin(), "event");
   HIP_TRY(ctx, launch_trace_closest(L, ctx->stream), "trace_closest launch");
   HIP_TRY(ctx, ctx->launch_end(), "event");

@@ -50,7 +50,14 @@ struct TraceLaunch {
   float* t;
   int32_t* obj;
   TraceCounts* counts;  // device pointer or null
+  int32_t variant;      // 0 persistent with ray refill, 1 one ray per thread
+  int64_t grid_blocks;  // persistent grid size (blocks)
+  unsigned long long* queue;  // persistent ray counter, zeroed before launch
 };
+
+// Blocks of `block` threads that fill the device at the persistent kernel's
+// occupancy.
+int64_t trace_persistent_grid(int block);
 
 hipError_t launch_trace_closest(const TraceLaunch& L, hipStream_t s);
 
@@ -106,6 +113,48 @@ hipError_t launch_sdf_trace(const SdfLaunch& L, hipStream_t s);
 // Doubled Perlin permutation table of RPerlinNoise(seed).
 void perlin_table(uint32_t seed, uint32_t* perm512);
 
+// ---- SDF surrogate (kernels/surrogate.hip) ----
+struct DevMlp {
+  float* params = nullptr;   // FP32 master parameters (flat layout, mlp_layout.hpp)
+  float* m1 = nullptr;       // Adam first moment
+  float* m2 = nullptr;       // Adam second moment
+  void* pack = nullptr;      // bf16 MFMA operands (64 KB)
+  float* small = nullptr;    // FP32 b0..b3, W4, b4
+  float* partial = nullptr;  // per-block gradient partials (+ loss)
+  float* grad = nullptr;     // summed gradient of the last batch
+  float* loss = nullptr;     // per-step loss history
+  int64_t loss_cap = 0;
+  int32_t partial_blocks = 0;
+  int32_t step = 0;          // Adam steps taken
+};
+
+size_t mlp_train_lds_bytes();
+
+struct HybridParams {
+  float tau;      // surrogate distance below which the exact field takes over
+  float margin;   // subtracted from every surrogate step
+  float k_exact;  // exact-phase step = max(f * k_exact, h_min)
+  float h_min;
+  float rejoin;   // exact-phase travel without a crossing before the surrogate resumes
+  int32_t n_bisect;
+  int32_t max_iters;
+};
+
+hipError_t launch_hybrid_trace(const DevVolume& field, const void* pack, const float* small, const float* ray_o,
+                               const float* ray_d, int64_t n, const HybridParams& hp, bool use_mlp, float* t,
+                               uint8_t* hit, int32_t* evals, int32_t* samples, hipStream_t s);
+hipError_t launch_mlp_infer(const void* pack, const float* small, const float* pts, int64_t n, float* out,
+                            hipStream_t s);
+hipError_t launch_mlp_train(const void* pack, const float* small, const float4* samples, int64_t n_samples,
+                            int64_t batch_start, int32_t batch, float over_weight, float* partial, int32_t blocks,
+                            hipStream_t s);
+hipError_t launch_mlp_update(float* params, float* m1, float* m2, const float* partial, int32_t n_blocks,
+                             int32_t step, float lr, float b1, float b2, float eps, int32_t do_adam, float* grad_out,
+                             void* pack, float* small, float* loss_out, hipStream_t s);
+hipError_t launch_edt(const DevVolume& field, float* scratch_a, float* scratch_b, float* out_bricked, hipStream_t s);
+hipError_t launch_sample(const DevVolume& edt, int64_t n, uint64_t seed, float near_band, int32_t max_tries,
+                         float4* out, hipStream_t s);
+
 }  // namespace re
 
 struct re_ctx {
@@ -119,8 +168,10 @@ struct re_ctx {
   int32_t next_id = 1;
   std::map<int32_t, re::DevScene> scenes;
   std::map<int32_t, re::DevVolume> volumes;
+  std::map<int32_t, re::DevMlp> mlps;
   // Staging for host-pointer calls, grown on demand.
-  void* counts_buf = nullptr;  // 256 bytes for work counters
+  void* counts_buf = nullptr;  // 256 bytes: work counters at 0, ray queue counter at 64
+  int64_t persistent_grid[3] = {0, 0, 0};  // per block size 64, 128, 256
   void* stage = nullptr;
   size_t stage_bytes = 0;
   std::string err;

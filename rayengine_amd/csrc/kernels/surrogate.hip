@@ -1,0 +1,676 @@
+// Learned SDF surrogate of the terrain field: training-data kernels (signed
+// Euclidean distance transform of the field's zero set, point sampler), the
+// fused MFMA forward/backward training kernel, the Adam + operand-packing
+// kernel, and batched inference.  No reference counterpart; the CPU oracle's
+// FP64 forward/backward checks these kernels.
+//
+// MFMA layout and the register chaining of layers: csrc/mlp_layout.hpp.
+#include "../internal.hpp"
+#include "../mlp_layout.hpp"
+
+namespace re {
+namespace {
+
+using namespace mlp;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ f32x4 mfma16(bf16x8 a, bf16x8 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// sin(2 pi x) and cos(2 pi x): v_sin_f32 / v_cos_f32 take the angle in
+// revolutions; the argument is reduced to [0, 1) first.
+__device__ __forceinline__ float sin_rev(float x) { return __builtin_amdgcn_sinf(__builtin_amdgcn_fractf(x)); }
+__device__ __forceinline__ float cos_rev(float x) { return __builtin_amdgcn_cosf(__builtin_amdgcn_fractf(x)); }
+
+__device__ __forceinline__ float sigmoid_f(float z) { return 1.0f / (1.0f + __expf(-z)); }
+
+// f_k = 2^(k/2).
+__device__ __forceinline__ float freq(int k) {
+  float base = (float)(1 << (k >> 1));
+  return (k & 1) ? base * 1.41421356237f : base;
+}
+
+// Device PCG32 (same generator as the host scene generators).
+struct DevPcg {
+  uint64_t state, inc;
+  __device__ DevPcg(uint64_t initstate, uint64_t initseq) {
+    state = 0u;
+    inc = (initseq << 1u) | 1u;
+    next();
+    state += initstate;
+    next();
+  }
+  __device__ uint32_t next() {
+    uint64_t old = state;
+    state = old * 6364136223846793005ULL + inc;
+    uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+    uint32_t rot = (uint32_t)(old >> 59u);
+    return (xs >> rot) | (xs << ((32u - rot) & 31u));
+  }
+  __device__ float uniform() { return (float)(next() >> 8) * (1.0f / 16777216.0f); }
+};
+
+// ----------------------------------------------------------------- features
+
+// The B operands of layer 0 for this lane's point: K-step c < 3 holds the
+// sin/cos pairs of frequencies 4g..4g+3 of coordinate c; K-step 3 holds the
+// raw coordinates (g = 0, s < 3) and zero padding.
+__device__ __forceinline__ void features(f3 p, int g, bf16x8 xb[4]) {
+  const float pc[3] = {p.x, p.y, p.z};
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const float x = freq(4 * g + q) * pc[c];
+      xb[c][2 * q] = (__bf16)sin_rev(x);
+      xb[c][2 * q + 1] = (__bf16)cos_rev(x);
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < 8; ++s) xb[3][s] = (__bf16)((g == 0 && s < 3) ? pc[s] : 0.0f);
+}
+
+// ----------------------------------------------------------------- forward
+
+struct TileAct {
+  f32x4 z[4][4];    // pre-activations [layer][M-tile], lane holds neurons 16m+4g+r
+  bf16x8 hb[4][2];  // bf16 B operands of h0..h3 (hb[3] feeds nothing but the staging)
+  float h3[4][4];   // FP32 h3 for the output layer
+};
+
+template <bool KEEP>
+__device__ __forceinline__ float forward_tile(const bf16x8* __restrict__ packF, const float* __restrict__ small,
+                                              const bf16x8 xb[4], int lane, TileAct& T) {
+  const int g = lane >> 4;
+  bf16x8 in[2];
+  // layer 0: 4 K-steps of 32 features
+  {
+    f32x4 acc[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      acc[m] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int c = 0; c < 4; ++c) acc[m] = mfma16(packF[(oPackF0 >> 3) + (m * 4 + c) * 64 + lane], xb[c], acc[m]);
+    }
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const float z = acc[m][r] + small[16 * m + 4 * g + r];
+        if (KEEP) T.z[0][m][r] = z;
+        const float h = z * sigmoid_f(z);
+        T.hb[0][m >> 1][(m & 1) * 4 + r] = (__bf16)h;
+      }
+  }
+  // layers 1..3: 2 K-steps of 32 neurons, chained in registers
+#pragma unroll
+  for (int L = 1; L < 4; ++L) {
+    in[0] = T.hb[L - 1][0];
+    in[1] = T.hb[L - 1][1];
+    const int off = (oPackF1 + (L - 1) * kPackL) >> 3;
+    f32x4 acc[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      acc[m] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int c = 0; c < 2; ++c) acc[m] = mfma16(packF[off + (m * 2 + c) * 64 + lane], in[c], acc[m]);
+    }
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const float z = acc[m][r] + small[64 * L + 16 * m + 4 * g + r];
+        if (KEEP) T.z[L][m][r] = z;
+        const float h = z * sigmoid_f(z);
+        T.hb[L][m >> 1][(m & 1) * 4 + r] = (__bf16)h;
+        if (L == 3) T.h3[m][r] = h;
+      }
+  }
+  // output: partial dot over this lane's 16 neurons, summed over the 4 groups
+  float o = 0.0f;
+#pragma unroll
+  for (int m = 0; m < 4; ++m)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) o += small[256 + 16 * m + 4 * g + r] * T.h3[m][r];
+  o += __shfl_xor(o, 16);
+  o += __shfl_xor(o, 32);
+  return o + small[320];
+}
+
+// Copies the packed forward operands (and optionally the backward ones) and
+// the small parameters into LDS.
+__device__ __forceinline__ void load_weights(const bf16x8* __restrict__ gpack, const float* __restrict__ gsmall,
+                                             bf16x8* spack, float* ssmall, int n_vec) {
+  for (int i = threadIdx.x; i < n_vec; i += blockDim.x) spack[i] = gpack[i];
+  for (int i = threadIdx.x; i < kSmall; i += blockDim.x) ssmall[i] = gsmall[i];
+  __syncthreads();
+}
+
+// ----------------------------------------------------------------- inference
+
+__global__ void __launch_bounds__(256) mlp_infer_kernel(const bf16x8* __restrict__ gpack,
+                                                        const float* __restrict__ gsmall,
+                                                        const float* __restrict__ pts, int64_t n,
+                                                        float* __restrict__ out) {
+  __shared__ bf16x8 spack[oPackB1 >> 3];
+  __shared__ float ssmall[kSmall];
+  load_weights(gpack, gsmall, spack, ssmall, oPackB1 >> 3);
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t n_waves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  const int64_t tiles = (n + 15) / 16;
+  for (int64_t tile = wave; tile < tiles; tile += n_waves) {
+    int64_t i = tile * 16 + (lane & 15);
+    const int64_t ic = i < n ? i : n - 1;
+    const f3 p{pts[3 * ic], pts[3 * ic + 1], pts[3 * ic + 2]};
+    bf16x8 xb[4];
+    features(p, lane >> 4, xb);
+    TileAct T;
+    const float o = forward_tile<false>(spack, ssmall, xb, lane, T);
+    if (lane < 16 && i < n) out[i] = o;
+  }
+}
+
+// ----------------------------------------------------------------- training
+
+// Per 64-point chunk (16 per wave) every wave runs forward and backward on its
+// tile and stages, transposed as [neuron][point] in LDS, the bf16 dZ of every
+// layer and every layer's input activations.  Wave w then accumulates the
+// weight gradient of layer w over the chunk's 64 points with K = points.
+constexpr int kChunkPts = 64;
+
+struct TrainLds {
+  bf16x8 pack[kPack >> 3];            // 64 KB: forward and backward operands
+  float small[kSmall + 3];
+  __bf16 dzT[4][kH][kChunkPts];       // 32 KB
+  __bf16 xT[kInPad][kChunkPts];       // 16 KB
+  __bf16 hT[4][kH][kChunkPts];        // 32 KB (h0..h3)
+  float dout[kChunkPts];
+};
+
+__device__ __forceinline__ float dsilu(float z) {
+  const float s = sigmoid_f(z);
+  return s * (1.0f + z * (1.0f - s));
+}
+
+__global__ void __launch_bounds__(256) mlp_train_kernel(const bf16x8* __restrict__ gpack,
+                                                        const float* __restrict__ gsmall,
+                                                        const float4* __restrict__ samples, int64_t n_samples,
+                                                        int64_t batch_start, int32_t batch, float over_weight,
+                                                        float* __restrict__ partial) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  TrainLds& S = *reinterpret_cast<TrainLds*>(lds_raw);
+  load_weights(gpack, gsmall, S.pack, S.small, kPack >> 3);
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const int g = lane >> 4, j = lane & 15;
+  const int p_local = 16 * w + j;
+  const float inv_b = 1.0f / (float)batch;
+
+  f32x4 acc[4][8];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 8; ++b) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+  float db = 0.0f, dw4 = 0.0f, db4 = 0.0f, loss = 0.0f;
+
+  const int n_chunks = batch / kChunkPts;
+  for (int ch = blockIdx.x; ch < n_chunks; ch += gridDim.x) {
+    const int64_t idx = (batch_start + (int64_t)ch * kChunkPts + p_local) % n_samples;
+    const float4 smp = samples[idx];
+    const f3 p{smp.x, smp.y, smp.z};
+    bf16x8 xb[4];
+    features(p, g, xb);
+    TileAct T;
+    const float o = forward_tile<true>(S.pack, S.small, xb, lane, T);
+    const float r = o - smp.w;
+    const float wgt = r > 0.0f ? over_weight : 1.0f;
+    const float dout = 2.0f * wgt * r * inv_b;
+    if (g == 0) loss += wgt * r * r * inv_b;
+
+    // backward through the hidden layers, chaining dZ in registers
+    float dz[4][4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+      for (int r2 = 0; r2 < 4; ++r2) dz[m][r2] = dout * S.small[256 + 16 * m + 4 * g + r2] * dsilu(T.z[3][m][r2]);
+#pragma unroll
+    for (int L = 3; L >= 0; --L) {
+      // stage dZ_L and the layer's input activations
+#pragma unroll
+      for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int r2 = 0; r2 < 4; ++r2) S.dzT[L][16 * m + 4 * g + r2][p_local] = (__bf16)dz[m][r2];
+      if (L == 0) break;
+      bf16x8 dzb[2];
+#pragma unroll
+      for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int r2 = 0; r2 < 4; ++r2) dzb[m >> 1][(m & 1) * 4 + r2] = (__bf16)dz[m][r2];
+      const int off = (oPackB1 + (L - 1) * kPackL) >> 3;
+      f32x4 dh[4];
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        dh[m] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int c = 0; c < 2; ++c) dh[m] = mfma16(S.pack[off + (m * 2 + c) * 64 + lane], dzb[c], dh[m]);
+      }
+#pragma unroll
+      for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int r2 = 0; r2 < 4; ++r2) dz[m][r2] = dh[m][r2] * dsilu(T.z[L - 1][m][r2]);
+    }
+    // stage activations: features, h0..h3, and dout
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+      for (int s = 0; s < 8; ++s) S.xT[32 * c + 8 * g + s][p_local] = xb[c][s];
+#pragma unroll
+    for (int L = 0; L < 4; ++L)
+#pragma unroll
+      for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int r2 = 0; r2 < 4; ++r2) S.hT[L][16 * m + 4 * g + r2][p_local] = T.hb[L][m >> 1][(m & 1) * 4 + r2];
+    if (g == 0) S.dout[p_local] = dout;
+    __syncthreads();
+
+    // weight gradient of layer w: dW = dZ^T (64 x pts) . H (pts x in), K = points
+    const int n_tiles = (w == 0) ? 8 : 4;
+#pragma unroll
+    for (int kc = 0; kc < 2; ++kc) {
+#pragma unroll
+      for (int mi = 0; mi < 4; ++mi) {
+        const bf16x8 a = *reinterpret_cast<const bf16x8*>(&S.dzT[w][16 * mi + j][32 * kc + 8 * g]);
+#pragma unroll
+        for (int nj = 0; nj < 8; ++nj) {
+          if (nj >= n_tiles) break;
+          const __bf16* brow = (w == 0) ? &S.xT[16 * nj + j][32 * kc + 8 * g] : &S.hT[w - 1][16 * nj + j][32 * kc + 8 * g];
+          acc[mi][nj] = mfma16(a, *reinterpret_cast<const bf16x8*>(brow), acc[mi][nj]);
+        }
+      }
+    }
+    // bias of layer w (lane = neuron), and the output layer on wave 3
+    {
+      float s = 0.0f;
+      for (int q = 0; q < kChunkPts; ++q) s += (float)S.dzT[w][lane][q];
+      db += s;
+      if (w == 3) {
+        float s4 = 0.0f, sb = 0.0f;
+        for (int q = 0; q < kChunkPts; ++q) {
+          s4 += S.dout[q] * (float)S.hT[3][lane][q];
+          sb += S.dout[q];
+        }
+        dw4 += s4;
+        db4 += sb;
+      }
+    }
+    __syncthreads();
+  }
+
+  // write this block's partial gradient (flat parameter layout) and loss
+  float* P = partial + (int64_t)blockIdx.x * (kParams + 1);
+  const int nin = (w == 0) ? kIn : kH;
+  const int oW = (w == 0) ? oW0 : (w == 1 ? oW1 : (w == 2 ? oW2 : oW3));
+  const int ob = (w == 0) ? ob0 : (w == 1 ? ob1 : (w == 2 ? ob2 : ob3));
+  const int n_tiles = (w == 0) ? 8 : 4;
+#pragma unroll
+  for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+    for (int nj = 0; nj < 8; ++nj) {
+      if (nj >= n_tiles) break;
+#pragma unroll
+      for (int r2 = 0; r2 < 4; ++r2) {
+        const int i = 16 * mi + 4 * g + r2, k = 16 * nj + j;
+        if (k < nin) P[oW + i * nin + k] = acc[mi][nj][r2];
+      }
+    }
+  P[ob + lane] = db;
+  if (w == 3) {
+    P[oW4 + lane] = dw4;
+    if (lane == 0) P[ob4] = db4;
+  }
+  // loss: lanes with g == 0 of every wave hold per-point terms
+  float l = loss;
+  for (int off = 32; off > 0; off >>= 1) l += __shfl_xor(l, off);
+  __shared__ float wl[4];
+  if (lane == 0) wl[w] = l;
+  __syncthreads();
+  if (threadIdx.x == 0) P[kParams] = wl[0] + wl[1] + wl[2] + wl[3];
+}
+
+// Sum of the blocks' partial gradients; optionally an Adam step
+// (Kingma & Ba 2015) on the FP32 master parameters; then the bf16 operand
+// pack and FP32 small parameters are rewritten from the parameters.
+__device__ __forceinline__ void pack_param(int p, float wv, __bf16* pack, float* small) {
+  if (p < ob0) {  // W0
+    const int i = p / kIn, k = p % kIn;
+    const int m = i >> 4, c = k >> 5, gg = (k & 31) >> 3, s = k & 7;
+    pack[oPackF0 + pack_index(m, c, 4, 16 * gg + (i & 15), s)] = (__bf16)wv;
+    return;
+  }
+  const int oWs[3] = {oW1, oW2, oW3}, obs[4] = {ob0, ob1, ob2, ob3};
+  for (int L = 0; L < 4; ++L)
+    if (p >= obs[L] && p < obs[L] + kH) {
+      small[64 * L + (p - obs[L])] = wv;
+      return;
+    }
+  for (int L = 1; L < 4; ++L)
+    if (p >= oWs[L - 1] && p < oWs[L - 1] + kH * kH) {
+      const int i = (p - oWs[L - 1]) / kH, k = (p - oWs[L - 1]) % kH;
+      int gg, s;
+      kperm_inv(k & 31, gg, s);
+      pack[oPackF1 + (L - 1) * kPackL + pack_index(i >> 4, k >> 5, 2, 16 * gg + (i & 15), s)] = (__bf16)wv;
+      kperm_inv(i & 31, gg, s);
+      pack[oPackB1 + (L - 1) * kPackL + pack_index(k >> 4, i >> 5, 2, 16 * gg + (k & 15), s)] = (__bf16)wv;
+      return;
+    }
+  if (p >= oW4 && p < oW4 + kH) small[256 + (p - oW4)] = wv;
+  else if (p == ob4) small[320] = wv;
+}
+
+__global__ void mlp_update_kernel(float* __restrict__ params, float* __restrict__ m1, float* __restrict__ m2,
+                                  const float* __restrict__ partial, int32_t n_blocks, int32_t step, float lr,
+                                  float b1, float b2, float eps, int32_t do_adam, float* __restrict__ grad_out,
+                                  __bf16* __restrict__ pack, float* __restrict__ small, float* __restrict__ loss_out) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p > kParams) return;
+  float gsum = 0.0f;
+  if (partial)
+    for (int b = 0; b < n_blocks; ++b) gsum += partial[(int64_t)b * (kParams + 1) + p];
+  if (p == kParams) {
+    if (loss_out) *loss_out = gsum;
+    return;
+  }
+  if (grad_out) grad_out[p] = gsum;
+  float wv = params[p];
+  if (do_adam) {
+    const float c1 = 1.0f - powf(b1, (float)step), c2 = 1.0f - powf(b2, (float)step);
+    const float mm = b1 * m1[p] + (1.0f - b1) * gsum;
+    const float vv = b2 * m2[p] + (1.0f - b2) * gsum * gsum;
+    m1[p] = mm;
+    m2[p] = vv;
+    wv = wv - lr * (mm / c1) / (sqrtf(vv / c2) + eps);
+    params[p] = wv;
+  }
+  pack_param(p, wv, pack, small);
+}
+
+// ----------------------------------------------------------------- training data
+
+// Squared distance (voxel units) from each voxel to the zero set, seeded at
+// voxels whose sign differs from a 6-neighbour with the linear-interpolation
+// estimate of the sub-voxel distance; INF elsewhere.
+__global__ void edt_seed_kernel(VolView V, float* __restrict__ g) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t n = (int64_t)V.dim[0] * V.dim[1] * V.dim[2];
+  if (i >= n) return;
+  const int x = (int)(i % V.dim[0]), y = (int)((i / V.dim[0]) % V.dim[1]), z = (int)(i / ((int64_t)V.dim[0] * V.dim[1]));
+  const float f = V.data[vox_index(V.bdim, x, y, z)];
+  float best = 1e20f;
+  const int dx[6] = {1, -1, 0, 0, 0, 0}, dy[6] = {0, 0, 1, -1, 0, 0}, dz[6] = {0, 0, 0, 0, 1, -1};
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const int xn = x + dx[k], yn = y + dy[k], zn = z + dz[k];
+    if (xn < 0 || yn < 0 || zn < 0 || xn >= V.dim[0] || yn >= V.dim[1] || zn >= V.dim[2]) continue;
+    const float fn = V.data[vox_index(V.bdim, xn, yn, zn)];
+    if ((f > 0.0f) != (fn > 0.0f)) {
+      const float fr = fabsf(f) / fabsf(f - fn);
+      best = fminf(best, fr * fr);
+    }
+  }
+  g[i] = best;
+}
+
+// One separable pass of the exact Euclidean distance transform: for every
+// line along `axis`, D(x) = min_x' (x - x')^2 + g(x').  One work-group per
+// line; the line is staged in LDS (at most 4096 voxels).
+__global__ void __launch_bounds__(256) edt_pass_kernel(const float* __restrict__ gin, float* __restrict__ gout,
+                                                       int32_t dx, int32_t dy, int32_t dz, int32_t axis) {
+  __shared__ float line[4096];
+  const int64_t lid = blockIdx.x;
+  int len, a, b;
+  int64_t base, stride;
+  if (axis == 0) { len = dx; a = (int)(lid % dy); b = (int)(lid / dy); base = ((int64_t)b * dy + a) * dx; stride = 1; }
+  else if (axis == 1) { len = dy; a = (int)(lid % dx); b = (int)(lid / dx); base = (int64_t)b * dy * dx + a; stride = dx; }
+  else { len = dz; a = (int)(lid % dx); b = (int)(lid / dx); base = (int64_t)b * dx + a; stride = (int64_t)dx * dy; }
+  for (int x = threadIdx.x; x < len; x += blockDim.x) line[x] = gin[base + x * stride];
+  __syncthreads();
+  for (int x = threadIdx.x; x < len; x += blockDim.x) {
+    float best = 1e20f;
+    for (int q = 0; q < len; ++q) {
+      const float d = (float)(x - q);
+      best = fminf(best, d * d + line[q]);
+    }
+    gout[base + x * stride] = best;
+  }
+}
+
+// Signed distance in world units, written into a bricked volume.
+__global__ void edt_finish_kernel(VolView V, const float* __restrict__ g, float* __restrict__ out, float spacing) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t n = (int64_t)V.dim[0] * V.dim[1] * V.dim[2];
+  if (i >= n) return;
+  const int x = (int)(i % V.dim[0]), y = (int)((i / V.dim[0]) % V.dim[1]), z = (int)(i / ((int64_t)V.dim[0] * V.dim[1]));
+  const int64_t vi = vox_index(V.bdim, x, y, z);
+  const float f = V.data[vi];
+  const float d = sqrtf(fminf(g[i], 1e12f)) * spacing;
+  out[vi] = f > 0.0f ? d : -d;
+}
+
+// 2^k training points: even indices uniform in the volume, odd indices near
+// the surface (|signed distance| < near_band, by bounded rejection).  Each
+// sample is (x, y, z, signed distance).
+__global__ void sample_kernel(VolView edt, float lo, float hi, int64_t n, uint64_t seed, float near_band,
+                              int32_t max_tries, float4* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  DevPcg rng(seed, (uint64_t)i);
+  f3 p{0, 0, 0};
+  float d = 0.0f;
+  const bool near = (i & 1) != 0;
+  for (int t = 0; t < (near ? max_tries : 1); ++t) {
+    p = f3{lo + (hi - lo) * rng.uniform(), lo + (hi - lo) * rng.uniform(), lo + (hi - lo) * rng.uniform()};
+    d = trilinear(edt, p);
+    if (!near || fabsf(d) < near_band) break;
+  }
+  out[i] = float4{p.x, p.y, p.z, d};
+}
+
+// ----------------------------------------------------------------- hybrid tracer
+
+// Surrogate distance at every lane's point: the wave evaluates the network
+// in 16-point tiles; tiles whose lanes do not need it are skipped.
+__device__ __forceinline__ float wave_surrogate(const bf16x8* spack, const float* ssmall, f3 p, bool need) {
+  const int lane = threadIdx.x & 63;
+  float mine = 0.0f;
+  const unsigned long long mask = __ballot(need);
+#pragma unroll 1
+  for (int q = 0; q < 4; ++q) {
+    if (((mask >> (16 * q)) & 0xffffull) == 0ull) continue;
+    const int src = 16 * q + (lane & 15);
+    const f3 pq{__shfl(p.x, src), __shfl(p.y, src), __shfl(p.z, src)};
+    bf16x8 xb[4];
+    features(pq, lane >> 4, xb);
+    TileAct T;
+    const float o = forward_tile<false>(spack, ssmall, xb, lane, T);
+    const float v = __shfl(o, lane & 15);
+    if ((lane >> 4) == q) mine = v;
+  }
+  return mine;
+}
+
+// Exact-phase march: h = max(f * k_exact, h_min) from a point with f > 0;
+// on the first sample with f <= 0 the crossing is bisected (n_bisect steps)
+// between the last positive and the first non-positive sample.
+// Surrogate phase: step by (s - margin) while the surrogate distance s
+// exceeds tau, then hand over to the exact phase; after rejoin of exact
+// travel without a crossing, return to the surrogate.
+template <bool USE_MLP>
+__global__ void __launch_bounds__(256) hybrid_trace_kernel(
+    VolView V, float bx1, float by1, float bz1, const bf16x8* __restrict__ gpack, const float* __restrict__ gsmall,
+    const float* __restrict__ ray_o, const float* __restrict__ ray_d, int64_t n, HybridParams hp,
+    float* __restrict__ t_out, uint8_t* __restrict__ hit_out, int32_t* __restrict__ evals_out,
+    int32_t* __restrict__ samples_out) {
+  __shared__ bf16x8 spack[USE_MLP ? (oPackB1 >> 3) : 1];
+  __shared__ float ssmall[kSmall];
+  if (USE_MLP) load_weights(gpack, gsmall, spack, ssmall, oPackB1 >> 3);
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t ic = i < n ? i : n - 1;
+  const f3 o{ray_o[3 * ic], ray_o[3 * ic + 1], ray_o[3 * ic + 2]};
+  const f3 d{ray_d[3 * ic], ray_d[3 * ic + 1], ray_d[3 * ic + 2]};
+  const float bmin[3] = {0.0f, 0.0f, 0.0f}, bmax[3] = {bx1, by1, bz1};
+  enum { SURR = 0, EXACT = 1, DONE = 2 };
+  int phase = DONE;
+  bool hit = false;
+  float t = 0.0f, t_far = 0.0f, f_prev = 0.0f, travel = 0.0f;
+  int32_t evals = 0, samples = 0;
+  float tn, tf;
+  if (i < n && ray_box(bmin, bmax, o, d, tn, tf)) {
+    t = tn;
+    t_far = tf;
+    phase = USE_MLP ? SURR : EXACT;
+    if (!USE_MLP) {
+      f_prev = trilinear(V, add(o, smul(t, d)));
+      ++samples;
+      if (f_prev <= 0.0f) { hit = true; phase = DONE; }
+    }
+  }
+#pragma unroll 1
+  for (int it = 0; it < hp.max_iters; ++it) {
+    if (!__any(phase != DONE)) break;
+    if (USE_MLP) {
+      const bool need = phase == SURR;
+      if (__any(need)) {
+        const float s = wave_surrogate(spack, ssmall, add(o, smul(t, d)), need);
+        if (need) {
+          ++evals;
+          if (s > hp.tau) {
+            t += s - hp.margin;
+            if (t >= t_far) phase = DONE;
+          } else {
+            phase = EXACT;
+            travel = 0.0f;
+            f_prev = trilinear(V, add(o, smul(t, d)));
+            ++samples;
+            if (f_prev <= 0.0f) {
+              hit = true;  // entered the surface region already inside: accept the current point
+              phase = DONE;
+            }
+          }
+        }
+      }
+    }
+    if (phase == EXACT) {
+      const float h = fmaxf(f_prev * hp.k_exact, hp.h_min);
+      const float t_next = t + h;
+      const float f_next = trilinear(V, add(o, smul(t_next, d)));
+      ++samples;
+      if (f_next <= 0.0f) {
+        float a = t, b = t_next;
+        for (int k = 0; k < hp.n_bisect; ++k) {
+          const float mid = 0.5f * (a + b);
+          const float fm = trilinear(V, add(o, smul(mid, d)));
+          ++samples;
+          if (fm > 0.0f) a = mid; else b = mid;
+        }
+        t = 0.5f * (a + b);
+        hit = true;
+        phase = DONE;
+      } else {
+        t = t_next;
+        f_prev = f_next;
+        travel += h;
+        if (t >= t_far) phase = DONE;
+        else if (USE_MLP && travel > hp.rejoin) phase = SURR;
+      }
+    }
+  }
+  if (i < n) {
+    t_out[i] = hit ? t : kInf;
+    hit_out[i] = hit ? 1 : 0;
+    if (evals_out) evals_out[i] = evals;
+    if (samples_out) samples_out[i] = samples;
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- launchers
+
+hipError_t launch_hybrid_trace(const DevVolume& field, const void* pack, const float* small, const float* ray_o,
+                               const float* ray_d, int64_t n, const HybridParams& hp, bool use_mlp, float* t,
+                               uint8_t* hit, int32_t* evals, int32_t* samples, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const unsigned blocks = (unsigned)((n + 255) / 256);
+  const VolView V = field.view();
+  if (use_mlp)
+    hipLaunchKernelGGL(hybrid_trace_kernel<true>, dim3(blocks), dim3(256), 0, s, V, field.box_max[0],
+                       field.box_max[1], field.box_max[2], (const bf16x8*)pack, small, ray_o, ray_d, n, hp, t, hit,
+                       evals, samples);
+  else
+    hipLaunchKernelGGL(hybrid_trace_kernel<false>, dim3(blocks), dim3(256), 0, s, V, field.box_max[0],
+                       field.box_max[1], field.box_max[2], (const bf16x8*)pack, small, ray_o, ray_d, n, hp, t, hit,
+                       evals, samples);
+  return hipGetLastError();
+}
+
+size_t mlp_train_lds_bytes() { return sizeof(TrainLds); }
+
+hipError_t launch_mlp_infer(const void* pack, const float* small, const float* pts, int64_t n, float* out,
+                            hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const int64_t tiles = (n + 15) / 16;
+  int64_t blocks = (tiles + 3) / 4;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(mlp_infer_kernel, dim3((unsigned)blocks), dim3(256), 0, s, (const bf16x8*)pack, small, pts, n,
+                     out);
+  return hipGetLastError();
+}
+
+hipError_t launch_mlp_train(const void* pack, const float* small, const float4* samples, int64_t n_samples,
+                            int64_t batch_start, int32_t batch, float over_weight, float* partial, int32_t blocks,
+                            hipStream_t s) {
+  hipLaunchKernelGGL(mlp_train_kernel, dim3(blocks), dim3(256), sizeof(TrainLds), s, (const bf16x8*)pack, small,
+                     samples, n_samples, batch_start, batch, over_weight, partial);
+  return hipGetLastError();
+}
+
+hipError_t launch_mlp_update(float* params, float* m1, float* m2, const float* partial, int32_t n_blocks,
+                             int32_t step, float lr, float b1, float b2, float eps, int32_t do_adam, float* grad_out,
+                             void* pack, float* small, float* loss_out, hipStream_t s) {
+  const int threads = 256;
+  const int blocks = (kParams + 1 + threads - 1) / threads;
+  hipLaunchKernelGGL(mlp_update_kernel, dim3(blocks), dim3(threads), 0, s, params, m1, m2, partial, n_blocks, step,
+                     lr, b1, b2, eps, do_adam, grad_out, (__bf16*)pack, small, loss_out);
+  return hipGetLastError();
+}
+
+hipError_t launch_edt(const DevVolume& field, float* scratch_a, float* scratch_b, float* out_bricked,
+                      hipStream_t s) {
+  const VolView V = field.view();
+  const int64_t n = (int64_t)V.dim[0] * V.dim[1] * V.dim[2];
+  const int64_t blocks = (n + 255) / 256;
+  hipLaunchKernelGGL(edt_seed_kernel, dim3((unsigned)blocks), dim3(256), 0, s, V, scratch_a);
+  hipLaunchKernelGGL(edt_pass_kernel, dim3((unsigned)((int64_t)V.dim[1] * V.dim[2])), dim3(256), 0, s, scratch_a,
+                     scratch_b, V.dim[0], V.dim[1], V.dim[2], 0);
+  hipLaunchKernelGGL(edt_pass_kernel, dim3((unsigned)((int64_t)V.dim[0] * V.dim[2])), dim3(256), 0, s, scratch_b,
+                     scratch_a, V.dim[0], V.dim[1], V.dim[2], 1);
+  hipLaunchKernelGGL(edt_pass_kernel, dim3((unsigned)((int64_t)V.dim[0] * V.dim[1])), dim3(256), 0, s, scratch_a,
+                     scratch_b, V.dim[0], V.dim[1], V.dim[2], 2);
+  hipLaunchKernelGGL(edt_finish_kernel, dim3((unsigned)blocks), dim3(256), 0, s, V, scratch_b, out_bricked,
+                     V.spacing[0]);
+  return hipGetLastError();
+}
+
+hipError_t launch_sample(const DevVolume& edt, int64_t n, uint64_t seed, float near_band, int32_t max_tries,
+                         float4* out, hipStream_t s) {
+  const VolView V = edt.view();
+  const float hi = (float)(V.dim[0] - 1) * V.spacing[0];
+  hipLaunchKernelGGL(sample_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, V, 0.0f, hi, n, seed,
+                     near_band, max_tries, out);
+  return hipGetLastError();
+}
+
+}  // namespace re

@@ -16,6 +16,45 @@
 namespace re {
 namespace {
 
+struct TriGeom {
+  f3 v0, e1, e2;
+};
+
+__device__ __forceinline__ TriGeom load_geom(const TriRec* __restrict__ tp, uint32_t j) {
+  const float* f = tp[j].v0;
+  return TriGeom{f3{f[0], f[1], f[2]}, f3{f[3], f[4], f[5]}, f3{f[6], f[7], f[8]}};
+}
+
+// Tests a leaf's triangles in record order (the reference's in-leaf order,
+// ray_functions.cuh:295-324).  The geometry of up to four records is loaded
+// before any of them is tested, so a lane has four independent loads in
+// flight instead of one; the prim id is read only for an accepted hit.
+template <bool ONE_SIDED, bool COUNT>
+__device__ __forceinline__ void test_leaf(const TriRec* __restrict__ tp, uint32_t count, f3 ro, f3 rd,
+                                          float& t_exit, bool& found, float& hit_t, int32_t& hit_prim,
+                                          unsigned& c_tri) {
+  for (uint32_t j = 0; j < count; j += 4) {
+    TriGeom g[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (j + q < count) g[q] = load_geom(tp, j + q);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (j + q >= count) break;
+      if (COUNT) ++c_tri;
+      float t;
+      const bool h = ONE_SIDED ? ray_tri(ro, rd, g[q].v0, g[q].e1, g[q].e2, t)
+                               : ray_tri_2s(ro, rd, g[q].v0, g[q].e1, g[q].e2, t);
+      if (h && t < t_exit) {
+        found = true;
+        hit_t = t;
+        hit_prim = tp[j + q].prim;
+        t_exit = t;
+      }
+    }
+  }
+}
+
 template <bool COUNT, bool ONE_SIDED>
 __global__ void __launch_bounds__(256) trace_closest_kernel(
     const NodeEntry* __restrict__ nodes, const LeafRec* __restrict__ leaves,
@@ -58,23 +97,7 @@ __global__ void __launch_bounds__(256) trace_closest_kernel(
       }
       const LeafRec L = leaves[ref_leaf_id(cur)];
       if (COUNT) ++c_leaf;
-      for (uint32_t j = 0; j < L.count; ++j) {
-        const TriRec tr = tris[L.first + j];
-        if (COUNT) ++c_tri;
-        float t;
-        bool h = ONE_SIDED ? ray_tri(ro, rd, f3{tr.v0[0], tr.v0[1], tr.v0[2]},
-                                     f3{tr.e1[0], tr.e1[1], tr.e1[2]},
-                                     f3{tr.e2[0], tr.e2[1], tr.e2[2]}, t)
-                           : ray_tri_2s(ro, rd, f3{tr.v0[0], tr.v0[1], tr.v0[2]},
-                                        f3{tr.e1[0], tr.e1[1], tr.e1[2]},
-                                        f3{tr.e2[0], tr.e2[1], tr.e2[2]}, t);
-        if (h && t < t_exit) {
-          found = true;
-          hit_t = t;
-          hit_prim = tr.prim;
-          t_exit = t;
-        }
-      }
+      test_leaf<ONE_SIDED, COUNT>(tris + L.first, L.count, ro, rd, t_exit, found, hit_t, hit_prim, c_tri);
       float tn, tf;
       if (!ray_box(L.bmin, L.bmax, ro, rd, tn, tf)) break;
       t_entry = tf;
@@ -102,17 +125,182 @@ __global__ void __launch_bounds__(256) trace_closest_kernel(
   }
 }
 
+// ---------------------------------------------------------------------------
+// Persistent variant.  A wave keeps 64 rays in flight and refills each lane
+// as soon as its ray finishes, from a wave-private chunk of CHUNK
+// consecutive ray indices taken from a global counter (one atomic per
+// chunk).  One loop iteration advances every live lane by one leaf visit, so
+// a ray that needs many visits (the cap, or a crowded leaf) no longer holds
+// 63 idle lanes.  Per-ray arithmetic is identical to trace_closest_kernel.
+// ---------------------------------------------------------------------------
+constexpr int kChunk = 64;
+
+template <bool COUNT, bool ONE_SIDED>
+__global__ void __launch_bounds__(256) trace_persistent_kernel(
+    const NodeEntry* __restrict__ nodes, const LeafRec* __restrict__ leaves,
+    const TriRec* __restrict__ tris, const ObjXform* __restrict__ objs, int32_t n_obj,
+    const float* __restrict__ ray_o, const float* __restrict__ ray_d, int64_t n, int32_t cap,
+    int32_t secondary, int32_t* __restrict__ prim_out, float* __restrict__ t_out,
+    int32_t* __restrict__ obj_out, TraceCounts* __restrict__ counts,
+    unsigned long long* __restrict__ queue) {
+  const int lane = (int)(threadIdx.x & 63);
+  const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  int64_t pool_next = 0, pool_end = 0;  // wave-uniform
+  bool drained = false;                 // wave-uniform
+
+  int64_t ray = -1;
+  f3 o{0, 0, 0}, d{0, 0, 0}, ro{0, 0, 0}, rd{0, 0, 0}, fr{0, 0, 0};
+  float t_entry = 0.0f, t_exit = 0.0f, hit_t = kInf, best_t = kInf;
+  int32_t cur = 0, visits = 0, k_obj = 0, hit_prim = -1, best_prim = -1, best_obj = -1;
+  bool live = false, found = false;
+  unsigned c_ent = 0, c_leaf = 0, c_tri = 0, c_hit = 0;
+
+  for (;;) {
+    // ---- refill idle lanes (wave-uniform control flow) ----
+    const unsigned long long need = __ballot(ray < 0);
+    if (need != 0ull && !drained) {
+      if (pool_next >= pool_end) {
+        unsigned long long b = 0;
+        if (lane == 0) b = atomicAdd(queue, (unsigned long long)kChunk);
+        b = __shfl(b, 0);
+        pool_next = (int64_t)b;
+        pool_end = (int64_t)b + kChunk < n ? (int64_t)b + kChunk : n;
+        if ((int64_t)b >= n) drained = true;
+      }
+      if (!drained) {
+        const int64_t avail = pool_end - pool_next;
+        const int rank = __popcll(need & below);
+        if (ray < 0 && rank < avail) {
+          ray = pool_next + rank;
+          o = f3{ray_o[3 * ray], ray_o[3 * ray + 1], ray_o[3 * ray + 2]};
+          d = f3{ray_d[3 * ray], ray_d[3 * ray + 1], ray_d[3 * ray + 2]};
+          best_t = kInf;
+          best_prim = -1;
+          best_obj = -1;
+          k_obj = -1;
+          live = false;
+        }
+        const int64_t taken = (int64_t)__popcll(need) < avail ? (int64_t)__popcll(need) : avail;
+        pool_next += taken;
+      }
+    }
+    if (!__any(ray >= 0)) {
+      if (drained) break;
+      continue;
+    }
+    if (ray < 0) continue;
+
+    // ---- start the next object whose root box the ray hits ----
+    while (!live && k_obj + 1 < n_obj) {
+      ++k_obj;
+      const ObjXform ob = objs[k_obj];
+      xform_ray(ob, o, d, secondary != 0, ro, rd);
+      fr = f3{1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z};
+      if (!ray_box_inv(ob.root_min, ob.root_max, ro, fr, t_entry, t_exit)) continue;
+      cur = ob.root;
+      visits = 0;
+      found = false;
+      hit_t = kInf;
+      hit_prim = -1;
+      live = (t_entry < t_exit && visits < cap);
+    }
+
+    // ---- one leaf visit ----
+    if (live) {
+      ++visits;
+      const f3 pe = add(ro, smul(t_entry, rd));
+      while (!ref_is_leaf(cur)) {
+        const NodeEntry e = nodes[cur];
+        if (COUNT) ++c_ent;
+        const uint32_t ax = e.meta >> 30;
+        const int32_t other = (int32_t)(e.meta & 0x3fffffffu);
+        if (ax == 3u) {
+          cur = leaf_ref((uint32_t)other);
+          break;
+        }
+        cur = (get(pe, (int)ax) < e.split) ? cur + 1 : other;
+      }
+      const LeafRec L = leaves[ref_leaf_id(cur)];
+      if (COUNT) ++c_leaf;
+      test_leaf<ONE_SIDED, COUNT>(tris + L.first, L.count, ro, rd, t_exit, found, hit_t, hit_prim, c_tri);
+      float tn, tf;
+      if (!ray_box_inv(L.bmin, L.bmax, ro, fr, tn, tf)) {
+        live = false;
+      } else {
+        t_entry = tf;
+        const f3 px = add(ro, smul(t_entry, rd));
+        const int face = exit_face(L.bmin, L.bmax, px);
+        const int32_t r = face < 0 ? kNoRope : L.rope[face];
+        if (r == kNoRope) {
+          live = false;
+        } else {
+          cur = r;
+          live = (t_entry < t_exit && visits < cap);
+        }
+      }
+      if (!live && found && hit_t < best_t) {
+        best_t = hit_t;
+        best_prim = hit_prim;
+        best_obj = k_obj;
+      }
+    }
+
+    // ---- retire the ray once every object is done ----
+    if (!live && k_obj + 1 >= n_obj) {
+      prim_out[ray] = best_prim;
+      t_out[ray] = best_t;
+      if (obj_out) obj_out[ray] = best_obj;
+      if (COUNT && best_prim >= 0) ++c_hit;
+      ray = -1;
+    }
+  }
+  if (COUNT) {
+    atomicAdd(&counts->entries, (unsigned long long)c_ent);
+    atomicAdd(&counts->leaves, (unsigned long long)c_leaf);
+    atomicAdd(&counts->tris, (unsigned long long)c_tri);
+    atomicAdd(&counts->rays_hit, (unsigned long long)c_hit);
+  }
+}
+
 }  // namespace
+
+int64_t trace_persistent_grid(int block) {
+  int dev = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, trace_persistent_kernel<false, true>, block, 0) !=
+      hipSuccess)
+    return 0;
+  return (int64_t)cus * (per_cu > 0 ? per_cu : 1);
+}
 
 hipError_t launch_trace_closest(const TraceLaunch& L, hipStream_t s) {
   if (L.n <= 0) return hipSuccess;
   const int block = L.block > 0 ? L.block : 256;
-  const int64_t grid = (L.n + block - 1) / block;
   const DevScene& S = *L.scene;
   // A ray can alternate between two leaves when its exit point lies within
   // the rope tolerance of two faces; the visit cap is what ends it, so
   // "unlimited" is a large bound that every ray reaches.
   int32_t cap = (L.cap > 0 && L.cap < kMaxLeafVisits) ? L.cap : kMaxLeafVisits;
+  if (L.variant == 0) {
+    // Persistent: enough blocks to fill every CU at the kernel's occupancy,
+    // never more than the rays need.
+    int64_t want = (L.n + block - 1) / block;
+    int64_t grid = L.grid_blocks > 0 ? L.grid_blocks : want;
+    if (grid > want) grid = want;
+#define RE_LAUNCH(C, O)                                                                                     \
+  hipLaunchKernelGGL((trace_persistent_kernel<C, O>), dim3((unsigned)grid), dim3(block), 0, s, S.nodes,     \
+                     S.leaves, S.tris, S.objs, S.n_obj, L.ray_o, L.ray_d, L.n, cap, L.secondary, L.prim, L.t, \
+                     L.obj, L.counts, L.queue)
+    if (L.counts) {
+      if (L.one_sided) RE_LAUNCH(true, true); else RE_LAUNCH(true, false);
+    } else {
+      if (L.one_sided) RE_LAUNCH(false, true); else RE_LAUNCH(false, false);
+    }
+#undef RE_LAUNCH
+    return hipGetLastError();
+  }
+  const int64_t grid = (L.n + block - 1) / block;
 #define RE_LAUNCH(C, O)                                                                      \
   hipLaunchKernelGGL((trace_closest_kernel<C, O>), dim3((unsigned)grid), dim3(block), 0, s,  \
                      S.nodes, S.leaves, S.tris, S.objs, S.n_obj, L.ray_o, L.ray_d, L.n, cap, \

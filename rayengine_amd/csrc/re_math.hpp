@@ -72,6 +72,24 @@ RE_HD bool ray_box(const float* bmin, const float* bmax, f3 o, f3 d, float& t_ne
   return true;
 }
 
+// The same slab test with the direction reciprocals computed once per ray
+// (fr = 1/d per component, bit-identical to forming them per call).
+RE_HD bool ray_box_inv(const float* bmin, const float* bmax, f3 o, f3 fr, float& t_near, float& t_far) {
+  float t1 = (bmin[0] - o.x) * fr.x;
+  float t2 = (bmax[0] - o.x) * fr.x;
+  float t3 = (bmin[1] - o.y) * fr.y;
+  float t4 = (bmax[1] - o.y) * fr.y;
+  float t5 = (bmin[2] - o.z) * fr.z;
+  float t6 = (bmax[2] - o.z) * fr.z;
+  float tmin = fmax_(fmax_(fmin_(t1, t2), fmin_(t3, t4)), fmin_(t5, t6));
+  float tmax = fmin_(fmin_(fmax_(t1, t2), fmax_(t3, t4)), fmax_(t5, t6));
+  if (tmax < 0.0f) return false;
+  if (tmin > tmax) return false;
+  t_near = tmin >= 0.0f ? tmin : 0.0f;
+  t_far = tmax;
+  return true;
+}
+
 // One-sided Moller-Trumbore.  RayEngine/src/cuda-src/ray_functions.cuh:173-223.
 // det < kEps rejects back faces and near-parallel rays; a hit needs t > kEps.
 RE_HD bool ray_tri(f3 o, f3 d, f3 v0, f3 e1, f3 e2, float& t_out) {

@@ -24,12 +24,13 @@ def _parity(p, t, po, to):
     assert np.array_equal(t.view(np.uint32), to.view(np.uint32)), f"{int((t.view(np.uint32) != to.view(np.uint32)).sum())} t differ"
 
 
+@pytest.mark.parametrize("variant", [0, 1])
 @pytest.mark.parametrize("cap", [50, 1, 4096])
-def test_small_scene_bit_exact(gpu_ctx, small, cap):
-    say(f"small scene, cap {cap}")
+def test_small_scene_bit_exact(gpu_ctx, small, cap, variant):
+    say(f"small scene, cap {cap}, variant {variant}")
     tri_v, tree, o, d = small
     scene = gpu_ctx.upload_scene([re.SceneObject(tree)])
-    p, t = scene.trace_closest(o, d, max_leaf_visits=cap)
+    p, t = scene.trace_closest(o, d, max_leaf_visits=cap, variant=variant)
     po, to = oracle.trace(*tree.export(), tree.identity_object(), o, d, cap=cap)
     _parity(p, t, po, to)
     if cap >= 50:
@@ -51,15 +52,17 @@ def test_two_sided_and_block_shapes(gpu_ctx, small):
     scene = gpu_ctx.upload_scene([re.SceneObject(tree)])
     po, to = oracle.trace(*tree.export(), tree.identity_object(), o, d, two_sided=True)
     for block in (64, 128, 256):
-        say(f"two-sided, block {block}")
-        p, t = scene.trace_closest(o, d, one_sided=False, block=block)
-        _parity(p, t, po, to)
+        for variant in (0, 1):
+            say(f"two-sided, block {block}, variant {variant}")
+            p, t = scene.trace_closest(o, d, one_sided=False, block=block, variant=variant)
+            _parity(p, t, po, to)
 
 
-def test_counters_match_oracle(gpu_ctx, small):
+@pytest.mark.parametrize("variant", [0, 1])
+def test_counters_match_oracle(gpu_ctx, small, variant):
     tri_v, tree, o, d = small
     scene = gpu_ctx.upload_scene([re.SceneObject(tree)])
-    p, t, cnt = scene.trace_counts(o, d)
+    p, t, cnt = scene.trace_counts(o, d, variant=variant)
     po, to, co = oracle.trace(*tree.export(), tree.identity_object(), o, d, counts=True)
     _parity(p, t, po, to)
     assert cnt[:3] == [int(x) for x in co.sum(0)]
@@ -80,10 +83,11 @@ def test_multi_object_transforms(gpu_ctx):
     nodes, leaves, tris, obs = scene.export()
     o, d = re.camera_rays(re.Camera(pos=(0.3, 0.5, 6.0), view=(0.0, -0.1, -1.0)), 160, 120)
     for secondary in (False, True):
-        say(f"multi-object, secondary={secondary}")
-        p, t, ob = scene.trace_closest(o, d, with_obj=True, secondary=secondary)
         po, to = oracle.trace(nodes, leaves, tris, obs, o, d, secondary=secondary)
-        _parity(p, t, po, to)
+        for variant in (0, 1):
+            say(f"multi-object, secondary={secondary}, variant {variant}")
+            p, t, ob = scene.trace_closest(o, d, with_obj=True, secondary=secondary, variant=variant)
+            _parity(p, t, po, to)
         assert set(np.unique(ob[p >= 0])) <= {0, 1, 2}
     assert (p >= 0).sum() > 1000
 
@@ -104,9 +108,10 @@ def test_degenerate_rays(gpu_ctx, small):
     o[128:512, 0] = leaves["bmin"][rng.integers(0, len(leaves), 384), 0]
     d[512:1024] = -0.0
     d[512:1024, 2] = -1.0
-    p, t = scene.trace_closest(o, d)
     po, to = oracle.trace(nodes, leaves, tris, tree.identity_object(), o, d)
-    _parity(p, t, po, to)
+    for variant in (0, 1):
+        p, t = scene.trace_closest(o, d, variant=variant)
+        _parity(p, t, po, to)
 
 
 def test_empty_batch_and_empty_scene(gpu_ctx):
@@ -163,6 +168,8 @@ def test_cluster_scene_subset_bit_exact(gpu_ctx, cluster):
     ro, rd = W.cluster_rays(tri_v, p, t)
     say(f"cluster all rays: {len(ro)}")
     pa, ta = scene.trace_closest(ro, rd)
+    pv, tv = scene.trace_closest(ro, rd, variant=1)
+    _parity(pa, ta, pv, tv)
     assert np.array_equal(pa[:len(o)], p) and np.array_equal(ta[:len(o)].view(np.uint32), t.view(np.uint32))
     rng = np.random.default_rng(11)
     sel = np.concatenate([rng.choice(len(o), 32768, replace=False),
