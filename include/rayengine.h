@@ -111,6 +111,10 @@ typedef struct re_kd_build_params {
   float cost_intersect;   /* SAH intersection cost (default 1.5) */
   float empty_bonus;      /* SAH factor for a split with an empty side (default 0.8) */
   int32_t n_bins;         /* bins per axis for large nodes (default 32) */
+  int32_t clip;           /* 1: a triangle straddling a split goes only to the children its
+                             clipped polygon reaches, with the clipped box ("perfect
+                             splits"); 0: by its bounding box (reference rule,
+                             KDTree.cpp:581-600).  Default 1. */
 } re_kd_build_params;
 
 typedef struct re_kd_info {
@@ -176,8 +180,8 @@ typedef struct re_trace_params {
   int32_t one_sided;       /* 1: reference one-sided test (ray_functions.cuh:182) */
   int32_t secondary;       /* is_secondary of stackless_kdtree_traversal */
   int32_t block;           /* threads per block, 0 = tuned default */
-  int32_t variant;         /* 0: persistent waves with per-lane ray refill (default);
-                              1: one ray per thread.  Results are identical. */
+  int32_t variant;         /* 0: one ray per thread (default); 1: persistent waves
+                              with per-lane ray refill.  Results are identical. */
 } re_trace_params;
 
 RE_API void re_trace_default_params(re_trace_params* p);

@@ -72,12 +72,15 @@ def _f(a, k=None):
 
 
 def trace(nodes, leaves, tris, objs, o, d, cap=50, two_sided=False, secondary=False, counts=False, threads=0):
-    """FP32 rope traversal; returns (prim, t[, counts (n,3) uint64])."""
+    """FP32 rope traversal; returns (prim, t[, counts]).  counts is (n, 4)
+    uint64: node entries, leaf records, triangle tests, and how the walk ended
+    (0 t_entry >= t_exit, 1 cap, 2 leaf slab miss, 3 no face within K_EPSILON,
+    4 rope -1, 5 root box missed)."""
     o, d = _f(o), _f(d)
     n = o.size // 3
     prim = np.empty(n, np.int32)
     t = np.empty(n, np.float32)
-    cnt = np.empty((n, 3), np.uint64) if counts else None
+    cnt = np.empty((n, 4), np.uint64) if counts else None
     lib().orc_trace(_p(nodes), _p(leaves), _p(tris), _p(objs), len(objs), _p(o), _p(d), n, cap, int(two_sided),
                     int(secondary), _p(prim), _p(t), None, _p(cnt), threads)
     return (prim, t, cnt) if counts else (prim, t)

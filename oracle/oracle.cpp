@@ -147,7 +147,10 @@ void transform(const Obj& ob, V3<T> o, V3<T> d, bool secondary, V3<T>& no, V3<T>
   nd = unit(nd);
 }
 
-struct Counts { uint64_t entries, leaves, tris; };
+// Work counters of a ray and how its last traversal loop ended:
+// 0 t_entry >= t_exit, 1 visit cap, 2 leaf slab miss, 3 no face within
+// K_EPSILON, 4 rope -1 (left the root box), 5 root box missed.
+struct Counts { uint64_t entries, leaves, tris, reason; };
 
 // stackless_kdtree_traversal, RayEngine/src/cuda-src/ray_functions.cuh:230-360,
 // for one object.  Returns true with (t, prim) of the accepted hit.
@@ -156,7 +159,10 @@ bool traverse(const Node* nodes, const Leaf* leaves, const Tri* tris, const Obj&
   V3<float> no, nd;
   transform(ob, o, d, secondary, no, nd);
   float t_entry, t_exit;
-  if (!box_test(ob.root_min, ob.root_max, no, nd, t_entry, t_exit)) return false;
+  if (!box_test(ob.root_min, ob.root_max, no, nd, t_entry, t_exit)) {
+    cnt.reason = 5;
+    return false;
+  }
   bool detected = false;
   int limit = 0;
   int32_t cur = ob.root;  // current node as a ref
@@ -189,13 +195,24 @@ bool traverse(const Node* nodes, const Leaf* leaves, const Tri* tris, const Obj&
       }
     }
     float tn, tf;
-    if (!box_test(L.bmin, L.bmax, no, nd, tn, tf)) break;  // :328-337
+    if (!box_test(L.bmin, L.bmax, no, nd, tn, tf)) { cnt.reason = 2; return detected; }  // :328-337
     t_entry = tf;
     V3<float> p_exit = no + (t_entry * nd);
     int32_t next = neighbour(L, p_exit);  // :341-347
-    if (next == -1) break;
+    if (next == -1) {
+      int face = -1;
+      for (int f = 0; f < 6 && face < 0; ++f) {
+        const float* pl = (f == 1 || f == 2 || f == 4) ? L.bmax : L.bmin;
+        const int ax = (f == 0 || f == 2) ? 0 : ((f == 1 || f == 3) ? 2 : 1);
+        const float pc = ax == 0 ? p_exit.x : (ax == 1 ? p_exit.y : p_exit.z);
+        if (std::fabs(pc - pl[ax]) < K_EPSILON) face = f;
+      }
+      cnt.reason = face < 0 ? 3 : 4;
+      return detected;
+    }
     cur = next;
   }
+  cnt.reason = (t_entry < t_exit) ? 1 : 0;
   return detected;
 }
 
@@ -226,7 +243,7 @@ ORC_API void orc_trace(const void* nodes, const void* leaves, const void* tris, 
     V3<float> d{ray_d[3 * i], ray_d[3 * i + 1], ray_d[3 * i + 2]};
     float best = K_INFINITY;
     int32_t bp = -1, bo = -1;
-    Counts c{0, 0, 0};
+    Counts c{0, 0, 0, 0};
     for (int k = 0; k < n_obj; ++k) {
       float t = K_INFINITY;
       int32_t p = -1;
@@ -240,9 +257,10 @@ ORC_API void orc_trace(const void* nodes, const void* leaves, const void* tris, 
     t_out[i] = best;
     if (obj_out) obj_out[i] = bo;
     if (counts) {
-      counts[3 * i] = c.entries;
-      counts[3 * i + 1] = c.leaves;
-      counts[3 * i + 2] = c.tris;
+      counts[4 * i] = c.entries;
+      counts[4 * i + 1] = c.leaves;
+      counts[4 * i + 2] = c.tris;
+      counts[4 * i + 3] = c.reason;
     }
   }
 }

@@ -44,7 +44,7 @@ class re_camera(C.Structure):
 
 class re_kd_build_params(C.Structure):
     _fields_ = [("max_leaf_tris", C.c_int32), ("max_depth", C.c_int32), ("cost_traversal", C.c_float),
-                ("cost_intersect", C.c_float), ("empty_bonus", C.c_float), ("n_bins", C.c_int32)]
+                ("cost_intersect", C.c_float), ("empty_bonus", C.c_float), ("n_bins", C.c_int32), ("clip", C.c_int32)]
 
 
 class re_kd_info(C.Structure):

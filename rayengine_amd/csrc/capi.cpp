@@ -59,6 +59,10 @@ RE_API void re_destroy(re_ctx* ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   for (auto& kv : ctx->scenes) free_scene(kv.second);
   for (auto& kv : ctx->volumes) (void)hipFree(kv.second.data);
+  for (auto& kv : ctx->mlps)
+    for (void* p : {(void*)kv.second.params, (void*)kv.second.m1, (void*)kv.second.m2, kv.second.pack,
+                    (void*)kv.second.small, (void*)kv.second.partial, (void*)kv.second.grad, (void*)kv.second.loss})
+      if (p) (void)hipFree(p);
   if (ctx->stage) (void)hipFree(ctx->stage);
   if (ctx->counts_buf) (void)hipFree(ctx->counts_buf);
   for (int i = 0; i < 2 * re_ctx::kRing; ++i)
@@ -305,7 +309,7 @@ re_status trace_common(re_ctx* ctx, int32_t scene_id, const float* ray_o, const 
     HIP_TRY(ctx, hipMemsetAsync(d_counts, 0, sizeof(TraceCounts), ctx->stream), "counts clear");
     L.counts = d_counts;
   }
-  if (L.variant == 0) HIP_TRY(ctx, hipMemsetAsync(L.queue, 0, sizeof(unsigned long long), ctx->stream), "queue clear");
+  if (L.variant == 1) HIP_TRY(ctx, hipMemsetAsync(L.queue, 0, sizeof(unsigned long long), ctx->stream), "queue clear");
   HIP_TRY(ctx, ctx->launch_begin(), "event");
   HIP_TRY(ctx, launch_trace_closest(L, ctx->stream), "trace_closest launch");
   HIP_TRY(ctx, ctx->launch_end(), "event");

@@ -53,21 +53,17 @@ inline float area(const float* lo, const float* hi) {
 const int kFaceAxis[6] = {0, 2, 0, 2, 1, 1};
 const bool kFaceHigh[6] = {false, true, true, false, true, false};
 
+// A triangle reference inside a node: its id and the bounding box of the
+// part of the triangle that lies in the node (the whole triangle's box when
+// clipping is off).
+struct Ref {
+  int32_t prim;
+  float lo[3], hi[3];
+};
+
 class Builder {
  public:
-  Builder(const float* tv, int64_t n, const re_kd_build_params& p) : tv_(tv), n_(n), p_(p) {
-    tb_.resize((size_t)n * 6);
-    for (int64_t i = 0; i < n; ++i) {
-      const float* t = tv + 9 * i;
-      for (int a = 0; a < 3; ++a) {
-        float lo = t[a], hi = t[a];
-        lo = std::min(lo, t[3 + a]); hi = std::max(hi, t[3 + a]);
-        lo = std::min(lo, t[6 + a]); hi = std::max(hi, t[6 + a]);
-        tb_[6 * i + a] = lo;
-        tb_[6 * i + 3 + a] = hi;
-      }
-    }
-  }
+  Builder(const float* tv, int64_t n, const re_kd_build_params& p) : tv_(tv), n_(n), p_(p) {}
 
   std::vector<BNode> nodes;
   float root_min[3], root_max[3];
@@ -77,13 +73,18 @@ class Builder {
       root_min[a] = kInf;
       root_max[a] = -kInf;
     }
-    for (int64_t i = 0; i < n_; ++i)
+    std::vector<Ref> all((size_t)n_);
+    for (int64_t i = 0; i < n_; ++i) {
+      const float* t = tv_ + 9 * i;
+      Ref& r = all[i];
+      r.prim = (int32_t)i;
       for (int a = 0; a < 3; ++a) {
-        root_min[a] = std::min(root_min[a], tb_[6 * i + a]);
-        root_max[a] = std::max(root_max[a], tb_[6 * i + 3 + a]);
+        r.lo[a] = std::min(std::min(t[a], t[3 + a]), t[6 + a]);
+        r.hi[a] = std::max(std::max(t[a], t[3 + a]), t[6 + a]);
+        root_min[a] = std::min(root_min[a], r.lo[a]);
+        root_max[a] = std::max(root_max[a], r.hi[a]);
       }
-    std::vector<int32_t> all((size_t)n_);
-    for (int64_t i = 0; i < n_; ++i) all[i] = (int32_t)i;
+    }
     nodes.reserve((size_t)std::max<int64_t>(16, n_ / 2));
     build(root_min, root_max, std::move(all), 1);
     int ropes[6] = {-1, -1, -1, -1, -1, -1};
@@ -94,7 +95,6 @@ class Builder {
   const float* tv_;
   int64_t n_;
   re_kd_build_params p_;
-  std::vector<float> tb_;
 
   struct Split {
     int axis = -1;
@@ -115,12 +115,12 @@ class Builder {
     return c;
   }
 
-  Split find_split(const float* lo, const float* hi, const std::vector<int32_t>& prims) const {
+  Split find_split(const float* lo, const float* hi, const std::vector<Ref>& refs) const {
     Split best;
     best.cost = INFINITY;
     float sa = area(lo, hi);
     float inv_sa = sa > 0.0f ? 1.0f / sa : 0.0f;
-    const int64_t n = (int64_t)prims.size();
+    const int64_t n = (int64_t)refs.size();
     for (int a = 0; a < 3; ++a) {
       float ext = hi[a] - lo[a];
       if (!(ext > 0.0f)) continue;
@@ -128,9 +128,9 @@ class Builder {
         const int nb = std::max(4, p_.n_bins);
         std::vector<int64_t> cmin(nb, 0), cmax(nb, 0);
         float scale = (float)nb / ext;
-        for (int32_t pi : prims) {
-          float pl = std::min(std::max(tb_[6 * pi + a], lo[a]), hi[a]);
-          float ph = std::min(std::max(tb_[6 * pi + 3 + a], lo[a]), hi[a]);
+        for (const Ref& r : refs) {
+          float pl = std::min(std::max(r.lo[a], lo[a]), hi[a]);
+          float ph = std::min(std::max(r.hi[a], lo[a]), hi[a]);
           int bl = std::min(nb - 1, std::max(0, (int)((pl - lo[a]) * scale)));
           int bh = std::min(nb - 1, std::max(0, (int)((ph - lo[a]) * scale)));
           cmin[bl]++;
@@ -146,22 +146,22 @@ class Builder {
           if (c < best.cost) best = Split{a, pos, c, nl, nr};
         }
       } else {
-        // Exact candidates: every primitive's minimum (it goes above) and the
+        // Exact candidates: every reference's minimum (it goes above) and the
         // float just past every maximum (it goes below).
         std::vector<float> cand;
         cand.reserve(2 * n);
-        for (int32_t pi : prims) {
-          cand.push_back(tb_[6 * pi + a]);
-          cand.push_back(std::nextafter(tb_[6 * pi + 3 + a], INFINITY));
+        for (const Ref& r : refs) {
+          cand.push_back(r.lo[a]);
+          cand.push_back(std::nextafter(r.hi[a], INFINITY));
         }
         std::sort(cand.begin(), cand.end());
         cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
         for (float pos : cand) {
           if (!(pos > lo[a] && pos < hi[a])) continue;
           int64_t nl = 0, nr = 0;
-          for (int32_t pi : prims) {
-            if (tb_[6 * pi + a] < pos) ++nl;
-            if (tb_[6 * pi + 3 + a] >= pos) ++nr;
+          for (const Ref& r : refs) {
+            if (r.lo[a] < pos) ++nl;
+            if (r.hi[a] >= pos) ++nr;
           }
           float c = cost_of(lo, hi, a, pos, nl, nr, inv_sa);
           if (c < best.cost) best = Split{a, pos, c, nl, nr};
@@ -171,7 +171,65 @@ class Builder {
     return best;
   }
 
-  int build(const float* lo, const float* hi, std::vector<int32_t>&& prims, int depth) {
+  // Bounding box of the triangle clipped to the box [lo, hi] grown by a
+  // relative tolerance (Sutherland-Hodgman in double precision), intersected
+  // with [lo, hi].  Returns false when nothing of the triangle lies in it.
+  bool clip_box(int32_t prim, const float* lo, const float* hi, Ref& out) const {
+    double poly[16][3], tmp[16][3];
+    int np = 3;
+    const float* t = tv_ + 9 * (int64_t)prim;
+    for (int v = 0; v < 3; ++v)
+      for (int a = 0; a < 3; ++a) poly[v][a] = t[3 * v + a];
+    for (int a = 0; a < 3 && np > 0; ++a) {
+      const double tol = 1e-6 * (std::fabs((double)lo[a]) + std::fabs((double)hi[a]) + 1e-3);
+      for (int side = 0; side < 2 && np > 0; ++side) {
+        // keep x_a >= lo - tol (side 0) or x_a <= hi + tol (side 1)
+        const double plane = side == 0 ? (double)lo[a] - tol : (double)hi[a] + tol;
+        const double sgn = side == 0 ? 1.0 : -1.0;
+        int nq = 0;
+        for (int v = 0; v < np; ++v) {
+          const double* P = poly[v];
+          const double* Q = poly[(v + 1) % np];
+          const double dp = sgn * (P[a] - plane), dq = sgn * (Q[a] - plane);
+          if (dp >= 0.0) {
+            for (int k = 0; k < 3; ++k) tmp[nq][k] = P[k];
+            ++nq;
+          }
+          if ((dp >= 0.0) != (dq >= 0.0)) {
+            const double s = dp / (dp - dq);
+            for (int k = 0; k < 3; ++k) tmp[nq][k] = P[k] + s * (Q[k] - P[k]);
+            ++nq;
+          }
+        }
+        np = nq;
+        for (int v = 0; v < np; ++v)
+          for (int k = 0; k < 3; ++k) poly[v][k] = tmp[v][k];
+      }
+    }
+    if (np == 0) return false;
+    out.prim = prim;
+    for (int a = 0; a < 3; ++a) {
+      double mn = poly[0][a], mx = poly[0][a];
+      for (int v = 1; v < np; ++v) {
+        mn = std::min(mn, poly[v][a]);
+        mx = std::max(mx, poly[v][a]);
+      }
+      // Round outward so the float box still contains the clipped polygon.
+      float fl = std::nextafter((float)mn, -INFINITY), fh = std::nextafter((float)mx, INFINITY);
+      out.lo[a] = std::max(fl, lo[a]);
+      out.hi[a] = std::min(fh, hi[a]);
+      if (out.lo[a] > out.hi[a]) out.lo[a] = out.hi[a] = std::min(std::max((float)mn, lo[a]), hi[a]);
+    }
+    return true;
+  }
+
+  static std::vector<int32_t> ids(const std::vector<Ref>& refs) {
+    std::vector<int32_t> v(refs.size());
+    for (size_t i = 0; i < refs.size(); ++i) v[i] = refs[i].prim;
+    return v;
+  }
+
+  int build(const float* lo, const float* hi, std::vector<Ref>&& refs, int depth) {
     int id = (int)nodes.size();
     nodes.emplace_back();
     {
@@ -180,32 +238,41 @@ class Builder {
       std::memcpy(b.bmax, hi, sizeof(b.bmax));
       b.depth = depth;
     }
-    const int64_t n = (int64_t)prims.size();
+    const int64_t n = (int64_t)refs.size();
     if (n <= p_.max_leaf_tris || depth >= p_.max_depth) {
-      nodes[id].prims = std::move(prims);
+      nodes[id].prims = ids(refs);
       return id;
     }
-    Split s = find_split(lo, hi, prims);
+    Split s = find_split(lo, hi, refs);
     if (s.axis < 0 || (s.nl == n && s.nr == n)) {
-      nodes[id].prims = std::move(prims);
+      nodes[id].prims = ids(refs);
       return id;
     }
-    std::vector<int32_t> L, R;
-    L.reserve(s.nl);
-    R.reserve(s.nr);
-    for (int32_t pi : prims) {
-      if (tb_[6 * pi + s.axis] < s.pos) L.push_back(pi);
-      if (tb_[6 * pi + 3 + s.axis] >= s.pos) R.push_back(pi);
-    }
-    if ((int64_t)L.size() == n && (int64_t)R.size() == n) {
-      nodes[id].prims = std::move(prims);
-      return id;
-    }
-    std::vector<int32_t>().swap(prims);
     float lhi[3] = {hi[0], hi[1], hi[2]};
     float rlo[3] = {lo[0], lo[1], lo[2]};
     lhi[s.axis] = s.pos;
     rlo[s.axis] = s.pos;
+    std::vector<Ref> L, R;
+    L.reserve(s.nl);
+    R.reserve(s.nr);
+    for (const Ref& r : refs) {
+      const bool to_l = r.lo[s.axis] < s.pos, to_r = r.hi[s.axis] >= s.pos;
+      if (to_l && to_r && p_.clip) {
+        // Straddler: keep it on a side only if the triangle itself reaches
+        // into that child, with the box of the part that does.
+        Ref c;
+        if (clip_box(r.prim, lo, lhi, c)) L.push_back(c);
+        if (clip_box(r.prim, rlo, hi, c)) R.push_back(c);
+        continue;
+      }
+      if (to_l) L.push_back(r);
+      if (to_r) R.push_back(r);
+    }
+    if ((int64_t)L.size() == n && (int64_t)R.size() == n) {
+      nodes[id].prims = ids(refs);
+      return id;
+    }
+    std::vector<Ref>().swap(refs);
     nodes[id].axis = s.axis;
     nodes[id].split = s.pos;
     int c0 = build(lo, lhi, std::move(L), depth + 1);
@@ -265,6 +332,7 @@ RE_API void re_kd_default_params(re_kd_build_params* p) {
   p->cost_intersect = 1.5f;
   p->empty_bonus = 0.8f;
   p->n_bins = 32;
+  p->clip = 1;
 }
 
 RE_API re_status re_kd_build(const float* tri_v, int64_t n_tris, const re_kd_build_params* params,

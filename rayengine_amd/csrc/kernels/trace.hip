@@ -282,7 +282,7 @@ hipError_t launch_trace_closest(const TraceLaunch& L, hipStream_t s) {
   // the rope tolerance of two faces; the visit cap is what ends it, so
   // "unlimited" is a large bound that every ray reaches.
   int32_t cap = (L.cap > 0 && L.cap < kMaxLeafVisits) ? L.cap : kMaxLeafVisits;
-  if (L.variant == 0) {
+  if (L.variant == 1) {
     // Persistent: enough blocks to fill every CU at the kernel's occupancy,
     // never more than the rays need.
     int64_t want = (L.n + block - 1) / block;

@@ -160,8 +160,10 @@ def test_traversal_equals_brute_force_below_cap(small):
     nodes, leaves, tris = tree.export()
     p, t, cnt = oracle.trace(nodes, leaves, tris, tree.identity_object(), o, d, cap=1 << 12, counts=True)
     pb, tb = oracle.brute(tri_v, tree.info["root_min"], tree.info["root_max"], o, d)
-    capped = cnt[:, 1] >= (1 << 12)
-    ok = ~capped
+    # Rays whose walk ended through one of the reference's silent failure
+    # exits (cap, leaf slab miss, no face within K_EPSILON) may miss hits; all
+    # others must equal brute force.
+    ok = np.isin(cnt[:, 3], [0, 4, 5])
     assert ok.mean() > 0.99
     assert np.array_equal(t[ok].view(np.uint32), tb[ok].view(np.uint32))
     # ties at equal t (shared edges) may resolve to another triangle id; none at distinct t
