@@ -340,6 +340,56 @@ RE_API re_status re_hybrid_trace(re_ctx* ctx, int32_t vol_id, int32_t mlp_id,
                                  const float* ray_d, int64_t n, float* t, uint8_t* hit,
                                  int32_t* evals, int32_t* samples, uint32_t flags);
 
+/* ------------------------------------------------------ batched workload
+ *
+ * Many triangle scenes and terrain tiles, each with its own slice of one ray
+ * array, traced by one persistent work-queue kernel: the rays are cut into
+ * items of item_rays, waves take items from a device counter, and the wave
+ * finishing a target's last item stamps its completion time.  The launch
+ * sequence is captured once into a hipGraph and replayed by re_batch_run.
+ * Results equal the single-workload entry points ray for ray.  No
+ * counterpart in the reference (one synchronous launch per frame,
+ * RayEngine/src/cuda-src/renderer.cu:701-738). */
+typedef struct re_batch_scene {
+  const re_kdtree* tree;
+  int64_t ray_begin;
+  int64_t ray_count;
+} re_batch_scene;
+
+typedef struct re_batch_tile {
+  int32_t vol_id;
+  int32_t mode; /* 0 exact-field march, 1 reference sphere trace, 2 surrogate hybrid */
+  int64_t ray_begin;
+  int64_t ray_count;
+} re_batch_tile;
+
+typedef struct re_batch_config {
+  int32_t item_rays; /* rays per work item, default 256 */
+  int32_t block;     /* threads per block (64, 128, 256), default 256 */
+  int32_t mlp_id;    /* network of surrogate tiles, -1 none */
+  re_trace_params trace;
+  re_sdf_params sdf;
+  re_hybrid_params hybrid;
+} re_batch_config;
+
+RE_API void re_batch_default_config(re_batch_config* c);
+/* Uploads the scenes, tiles and rays (host arrays) and captures the graph.
+ * Results of a triangle ray are (prim id, t); of a terrain ray (0 hit / -1
+ * miss, t). */
+RE_API re_status re_batch_create(re_ctx* ctx, const re_batch_scene* scenes, int32_t n_scenes,
+                                 const re_batch_tile* tiles, int32_t n_tiles, const float* ray_o,
+                                 const float* ray_d, int64_t n_rays, const re_batch_config* config,
+                                 int32_t* batch_id);
+/* Replays the captured graph (RE_ASYNC: do not wait). */
+RE_API re_status re_batch_run(re_ctx* ctx, int32_t batch_id, uint32_t flags);
+RE_API re_status re_batch_info(re_ctx* ctx, int32_t batch_id, int64_t* n_rays, int64_t* n_items,
+                               int32_t* n_slots);
+RE_API re_status re_batch_results(re_ctx* ctx, int32_t batch_id, int32_t* prim, float* t);
+/* Completion time of every target of the last run (scenes first, then
+ * tiles), milliseconds after the run started, from device timestamps. */
+RE_API re_status re_batch_completion_ms(re_ctx* ctx, int32_t batch_id, float* ms);
+RE_API re_status re_batch_release(re_ctx* ctx, int32_t batch_id);
+
 /* Elapsed milliseconds of the last kernel launch of the context, measured
  * with HIP events recorded around the launch on its stream (waits for it).
  * Replaces the reference's FPS counter (RayEngine/src/Engine/Window.cpp:46-56). */

@@ -415,6 +415,65 @@ def sample_points(dist_vol: "Volume", n: int, seed: int = 1, near_band: float = 
     return out
 
 
+class Batch:
+    """A batched mixed workload captured in a hipGraph (re_batch_*).
+
+    scenes: list of (KDTree, ray_begin, ray_count); tiles: list of
+    (Volume, mode, ray_begin, ray_count) with mode 0 exact march, 1 reference
+    sphere trace, 2 surrogate hybrid."""
+
+    def __init__(self, ctx: Context, scenes, tiles, ray_o, ray_d, item_rays=256, block=256, surrogate=None,
+                 trace=None, sdf=None, hybrid=None):
+        self.ctx = ctx
+        o, d = _f32c(ray_o, 3), _f32c(ray_d, 3)
+        self.n_rays = o.size // 3
+        sc = (_lib.re_batch_scene * max(len(scenes), 1))()
+        for i, (tree, b, c) in enumerate(scenes):
+            sc[i].tree, sc[i].ray_begin, sc[i].ray_count = tree.handle, int(b), int(c)
+        tl = (_lib.re_batch_tile * max(len(tiles), 1))()
+        for i, (vol, mode, b, c) in enumerate(tiles):
+            tl[i].vol_id, tl[i].mode, tl[i].ray_begin, tl[i].ray_count = vol.id, int(mode), int(b), int(c)
+        cfg = _lib.re_batch_config()
+        lib().re_batch_default_config(C.byref(cfg))
+        cfg.item_rays, cfg.block = int(item_rays), int(block)
+        cfg.mlp_id = surrogate.id if surrogate is not None else -1
+        if trace:
+            cfg.trace = _trace_params(**trace)
+        if sdf:
+            cfg.sdf = sdf_params(**sdf)
+        if hybrid:
+            cfg.hybrid = hybrid_params(**hybrid)
+        bid = C.c_int32()
+        check(lib().re_batch_create(ctx.handle, sc, len(scenes), tl, len(tiles), ptr(o), ptr(d), self.n_rays,
+                                    C.byref(cfg), C.byref(bid)), ctx.handle)
+        self.id = bid.value
+        self.n_slots = len(scenes) + len(tiles)
+
+    def run(self, sync=True):
+        check(lib().re_batch_run(self.ctx.handle, self.id, 0 if sync else RE_ASYNC), self.ctx.handle)
+
+    def results(self):
+        p = np.empty(self.n_rays, np.int32)
+        t = np.empty(self.n_rays, np.float32)
+        check(lib().re_batch_results(self.ctx.handle, self.id, ptr(p), ptr(t)), self.ctx.handle)
+        return p, t
+
+    def completion_ms(self):
+        ms = np.empty(self.n_slots, np.float32)
+        check(lib().re_batch_completion_ms(self.ctx.handle, self.id, ptr(ms)), self.ctx.handle)
+        return ms
+
+    def items(self):
+        n_items = C.c_int64()
+        check(lib().re_batch_info(self.ctx.handle, self.id, None, C.byref(n_items), None), self.ctx.handle)
+        return n_items.value
+
+    def release(self):
+        if self.id is not None:
+            check(lib().re_batch_release(self.ctx.handle, self.id), self.ctx.handle)
+            self.id = None
+
+
 def _trace_params(max_leaf_visits=50, one_sided=True, secondary=False, block=0, variant=0):
     return _lib.re_trace_params(int(max_leaf_visits), int(bool(one_sided)), int(bool(secondary)), int(block),
                                 int(variant))

@@ -85,6 +85,19 @@ class re_hybrid_params(C.Structure):
                 ("max_iters", C.c_int32), ("block", C.c_int32)]
 
 
+class re_batch_scene(C.Structure):
+    _fields_ = [("tree", C.c_void_p), ("ray_begin", C.c_int64), ("ray_count", C.c_int64)]
+
+
+class re_batch_tile(C.Structure):
+    _fields_ = [("vol_id", C.c_int32), ("mode", C.c_int32), ("ray_begin", C.c_int64), ("ray_count", C.c_int64)]
+
+
+class re_batch_config(C.Structure):
+    _fields_ = [("item_rays", C.c_int32), ("block", C.c_int32), ("mlp_id", C.c_int32), ("trace", re_trace_params),
+                ("sdf", re_sdf_params), ("hybrid", re_hybrid_params)]
+
+
 _P = C.c_void_p
 _F = C.POINTER(C.c_float)
 _I32 = C.POINTER(C.c_int32)
@@ -142,6 +155,14 @@ SIGNATURES = {
     "re_hybrid_default_params": (None, [C.POINTER(re_hybrid_params)]),
     "re_hybrid_trace": (C.c_int, [_P, C.c_int32, C.c_int32, C.POINTER(re_hybrid_params), _P, _P, C.c_int64, _P, _P,
                                   _P, _P, C.c_uint32]),
+    "re_batch_default_config": (None, [C.POINTER(re_batch_config)]),
+    "re_batch_create": (C.c_int, [_P, C.POINTER(re_batch_scene), C.c_int32, C.POINTER(re_batch_tile), C.c_int32, _P,
+                                  _P, C.c_int64, C.POINTER(re_batch_config), _I32]),
+    "re_batch_run": (C.c_int, [_P, C.c_int32, C.c_uint32]),
+    "re_batch_info": (C.c_int, [_P, C.c_int32, _I64, _I64, _I32]),
+    "re_batch_results": (C.c_int, [_P, C.c_int32, _P, _P]),
+    "re_batch_completion_ms": (C.c_int, [_P, C.c_int32, _P]),
+    "re_batch_release": (C.c_int, [_P, C.c_int32]),
     "re_last_kernel_ms": (C.c_int, [_P, _F]),
     "re_kernel_times": (C.c_int, [_P, _F, C.c_int32, _I32]),
 }

@@ -59,6 +59,15 @@ RE_API void re_destroy(re_ctx* ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   for (auto& kv : ctx->scenes) free_scene(kv.second);
   for (auto& kv : ctx->volumes) (void)hipFree(kv.second.data);
+  for (auto& kv : ctx->batches) {
+    DevBatch& b = kv.second;
+    if (b.exec) (void)hipGraphExecDestroy(b.exec);
+    if (b.graph) (void)hipGraphDestroy(b.graph);
+    for (void* p : {(void*)b.arena.nodes, (void*)b.arena.leaves, (void*)b.arena.tris, (void*)b.arena.objs,
+                    (void*)b.tiles, (void*)b.ray_o, (void*)b.ray_d, (void*)b.prim, (void*)b.t, (void*)b.hit_scratch,
+                    (void*)b.items, (void*)b.ctl, (void*)b.remaining, (void*)b.remaining_init, (void*)b.done_ts})
+      if (p) (void)hipFree(p);
+  }
   for (auto& kv : ctx->mlps)
     for (void* p : {(void*)kv.second.params, (void*)kv.second.m1, (void*)kv.second.m2, kv.second.pack,
                     (void*)kv.second.small, (void*)kv.second.partial, (void*)kv.second.grad, (void*)kv.second.loss})

@@ -1,0 +1,294 @@
+// C-ABI entry points of the batched mixed workload: one device work queue
+// over many triangle scenes and terrain tiles, its launch sequence captured
+// once into a hipGraph and replayed.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <numeric>
+#include <vector>
+
+#include "capi_util.hpp"
+
+using namespace re;
+
+namespace {
+
+void free_batch(DevBatch& b) {
+  if (b.exec) (void)hipGraphExecDestroy(b.exec);
+  if (b.graph) (void)hipGraphDestroy(b.graph);
+  for (void* p : {(void*)b.arena.nodes, (void*)b.arena.leaves, (void*)b.arena.tris, (void*)b.arena.objs,
+                  (void*)b.tiles, (void*)b.ray_o, (void*)b.ray_d, (void*)b.prim, (void*)b.t, (void*)b.hit_scratch,
+                  (void*)b.items, (void*)b.ctl, (void*)b.remaining, (void*)b.remaining_init, (void*)b.done_ts})
+    if (p) (void)hipFree(p);
+  b = DevBatch{};
+}
+
+}  // namespace
+
+extern "C" {
+
+RE_API void re_batch_default_config(re_batch_config* c) {
+  if (!c) return;
+  c->item_rays = 256;
+  c->block = 256;
+  c->mlp_id = -1;
+  re_trace_default_params(&c->trace);
+  re_sdf_default_params(&c->sdf);
+  re_hybrid_default_params(&c->hybrid);
+}
+
+RE_API re_status re_batch_create(re_ctx* ctx, const re_batch_scene* scenes, int32_t n_scenes,
+                                 const re_batch_tile* tiles, int32_t n_tiles, const float* ray_o, const float* ray_d,
+                                 int64_t n_rays, const re_batch_config* config, int32_t* batch_id) {
+  if (!ctx || !batch_id || n_scenes < 0 || n_tiles < 0 || n_scenes + n_tiles <= 0 || n_rays <= 0 || !ray_o || !ray_d ||
+      (n_scenes > 0 && !scenes) || (n_tiles > 0 && !tiles))
+    return fail(RE_ERR_INVALID, "re_batch_create: bad arguments");
+  re_batch_config cfg;
+  re_batch_default_config(&cfg);
+  if (config) cfg = *config;
+  if (cfg.item_rays < 1 || cfg.item_rays > (1 << 16) || !(cfg.block == 64 || cfg.block == 128 || cfg.block == 256))
+    return set_err(ctx, RE_ERR_INVALID, "re_batch_create: item_rays in [1,65536], block in {64,128,256}");
+  for (int32_t k = 0; k < n_scenes; ++k)
+    if (!scenes[k].tree || scenes[k].ray_begin < 0 || scenes[k].ray_count < 0 ||
+        scenes[k].ray_begin + scenes[k].ray_count > n_rays)
+      return set_err(ctx, RE_ERR_INVALID, "re_batch_create: scene ray range outside the ray arrays");
+  bool any_surrogate = false;
+  for (int32_t k = 0; k < n_tiles; ++k) {
+    const re_batch_tile& tl = tiles[k];
+    if (tl.ray_begin < 0 || tl.ray_count < 0 || tl.ray_begin + tl.ray_count > n_rays || tl.mode < 0 || tl.mode > 2)
+      return set_err(ctx, RE_ERR_INVALID, "re_batch_create: tile ray range or mode invalid");
+    if (ctx->volumes.find(tl.vol_id) == ctx->volumes.end())
+      return set_err(ctx, RE_ERR_NOT_FOUND, "re_batch_create: unknown tile volume");
+    any_surrogate |= tl.mode == 2;
+  }
+  if (any_surrogate && ctx->mlps.find(cfg.mlp_id) == ctx->mlps.end())
+    return set_err(ctx, RE_ERR_NOT_FOUND, "re_batch_create: surrogate tiles need a valid mlp_id");
+  HIP_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+
+  DevBatch b;
+  b.n_scenes = n_scenes;
+  b.n_tiles = n_tiles;
+  b.n_slots = n_scenes + n_tiles;
+  b.n_rays = n_rays;
+  b.mlp_id = cfg.mlp_id;
+  // Scenes: one arena with one object per scene (identity transform).
+  if (n_scenes > 0) {
+    std::vector<re_object> objs(n_scenes);
+    for (int32_t k = 0; k < n_scenes; ++k) {
+      re_object& o = objs[k];
+      o = re_object{};
+      o.tree = scenes[k].tree;
+      o.cos_rot_y = 1.0f;
+    }
+    int32_t sid = 0;
+    re_status st = re_scene_upload(ctx, objs.data(), n_scenes, &sid);
+    if (st) return st;
+    b.arena = ctx->scenes[sid];
+    ctx->scenes.erase(sid);
+  }
+  const size_t rb = (size_t)n_rays * 12;
+  std::vector<BatchTile> htiles(n_tiles);
+  const DevVolume* first_vol = nullptr;
+  for (int32_t k = 0; k < n_tiles; ++k) {
+    const DevVolume& v = ctx->volumes[tiles[k].vol_id];
+    if (!first_vol) first_vol = &v;
+    htiles[k].vol = v.view();
+    for (int a = 0; a < 3; ++a) htiles[k].box_max[a] = v.box_max[a];
+    b.tile_vol.push_back(tiles[k].vol_id);
+    b.tile_mode.push_back(tiles[k].mode);
+  }
+  // Work items, targets in ascending ray count (shortest first).
+  std::vector<int32_t> order(b.n_slots);
+  std::iota(order.begin(), order.end(), 0);
+  auto count_of = [&](int32_t s) { return s < n_scenes ? scenes[s].ray_count : tiles[s - n_scenes].ray_count; };
+  std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t c) { return count_of(a) < count_of(c); });
+  std::vector<BatchItem> items;
+  std::vector<int32_t> rem(b.n_slots, 0);
+  for (int32_t s : order) {
+    int32_t kind;
+    int64_t begin, count;
+    int32_t target;
+    if (s < n_scenes) {
+      kind = kItemTri;
+      target = s;
+      begin = scenes[s].ray_begin;
+      count = scenes[s].ray_count;
+    } else {
+      const re_batch_tile& tl = tiles[s - n_scenes];
+      target = s - n_scenes;
+      begin = tl.ray_begin;
+      count = tl.ray_count;
+      if (tl.mode == 2) {
+        b.surrogate.push_back(SurrogateTile{target, s, begin, count, nullptr});
+        rem[s] = 1;
+        continue;
+      }
+      kind = tl.mode == 0 ? kItemExact : kItemPlain;
+    }
+    for (int64_t off = 0; off < count; off += cfg.item_rays) {
+      const int32_t c = (int32_t)std::min<int64_t>(cfg.item_rays, count - off);
+      items.push_back(BatchItem{kind, target, begin + off, c, s});
+      ++rem[s];
+    }
+  }
+  b.n_items = (int64_t)items.size();
+  hipError_t e = hipSuccess;
+  auto alloc = [&](void** p, size_t bytes) {
+    if (e == hipSuccess) e = hipMalloc(p, bytes > 0 ? bytes : 64);
+  };
+  alloc((void**)&b.tiles, htiles.size() * sizeof(BatchTile));
+  alloc((void**)&b.ray_o, rb);
+  alloc((void**)&b.ray_d, rb);
+  alloc((void**)&b.prim, (size_t)n_rays * 4);
+  alloc((void**)&b.t, (size_t)n_rays * 4);
+  alloc((void**)&b.hit_scratch, (size_t)n_rays);
+  alloc((void**)&b.items, items.size() * sizeof(BatchItem));
+  alloc((void**)&b.ctl, 64);
+  alloc((void**)&b.remaining, (size_t)b.n_slots * 4);
+  alloc((void**)&b.remaining_init, (size_t)b.n_slots * 4);
+  alloc((void**)&b.done_ts, (size_t)b.n_slots * 8);
+  if (e == hipSuccess && !htiles.empty())
+    e = hipMemcpyAsync(b.tiles, htiles.data(), htiles.size() * sizeof(BatchTile), hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(b.ray_o, ray_o, rb, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(b.ray_d, ray_d, rb, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess && !items.empty())
+    e = hipMemcpyAsync(b.items, items.data(), items.size() * sizeof(BatchItem), hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(b.remaining_init, rem.data(), (size_t)b.n_slots * 4, hipMemcpyHostToDevice, ctx->stream);
+  // Rays no item covers keep the miss result.
+  std::vector<int32_t> miss_p((size_t)n_rays, -1);
+  std::vector<float> miss_t((size_t)n_rays, kInf);
+  if (e == hipSuccess) e = hipMemcpyAsync(b.prim, miss_p.data(), (size_t)n_rays * 4, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(b.t, miss_t.data(), (size_t)n_rays * 4, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) {
+    free_batch(b);
+    return hip_err(ctx, e, "re_batch_create");
+  }
+  int rate_khz = 0;
+  if (hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, ctx->device) == hipSuccess && rate_khz > 0)
+    b.wall_clock_khz = (double)rate_khz;
+
+  BatchLaunch& L = b.launch;
+  L.nodes = b.arena.nodes;
+  L.leaves = b.arena.leaves;
+  L.tris = b.arena.tris;
+  L.objs = b.arena.objs;
+  L.tiles = b.tiles;
+  L.items = b.items;
+  L.n_items = b.n_items;
+  L.ray_o = b.ray_o;
+  L.ray_d = b.ray_d;
+  L.prim = b.prim;
+  L.t = b.t;
+  L.ctl = b.ctl;
+  L.remaining = b.remaining;
+  L.remaining_init = b.remaining_init;
+  L.done_ts = b.done_ts;
+  L.n_slots = b.n_slots;
+  L.cap = (cfg.trace.max_leaf_visits > 0 && cfg.trace.max_leaf_visits < kMaxLeafVisits) ? cfg.trace.max_leaf_visits
+                                                                                          : kMaxLeafVisits;
+  L.one_sided = cfg.trace.one_sided;
+  L.secondary = cfg.trace.secondary;
+  L.sdf = BatchSdf{cfg.sdf.max_steps, cfg.sdf.eps, cfg.sdf.step_scale, cfg.sdf.step_rule};
+  const float vox = first_vol ? first_vol->spacing[0] : 1.0f;
+  // Exact-march distances are given in voxels of each tile; tiles of one
+  // batch share the voxel size of the first tile.
+  b.hp = HybridParams{cfg.hybrid.tau_voxels * vox, cfg.hybrid.margin_voxels * vox, cfg.hybrid.k_exact,
+                      cfg.hybrid.h_min_voxels * vox, cfg.hybrid.rejoin_voxels * vox, cfg.hybrid.n_bisect,
+                      cfg.hybrid.max_iters};
+  L.hp = b.hp;
+  L.block = cfg.block;
+  L.grid_blocks = batch_grid(cfg.block);
+
+  // Capture reset + persistent kernel (+ surrogate tiles) once.
+  e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) e = launch_batch_reset(L, ctx->stream);
+  if (e == hipSuccess) e = launch_batch(L, ctx->stream);
+  for (const SurrogateTile& st : b.surrogate) {
+    if (e != hipSuccess) break;
+    const DevVolume& v = ctx->volumes[b.tile_vol[st.tile]];
+    const DevMlp& m = ctx->mlps[b.mlp_id];
+    HybridParams hp = b.hp;
+    const float tv = v.spacing[0];
+    hp.tau = cfg.hybrid.tau_voxels * tv;
+    hp.margin = cfg.hybrid.margin_voxels * tv;
+    hp.h_min = cfg.hybrid.h_min_voxels * tv;
+    hp.rejoin = cfg.hybrid.rejoin_voxels * tv;
+    e = launch_hybrid_trace(v, m.pack, m.small, b.ray_o + 3 * st.begin, b.ray_d + 3 * st.begin, st.count, hp, true,
+                            b.t + st.begin, b.hit_scratch + st.begin, nullptr, nullptr, ctx->stream);
+    if (e == hipSuccess) e = launch_hit_to_prim(b.hit_scratch + st.begin, b.prim + st.begin, st.count, ctx->stream);
+    if (e == hipSuccess) e = launch_batch_stamp(b.done_ts, st.slot, b.remaining, ctx->stream);
+  }
+  hipGraph_t g = nullptr;
+  hipError_t e2 = hipStreamEndCapture(ctx->stream, &g);
+  if (e == hipSuccess) e = e2;
+  b.graph = g;
+  if (e == hipSuccess) e = hipGraphInstantiate(&b.exec, b.graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    free_batch(b);
+    return hip_err(ctx, e, "re_batch_create graph capture");
+  }
+  int32_t id = ctx->next_id++;
+  ctx->batches[id] = b;
+  *batch_id = id;
+  return RE_OK;
+}
+
+RE_API re_status re_batch_run(re_ctx* ctx, int32_t batch_id, uint32_t flags) {
+  if (!ctx) return fail(RE_ERR_INVALID, "re_batch_run: null ctx");
+  auto it = ctx->batches.find(batch_id);
+  if (it == ctx->batches.end()) return set_err(ctx, RE_ERR_NOT_FOUND, "re_batch_run: unknown batch");
+  HIP_TRY(ctx, ctx->launch_begin(), "event");
+  HIP_TRY(ctx, hipGraphLaunch(it->second.exec, ctx->stream), "graph launch");
+  HIP_TRY(ctx, ctx->launch_end(), "event");
+  if (!(flags & RE_ASYNC)) HIP_TRY(ctx, hipStreamSynchronize(ctx->stream), "batch sync");
+  return RE_OK;
+}
+
+RE_API re_status re_batch_info(re_ctx* ctx, int32_t batch_id, int64_t* n_rays, int64_t* n_items, int32_t* n_slots) {
+  if (!ctx) return fail(RE_ERR_INVALID, "re_batch_info: null ctx");
+  auto it = ctx->batches.find(batch_id);
+  if (it == ctx->batches.end()) return set_err(ctx, RE_ERR_NOT_FOUND, "re_batch_info: unknown batch");
+  if (n_rays) *n_rays = it->second.n_rays;
+  if (n_items) *n_items = it->second.n_items;
+  if (n_slots) *n_slots = it->second.n_slots;
+  return RE_OK;
+}
+
+RE_API re_status re_batch_results(re_ctx* ctx, int32_t batch_id, int32_t* prim, float* t) {
+  if (!ctx || !prim || !t) return fail(RE_ERR_INVALID, "re_batch_results: null argument");
+  auto it = ctx->batches.find(batch_id);
+  if (it == ctx->batches.end()) return set_err(ctx, RE_ERR_NOT_FOUND, "re_batch_results: unknown batch");
+  const DevBatch& b = it->second;
+  HIP_TRY(ctx, hipMemcpyAsync(prim, b.prim, (size_t)b.n_rays * 4, hipMemcpyDeviceToHost, ctx->stream), "copy prim");
+  HIP_TRY(ctx, hipMemcpyAsync(t, b.t, (size_t)b.n_rays * 4, hipMemcpyDeviceToHost, ctx->stream), "copy t");
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream), "results sync");
+  return RE_OK;
+}
+
+RE_API re_status re_batch_completion_ms(re_ctx* ctx, int32_t batch_id, float* ms) {
+  if (!ctx || !ms) return fail(RE_ERR_INVALID, "re_batch_completion_ms: null argument");
+  auto it = ctx->batches.find(batch_id);
+  if (it == ctx->batches.end()) return set_err(ctx, RE_ERR_NOT_FOUND, "re_batch_completion_ms: unknown batch");
+  const DevBatch& b = it->second;
+  std::vector<unsigned long long> ts((size_t)b.n_slots), ctl(8);
+  HIP_TRY(ctx, hipMemcpyAsync(ts.data(), b.done_ts, (size_t)b.n_slots * 8, hipMemcpyDeviceToHost, ctx->stream), "copy ts");
+  HIP_TRY(ctx, hipMemcpyAsync(ctl.data(), b.ctl, 64, hipMemcpyDeviceToHost, ctx->stream), "copy ctl");
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream), "completion sync");
+  for (int32_t s = 0; s < b.n_slots; ++s)
+    ms[s] = ts[s] >= ctl[1] ? (float)((double)(ts[s] - ctl[1]) / b.wall_clock_khz) : 0.0f;
+  return RE_OK;
+}
+
+RE_API re_status re_batch_release(re_ctx* ctx, int32_t batch_id) {
+  if (!ctx) return fail(RE_ERR_INVALID, "re_batch_release: null ctx");
+  auto it = ctx->batches.find(batch_id);
+  if (it == ctx->batches.end()) return set_err(ctx, RE_ERR_NOT_FOUND, "re_batch_release: unknown batch");
+  (void)hipStreamSynchronize(ctx->stream);
+  free_batch(it->second);
+  ctx->batches.erase(it);
+  return RE_OK;
+}
+
+}  // extern "C"

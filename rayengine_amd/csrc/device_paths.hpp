@@ -1,0 +1,174 @@
+// Per-ray device paths shared by the single-workload kernels and the batched
+// work-queue kernel, so a ray gives bit-identical results whichever kernel
+// runs it.
+#pragma once
+
+#include "internal.hpp"
+
+namespace re {
+
+struct TriGeom {
+  f3 v0, e1, e2;
+};
+
+__device__ __forceinline__ TriGeom load_geom(const TriRec* __restrict__ tp, uint32_t j) {
+  const float* f = tp[j].v0;
+  return TriGeom{f3{f[0], f[1], f[2]}, f3{f[3], f[4], f[5]}, f3{f[6], f[7], f[8]}};
+}
+
+// Tests a leaf's triangles in record order (the reference's in-leaf order,
+// ray_functions.cuh:295-324).  The geometry of up to four records is loaded
+// before any of them is tested, so a lane has four independent loads in
+// flight instead of one; the prim id is read only for an accepted hit.
+template <bool ONE_SIDED, bool COUNT>
+__device__ __forceinline__ void test_leaf(const TriRec* __restrict__ tp, uint32_t count, f3 ro, f3 rd,
+                                          float& t_exit, bool& found, float& hit_t, int32_t& hit_prim,
+                                          unsigned& c_tri) {
+  for (uint32_t j = 0; j < count; j += 4) {
+    TriGeom g[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (j + q < count) g[q] = load_geom(tp, j + q);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (j + q >= count) break;
+      if (COUNT) ++c_tri;
+      float t;
+      const bool h = ONE_SIDED ? ray_tri(ro, rd, g[q].v0, g[q].e1, g[q].e2, t)
+                               : ray_tri_2s(ro, rd, g[q].v0, g[q].e1, g[q].e2, t);
+      if (h && t < t_exit) {
+        found = true;
+        hit_t = t;
+        hit_prim = tp[j + q].prim;
+        t_exit = t;
+      }
+    }
+  }
+}
+
+struct RayCounts {
+  unsigned ent = 0, leaf = 0, tri = 0;
+};
+
+// Closest hit of one ray over objects [k0, k1): stackless_kdtree_traversal and
+// trace_scene, RayEngine/src/cuda-src/ray_functions.cuh:230-396.
+template <bool COUNT, bool ONE_SIDED>
+__device__ __forceinline__ void trace_ray(const NodeEntry* __restrict__ nodes, const LeafRec* __restrict__ leaves,
+                                          const TriRec* __restrict__ tris, const ObjXform* __restrict__ objs,
+                                          int32_t k0, int32_t k1, f3 o, f3 d, int32_t cap, bool secondary,
+                                          float& best_t, int32_t& best_prim, int32_t& best_obj, RayCounts& rc) {
+  best_t = kInf;
+  best_prim = -1;
+  best_obj = -1;
+  for (int32_t k = k0; k < k1; ++k) {
+    const ObjXform ob = objs[k];
+    f3 ro, rd;
+    xform_ray(ob, o, d, secondary, ro, rd);
+    float t_entry, t_exit;
+    if (!ray_box(ob.root_min, ob.root_max, ro, rd, t_entry, t_exit)) continue;
+    int32_t cur = ob.root;
+    bool found = false;
+    float hit_t = kInf;
+    int32_t hit_prim = -1;
+    int32_t visits = 0;
+    while (t_entry < t_exit && visits < cap) {
+      ++visits;
+      const f3 pe = add(ro, smul(t_entry, rd));
+      while (!ref_is_leaf(cur)) {
+        const NodeEntry e = nodes[cur];
+        if (COUNT) ++rc.ent;
+        const uint32_t ax = e.meta >> 30;
+        const int32_t other = (int32_t)(e.meta & 0x3fffffffu);
+        if (ax == 3u) {
+          cur = leaf_ref((uint32_t)other);
+          break;
+        }
+        cur = (get(pe, (int)ax) < e.split) ? cur + 1 : other;
+      }
+      const LeafRec L = leaves[ref_leaf_id(cur)];
+      if (COUNT) ++rc.leaf;
+      test_leaf<ONE_SIDED, COUNT>(tris + L.first, L.count, ro, rd, t_exit, found, hit_t, hit_prim, rc.tri);
+      float tn, tf;
+      if (!ray_box(L.bmin, L.bmax, ro, rd, tn, tf)) break;
+      t_entry = tf;
+      const f3 px = add(ro, smul(t_entry, rd));
+      const int face = exit_face(L.bmin, L.bmax, px);
+      if (face < 0) break;
+      const int32_t r = L.rope[face];
+      if (r == kNoRope) break;
+      cur = r;
+    }
+    if (found && hit_t < best_t) {
+      best_t = hit_t;
+      best_prim = hit_prim;
+      best_obj = k;
+    }
+  }
+}
+
+// Reference sphere-trace march (single_ray_sphere_trace,
+// RayEngine/src/cuda-src/sphere_tracing.cuh:182-216) clipped to the volume
+// box: sample, step, then test d <= eps * t on the stepped t.
+__device__ __forceinline__ bool plain_march(const VolView& V, const float* bmin, const float* bmax, f3 o, f3 d,
+                                            int32_t max_steps, float eps, float step_scale, int32_t step_rule,
+                                            float& t_out, int32_t& steps) {
+  float t_near, t_far;
+  steps = 0;
+  t_out = kInf;
+  if (!ray_box(bmin, bmax, o, d, t_near, t_far)) return false;
+  float t = t_near;
+  while (t < t_far && steps < max_steps) {
+    const f3 p = add(o, smul(t, d));
+    const float dist = trilinear(V, p);
+    const float k = step_rule == 0 ? step_scale : fmaxf(0.45f, fminf(t * 0.6f, 0.8f));
+    t += dist * k;
+    ++steps;
+    if (dist <= eps * t) {
+      t_out = t;
+      return true;
+    }
+  }
+  return false;
+}
+
+// Exact-field march: from the volume entry, step max(f * k_exact, h_min)
+// while the trilinear field stays positive; bisect the first sign change
+// n_bisect times.  A ray entering at f <= 0 hits at the entry.
+__device__ __forceinline__ bool exact_march(const VolView& V, const float* bmin, const float* bmax, f3 o, f3 d,
+                                            float k_exact, float h_min, int32_t n_bisect, int32_t max_iters,
+                                            float& t_out, int32_t& samples) {
+  float tn, tf;
+  samples = 0;
+  t_out = kInf;
+  if (!ray_box(bmin, bmax, o, d, tn, tf)) return false;
+  float t = tn;
+  float f_prev = trilinear(V, add(o, smul(t, d)));
+  ++samples;
+  if (f_prev <= 0.0f) {
+    t_out = t;
+    return true;
+  }
+  for (int it = 0; it < max_iters; ++it) {
+    const float h = fmaxf(f_prev * k_exact, h_min);
+    const float t_next = t + h;
+    const float f_next = trilinear(V, add(o, smul(t_next, d)));
+    ++samples;
+    if (f_next <= 0.0f) {
+      float a = t, b = t_next;
+      for (int k = 0; k < n_bisect; ++k) {
+        const float mid = 0.5f * (a + b);
+        const float fm = trilinear(V, add(o, smul(mid, d)));
+        ++samples;
+        if (fm > 0.0f) a = mid; else b = mid;
+      }
+      t_out = 0.5f * (a + b);
+      return true;
+    }
+    t = t_next;
+    f_prev = f_next;
+    if (t >= tf) return false;
+  }
+  return false;
+}
+
+}  // namespace re

@@ -143,6 +143,92 @@ struct HybridParams {
 hipError_t launch_hybrid_trace(const DevVolume& field, const void* pack, const float* small, const float* ray_o,
                                const float* ray_d, int64_t n, const HybridParams& hp, bool use_mlp, float* t,
                                uint8_t* hit, int32_t* evals, int32_t* samples, hipStream_t s);
+
+// ---- batched work queue (kernels/batch.hip) ----
+enum BatchKind : int32_t { kItemTri = 0, kItemExact = 1, kItemPlain = 2 };
+
+struct BatchItem {
+  int32_t kind;    // BatchKind
+  int32_t target;  // scene object index (triangles) or tile index (terrain)
+  int64_t begin;   // first ray
+  int32_t count;   // rays in the item
+  int32_t slot;    // completion slot (scenes first, then tiles)
+};
+
+struct BatchTile {
+  VolView vol;
+  float box_max[3];
+};
+
+struct BatchSdf {
+  int32_t max_steps;
+  float eps, step_scale;
+  int32_t step_rule;
+};
+
+struct BatchLaunch {
+  const NodeEntry* nodes;
+  const LeafRec* leaves;
+  const TriRec* tris;
+  const ObjXform* objs;
+  const BatchTile* tiles;
+  const BatchItem* items;
+  int64_t n_items;
+  const float* ray_o;
+  const float* ray_d;
+  int32_t* prim;
+  float* t;
+  unsigned long long* ctl;           // [0] item counter, [1] start timestamp
+  int32_t* remaining;                // per slot: items left
+  const int32_t* remaining_init;
+  unsigned long long* done_ts;       // per slot completion timestamp
+  int32_t n_slots;
+  int32_t cap, one_sided, secondary;
+  BatchSdf sdf;
+  HybridParams hp;                   // exact-march parameters (world units)
+  int32_t block;
+  int64_t grid_blocks;
+};
+
+// A batch resident on the device with its launch sequence captured once.
+struct SurrogateTile {
+  int32_t tile, slot;
+  int64_t begin, count;
+  uint8_t* hit;  // scratch hit flags, converted into prim by the stamp step
+};
+
+struct DevBatch {
+  DevScene arena;  // every scene's tree, one object per scene
+  BatchTile* tiles = nullptr;
+  std::vector<int32_t> tile_vol;  // volume id per tile
+  std::vector<int32_t> tile_mode;
+  float* ray_o = nullptr;
+  float* ray_d = nullptr;
+  int64_t n_rays = 0;
+  int32_t* prim = nullptr;
+  float* t = nullptr;
+  uint8_t* hit_scratch = nullptr;
+  BatchItem* items = nullptr;
+  int64_t n_items = 0;
+  unsigned long long* ctl = nullptr;
+  int32_t* remaining = nullptr;
+  int32_t* remaining_init = nullptr;
+  unsigned long long* done_ts = nullptr;
+  int32_t n_slots = 0, n_scenes = 0, n_tiles = 0;
+  std::vector<SurrogateTile> surrogate;
+  int32_t mlp_id = -1;
+  BatchLaunch launch;
+  HybridParams hp;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  double wall_clock_khz = 100000.0;
+};
+
+hipError_t launch_batch_reset(const BatchLaunch& L, hipStream_t s);
+hipError_t launch_hit_to_prim(const uint8_t* hit, int32_t* prim, int64_t n, hipStream_t s);
+hipError_t launch_batch(const BatchLaunch& L, hipStream_t s);
+hipError_t launch_batch_stamp(unsigned long long* done_ts, int32_t slot, int32_t* remaining, hipStream_t s);
+int64_t batch_grid(int block);
 hipError_t launch_mlp_infer(const void* pack, const float* small, const float* pts, int64_t n, float* out,
                             hipStream_t s);
 hipError_t launch_mlp_train(const void* pack, const float* small, const float4* samples, int64_t n_samples,
@@ -169,6 +255,7 @@ struct re_ctx {
   std::map<int32_t, re::DevScene> scenes;
   std::map<int32_t, re::DevVolume> volumes;
   std::map<int32_t, re::DevMlp> mlps;
+  std::map<int32_t, re::DevBatch> batches;
   // Staging for host-pointer calls, grown on demand.
   void* counts_buf = nullptr;  // 256 bytes: work counters at 0, ray queue counter at 64
   int64_t persistent_grid[3] = {0, 0, 0};  // per block size 64, 128, 256

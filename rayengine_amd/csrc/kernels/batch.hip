@@ -1,0 +1,126 @@
+// Batched mixed workload in one persistent launch: many triangle scenes and
+// terrain tiles, each with its own ray batch, cut into work items of a few
+// dozen to a few hundred rays.  Every wave takes the next item from a device
+// counter and runs its rays with the same per-ray paths as the single-
+// workload kernels (so batched results equal unbatched ones bit for bit);
+// the wave that finishes a target's last item stamps its completion time.
+// The launch sequence (reset, persistent kernel) is captured once in a
+// hipGraph and replayed.  No counterpart in the reference, which launched
+// one kernel per frame and synchronised after each (renderer.cu:701-738).
+#include "../device_paths.hpp"
+
+namespace re {
+namespace {
+
+__global__ void batch_reset_kernel(unsigned long long* __restrict__ ctl, int32_t* __restrict__ remaining,
+                                   const int32_t* __restrict__ remaining_init,
+                                   unsigned long long* __restrict__ done_ts, int32_t n_slots) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) {
+    ctl[0] = 0ull;
+    ctl[1] = wall_clock64();
+  }
+  if (i < n_slots) {
+    remaining[i] = remaining_init[i];
+    done_ts[i] = 0ull;
+  }
+}
+
+__global__ void batch_stamp_kernel(unsigned long long* __restrict__ done_ts, int32_t slot,
+                                   int32_t* __restrict__ remaining) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    remaining[slot] = 0;
+    done_ts[slot] = wall_clock64();
+  }
+}
+
+__global__ void hit_to_prim_kernel(const uint8_t* __restrict__ hit, int32_t* __restrict__ prim, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) prim[i] = hit[i] ? 0 : -1;
+}
+
+template <bool ONE_SIDED>
+__global__ void __launch_bounds__(256) batch_kernel(BatchLaunch L) {
+  const int lane = threadIdx.x & 63;
+  for (;;) {
+    unsigned long long it = 0;
+    if (lane == 0) it = atomicAdd(&L.ctl[0], 1ull);
+    it = __shfl(it, 0);
+    if ((int64_t)it >= L.n_items) break;
+    const BatchItem item = L.items[it];
+    for (int r = lane; r < item.count; r += 64) {
+      const int64_t i = item.begin + r;
+      const f3 o{L.ray_o[3 * i], L.ray_o[3 * i + 1], L.ray_o[3 * i + 2]};
+      const f3 d{L.ray_d[3 * i], L.ray_d[3 * i + 1], L.ray_d[3 * i + 2]};
+      if (item.kind == kItemTri) {
+        float bt;
+        int32_t bp, bo;
+        RayCounts rc;
+        trace_ray<false, ONE_SIDED>(L.nodes, L.leaves, L.tris, L.objs, item.target, item.target + 1, o, d, L.cap,
+                                    L.secondary != 0, bt, bp, bo, rc);
+        L.prim[i] = bp;
+        L.t[i] = bt;
+      } else {
+        const BatchTile tile = L.tiles[item.target];
+        const float bmin[3] = {0.0f, 0.0f, 0.0f};
+        float t;
+        int32_t n;
+        bool h;
+        if (item.kind == kItemExact)
+          h = exact_march(tile.vol, bmin, tile.box_max, o, d, L.hp.k_exact, L.hp.h_min, L.hp.n_bisect,
+                          L.hp.max_iters, t, n);
+        else
+          h = plain_march(tile.vol, bmin, tile.box_max, o, d, L.sdf.max_steps, L.sdf.eps, L.sdf.step_scale,
+                          L.sdf.step_rule, t, n);
+        L.prim[i] = h ? 0 : -1;
+        L.t[i] = h ? t : kInf;
+      }
+    }
+    if (lane == 0) {
+      if (atomicSub(&L.remaining[item.slot], 1) == 1) L.done_ts[item.slot] = wall_clock64();
+    }
+  }
+}
+
+}  // namespace
+
+int64_t batch_grid(int block) {
+  int dev = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel<true>, block, 0) != hipSuccess) return 0;
+  return (int64_t)cus * (per_cu > 0 ? per_cu : 1);
+}
+
+hipError_t launch_batch_reset(const BatchLaunch& L, hipStream_t s) {
+  const int n = L.n_slots > 1 ? L.n_slots : 1;
+  hipLaunchKernelGGL(batch_reset_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, L.ctl, L.remaining,
+                     L.remaining_init, L.done_ts, L.n_slots);
+  return hipGetLastError();
+}
+
+hipError_t launch_hit_to_prim(const uint8_t* hit, int32_t* prim, int64_t n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(hit_to_prim_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, hit, prim, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_stamp(unsigned long long* done_ts, int32_t slot, int32_t* remaining, hipStream_t s) {
+  hipLaunchKernelGGL(batch_stamp_kernel, dim3(1), dim3(64), 0, s, done_ts, slot, remaining);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch(const BatchLaunch& L, hipStream_t s) {
+  if (L.n_items <= 0) return hipSuccess;
+  const int block = L.block > 0 ? L.block : 256;
+  int64_t grid = L.grid_blocks > 0 ? L.grid_blocks : 1024;
+  const int64_t want = (L.n_items * 64 + block - 1) / block;
+  if (grid > want) grid = want;
+  if (L.one_sided)
+    hipLaunchKernelGGL(batch_kernel<true>, dim3((unsigned)grid), dim3(block), 0, s, L);
+  else
+    hipLaunchKernelGGL(batch_kernel<false>, dim3((unsigned)grid), dim3(block), 0, s, L);
+  return hipGetLastError();
+}
+
+}  // namespace re

@@ -5,7 +5,7 @@
 // FP64 forward/backward checks these kernels.
 //
 // MFMA layout and the register chaining of layers: csrc/mlp_layout.hpp.
-#include "../internal.hpp"
+#include "../device_paths.hpp"
 #include "../mlp_layout.hpp"
 
 namespace re {
@@ -521,6 +521,19 @@ __global__ void __launch_bounds__(256) hybrid_trace_kernel(
   const f3 o{ray_o[3 * ic], ray_o[3 * ic + 1], ray_o[3 * ic + 2]};
   const f3 d{ray_d[3 * ic], ray_d[3 * ic + 1], ray_d[3 * ic + 2]};
   const float bmin[3] = {0.0f, 0.0f, 0.0f}, bmax[3] = {bx1, by1, bz1};
+  if (!USE_MLP) {
+    // Without the network: the exact-field march from the volume entry.
+    if (i < n) {
+      float t;
+      int32_t samples;
+      const bool h = exact_march(V, bmin, bmax, o, d, hp.k_exact, hp.h_min, hp.n_bisect, hp.max_iters, t, samples);
+      t_out[i] = t;
+      hit_out[i] = h ? 1 : 0;
+      if (evals_out) evals_out[i] = 0;
+      if (samples_out) samples_out[i] = samples;
+    }
+    return;
+  }
   enum { SURR = 0, EXACT = 1, DONE = 2 };
   int phase = DONE;
   bool hit = false;

@@ -14,7 +14,7 @@
 // samples.  Normals by forward differences of the sampled field with
 // delta = min(0.001 t, 0.002) (compute_sdf_normal,
 // RayEngine/src/cuda-src/sdf_functions.cuh:506-516).
-#include "../internal.hpp"
+#include "../device_paths.hpp"
 #include "../terrain_math.hpp"
 
 namespace re {
@@ -59,24 +59,9 @@ __global__ void __launch_bounds__(256) sdf_trace_kernel(VolView V, float bx0, fl
   const f3 o{ray_o[3 * i], ray_o[3 * i + 1], ray_o[3 * i + 2]};
   const f3 d{ray_d[3 * i], ray_d[3 * i + 1], ray_d[3 * i + 2]};
   const float bmin[3] = {bx0, by0, bz0}, bmax[3] = {bx1, by1, bz1};
-  float t_near, t_far;
-  bool hit = false;
   int32_t steps = 0;
   float t = kInf;
-  if (ray_box(bmin, bmax, o, d, t_near, t_far)) {
-    t = t_near;
-    while (t < t_far && steps < max_steps) {
-      const f3 p = add(o, smul(t, d));
-      const float dist = trilinear(V, p);
-      const float k = step_rule == 0 ? step_scale : fmaxf(0.45f, fminf(t * 0.6f, 0.8f));
-      t += dist * k;
-      ++steps;
-      if (dist <= eps * t) {
-        hit = true;
-        break;
-      }
-    }
-  }
+  const bool hit = plain_march(V, bmin, bmax, o, d, max_steps, eps, step_scale, step_rule, t, steps);
   t_out[i] = hit ? t : kInf;
   hit_out[i] = hit ? 1 : 0;
   if (steps_out) steps_out[i] = steps;

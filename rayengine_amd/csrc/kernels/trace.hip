@@ -11,49 +11,10 @@
 // One ray per lane; the node/leaf/triangle records are laid out for single
 // wide loads (8-byte entries, 64-byte leaves, 40-byte triangles in leaf
 // order), so each step of the pointer chase is one or a few cache lines.
-#include "../internal.hpp"
+#include "../device_paths.hpp"
 
 namespace re {
 namespace {
-
-struct TriGeom {
-  f3 v0, e1, e2;
-};
-
-__device__ __forceinline__ TriGeom load_geom(const TriRec* __restrict__ tp, uint32_t j) {
-  const float* f = tp[j].v0;
-  return TriGeom{f3{f[0], f[1], f[2]}, f3{f[3], f[4], f[5]}, f3{f[6], f[7], f[8]}};
-}
-
-// Tests a leaf's triangles in record order (the reference's in-leaf order,
-// ray_functions.cuh:295-324).  The geometry of up to four records is loaded
-// before any of them is tested, so a lane has four independent loads in
-// flight instead of one; the prim id is read only for an accepted hit.
-template <bool ONE_SIDED, bool COUNT>
-__device__ __forceinline__ void test_leaf(const TriRec* __restrict__ tp, uint32_t count, f3 ro, f3 rd,
-                                          float& t_exit, bool& found, float& hit_t, int32_t& hit_prim,
-                                          unsigned& c_tri) {
-  for (uint32_t j = 0; j < count; j += 4) {
-    TriGeom g[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q)
-      if (j + q < count) g[q] = load_geom(tp, j + q);
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      if (j + q >= count) break;
-      if (COUNT) ++c_tri;
-      float t;
-      const bool h = ONE_SIDED ? ray_tri(ro, rd, g[q].v0, g[q].e1, g[q].e2, t)
-                               : ray_tri_2s(ro, rd, g[q].v0, g[q].e1, g[q].e2, t);
-      if (h && t < t_exit) {
-        found = true;
-        hit_t = t;
-        hit_prim = tp[j + q].prim;
-        t_exit = t;
-      }
-    }
-  }
-}
 
 template <bool COUNT, bool ONE_SIDED>
 __global__ void __launch_bounds__(256) trace_closest_kernel(
@@ -66,61 +27,18 @@ __global__ void __launch_bounds__(256) trace_closest_kernel(
   if (i >= n) return;
   const f3 o{ray_o[3 * i], ray_o[3 * i + 1], ray_o[3 * i + 2]};
   const f3 d{ray_d[3 * i], ray_d[3 * i + 1], ray_d[3 * i + 2]};
-  float best_t = kInf;
-  int32_t best_prim = -1, best_obj = -1;
-  unsigned c_ent = 0, c_leaf = 0, c_tri = 0;
-
-  for (int32_t k = 0; k < n_obj; ++k) {
-    const ObjXform ob = objs[k];
-    f3 ro, rd;
-    xform_ray(ob, o, d, secondary != 0, ro, rd);
-    float t_entry, t_exit;
-    if (!ray_box(ob.root_min, ob.root_max, ro, rd, t_entry, t_exit)) continue;
-    int32_t cur = ob.root;
-    bool found = false;
-    float hit_t = kInf;
-    int32_t hit_prim = -1;
-    int32_t visits = 0;
-    while (t_entry < t_exit && visits < cap) {
-      ++visits;
-      const f3 pe = add(ro, smul(t_entry, rd));
-      while (!ref_is_leaf(cur)) {
-        const NodeEntry e = nodes[cur];
-        if (COUNT) ++c_ent;
-        const uint32_t ax = e.meta >> 30;
-        const int32_t other = (int32_t)(e.meta & 0x3fffffffu);
-        if (ax == 3u) {
-          cur = leaf_ref((uint32_t)other);
-          break;
-        }
-        cur = (get(pe, (int)ax) < e.split) ? cur + 1 : other;
-      }
-      const LeafRec L = leaves[ref_leaf_id(cur)];
-      if (COUNT) ++c_leaf;
-      test_leaf<ONE_SIDED, COUNT>(tris + L.first, L.count, ro, rd, t_exit, found, hit_t, hit_prim, c_tri);
-      float tn, tf;
-      if (!ray_box(L.bmin, L.bmax, ro, rd, tn, tf)) break;
-      t_entry = tf;
-      const f3 px = add(ro, smul(t_entry, rd));
-      const int face = exit_face(L.bmin, L.bmax, px);
-      if (face < 0) break;
-      const int32_t r = L.rope[face];
-      if (r == kNoRope) break;
-      cur = r;
-    }
-    if (found && hit_t < best_t) {
-      best_t = hit_t;
-      best_prim = hit_prim;
-      best_obj = k;
-    }
-  }
+  float best_t;
+  int32_t best_prim, best_obj;
+  RayCounts rc;
+  trace_ray<COUNT, ONE_SIDED>(nodes, leaves, tris, objs, 0, n_obj, o, d, cap, secondary != 0, best_t, best_prim,
+                              best_obj, rc);
   prim_out[i] = best_prim;
   t_out[i] = best_t;
   if (obj_out) obj_out[i] = best_obj;
   if (COUNT) {
-    atomicAdd(&counts->entries, (unsigned long long)c_ent);
-    atomicAdd(&counts->leaves, (unsigned long long)c_leaf);
-    atomicAdd(&counts->tris, (unsigned long long)c_tri);
+    atomicAdd(&counts->entries, (unsigned long long)rc.ent);
+    atomicAdd(&counts->leaves, (unsigned long long)rc.leaf);
+    atomicAdd(&counts->tris, (unsigned long long)rc.tri);
     if (best_prim >= 0) atomicAdd(&counts->rays_hit, 1ull);
   }
 }

@@ -51,3 +51,53 @@ TERRAIN_SPACING = (1.0 / 256, 1.0 / 256, 1.0 / 256)
 
 def terrain_rays(width: int = 1920, height: int = 1080):
     return camera_rays(TERRAIN_CAMERA, width, height)
+
+
+# Batched mixed workload: n_scenes cluster soups with log-uniform triangle
+# counts in [tri_lo, tri_hi] (seeds 100 + k) and n_tiles terrain tiles with
+# log-uniform sizes in [64, 256]^3 (permutation seeds 2016 + k), each target
+# with a log-uniform incoherent ray batch in [ray_lo, ray_hi]: origins on a
+# sphere around the target, directions towards uniform points inside it.
+def _log_uniform(rng, lo, hi, n):
+    return np.exp(rng.uniform(np.log(lo), np.log(hi), n))
+
+
+def _incoherent_rays(rng, lo, hi, n):
+    c = (lo + hi) / 2
+    r = 0.75 * np.linalg.norm(hi - lo) + 1e-3
+    u = rng.standard_normal((n, 3))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    o = c + r * u
+    tgt = lo + (hi - lo) * rng.uniform(0, 1, (n, 3))
+    d = tgt - o
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return o.astype(np.float32), d.astype(np.float32)
+
+
+def batch_workload(n_scenes=64, n_tiles=16, tri_lo=1e3, tri_hi=5e5, ray_lo=1.6e4, ray_hi=2e6, tile_lo=64,
+                   tile_hi=256, seed=7):
+    """Returns (scene_soups, tile_dims, tile_seeds, ray_o, ray_d, scene_ranges, tile_ranges)."""
+    rng = np.random.default_rng(seed)
+    n_tris = np.round(_log_uniform(rng, tri_lo, tri_hi, n_scenes)).astype(int)
+    n_rays_s = np.round(_log_uniform(rng, ray_lo, ray_hi, n_scenes)).astype(int)
+    dims = (np.round(_log_uniform(rng, tile_lo, tile_hi, n_tiles) / 8) * 8).astype(int)
+    n_rays_t = np.round(_log_uniform(rng, ray_lo, ray_hi, n_tiles)).astype(int)
+    soups, os_, ds_, scene_ranges, tile_ranges = [], [], [], [], []
+    off = 0
+    for k in range(n_scenes):
+        soup = cluster_soup(100 + k, int(n_tris[k]))
+        soups.append(soup)
+        v = soup.reshape(-1, 3)
+        o, d = _incoherent_rays(rng, v.min(0).astype(np.float64), v.max(0).astype(np.float64), int(n_rays_s[k]))
+        os_.append(o)
+        ds_.append(d)
+        scene_ranges.append((off, len(o)))
+        off += len(o)
+    for k in range(n_tiles):
+        o, d = _incoherent_rays(rng, np.zeros(3), np.ones(3), int(n_rays_t[k]))
+        os_.append(o)
+        ds_.append(d)
+        tile_ranges.append((off, len(o)))
+        off += len(o)
+    return (soups, [int(x) for x in dims], [2016 + k for k in range(n_tiles)], np.concatenate(os_),
+            np.concatenate(ds_), scene_ranges, tile_ranges)
