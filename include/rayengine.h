@@ -161,6 +161,12 @@ typedef struct re_object {
 RE_API re_status re_scene_upload(re_ctx* ctx, const re_object* objs, int32_t n_obj,
                                  int32_t* scene_id);
 RE_API re_status re_scene_release(re_ctx* ctx, int32_t scene_id);
+/* Rewrites the objects' transforms (location, rotation, is_character) of a
+ * resident scene; trees and prim offsets stay.  Replaces update_objects
+ * (RayEngine/src/cuda-src/cuda_memory_functions.cuh:50-56), called per tick
+ * by RScene::Tick (RayEngine/src/World/Scene.cpp:79-91). */
+RE_API re_status re_scene_update_objects(re_ctx* ctx, int32_t scene_id, const re_object* objs,
+                                         int32_t n_obj);
 /* Device footprint of a scene in bytes. */
 RE_API re_status re_scene_bytes(re_ctx* ctx, int32_t scene_id, int64_t* bytes);
 /* sizes[0..3] = node entries, leaf records, triangle records, objects. */

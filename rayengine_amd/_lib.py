@@ -123,6 +123,7 @@ SIGNATURES = {
     "re_kd_export": (C.c_int, [_P, _P, _P, _P]),
     "re_scene_upload": (C.c_int, [_P, C.POINTER(re_object), C.c_int32, _I32]),
     "re_scene_release": (C.c_int, [_P, C.c_int32]),
+    "re_scene_update_objects": (C.c_int, [_P, C.c_int32, C.POINTER(re_object), C.c_int32]),
     "re_scene_bytes": (C.c_int, [_P, C.c_int32, _I64]),
     "re_scene_info": (C.c_int, [_P, C.c_int32, _I64]),
     "re_scene_export": (C.c_int, [_P, C.c_int32, _P, _P, _P, _P]),

@@ -171,6 +171,26 @@ RE_API re_status re_scene_upload(re_ctx* ctx, const re_object* objs, int32_t n_o
   return RE_OK;
 }
 
+RE_API re_status re_scene_update_objects(re_ctx* ctx, int32_t scene_id, const re_object* objs, int32_t n_obj) {
+  if (!ctx || !objs) return fail(RE_ERR_INVALID, "re_scene_update_objects: null argument");
+  auto it = ctx->scenes.find(scene_id);
+  if (it == ctx->scenes.end()) return set_err(ctx, RE_ERR_NOT_FOUND, "re_scene_update_objects: unknown scene");
+  DevScene& s = it->second;
+  if (n_obj != s.n_obj) return set_err(ctx, RE_ERR_INVALID, "re_scene_update_objects: object count differs from the upload");
+  std::vector<ObjXform> xf((size_t)n_obj);
+  HIP_TRY(ctx, hipMemcpyAsync(xf.data(), s.objs, xf.size() * sizeof(ObjXform), hipMemcpyDeviceToHost, ctx->stream), "read objects");
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream), "read objects sync");
+  for (int32_t k = 0; k < n_obj; ++k) {
+    for (int a = 0; a < 3; ++a) xf[k].loc[a] = objs[k].location[a];
+    xf[k].sin_y = objs[k].sin_rot_y;
+    xf[k].cos_y = objs[k].cos_rot_y;
+    xf[k].is_character = objs[k].is_character;
+  }
+  HIP_TRY(ctx, hipMemcpyAsync(s.objs, xf.data(), xf.size() * sizeof(ObjXform), hipMemcpyHostToDevice, ctx->stream), "write objects");
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream), "write objects sync");
+  return RE_OK;
+}
+
 RE_API re_status re_scene_release(re_ctx* ctx, int32_t scene_id) {
   if (!ctx) return fail(RE_ERR_INVALID, "re_scene_release: null ctx");
   auto it = ctx->scenes.find(scene_id);
