@@ -186,8 +186,11 @@ typedef struct re_trace_params {
   int32_t one_sided;       /* 1: reference one-sided test (ray_functions.cuh:182) */
   int32_t secondary;       /* is_secondary of stackless_kdtree_traversal */
   int32_t block;           /* threads per block, 0 = tuned default */
-  int32_t variant;         /* 0: one ray per thread (default); 1: persistent waves
-                              with per-lane ray refill.  Results are identical. */
+  int32_t variant;         /* 0: one ray per thread, blocks in ray order (default);
+                              1: persistent waves with per-lane ray refill;
+                              2: one ray per thread, last block of rays first;
+                              3: as 2 with a 6-wave register budget.
+                              Results are identical for every variant. */
 } re_trace_params;
 
 RE_API void re_trace_default_params(re_trace_params* p);
@@ -269,7 +272,7 @@ RE_API re_status re_sdf_trace(re_ctx* ctx, int32_t vol_id, const re_sdf_params* 
 /* ------------------------------------------------------ SDF surrogate
  *
  * A learned stand-in for the terrain field with no counterpart in the
- * reference: 16-frequency Fourier features of p, 4 hidden SiLU layers of 64,
+ * reference: 16-frequency Fourier features of p, 4 hidden hard-SiLU layers of 64,
  * scalar output, BF16 MFMA with FP32 accumulation.  Parameters are one flat
  * float vector of re_mlp_param_count() values:
  *   W0[64][99] b0[64] W1[64][64] b1[64] W2[64][64] b2[64] W3[64][64] b3[64] W4[64] b4

@@ -140,7 +140,9 @@ RE_API re_status re_scene_upload(re_ctx* ctx, const re_object* objs, int32_t n_o
   s.n_nodes = (int64_t)nodes.size();
   s.n_leaves = (int64_t)leaves.size();
   s.n_trirecs = (int64_t)tris.size();
-  auto up = [&](void** dst, const void* src, size_t bytes, const char* what) -> re_status {
+  // Trailing pad entries (leaf entries of leaf 0) for window reads at the end.
+  nodes.resize(nodes.size() + kNodePad, NodeEntry{0.0f, kLeafMeta});
+  auto up =[&](void** dst, const void* src, size_t bytes, const char* what) -> re_status {
     if (bytes == 0) bytes = 64;  // keep every pointer valid
     hipError_t e = hipMalloc(dst, bytes);
     if (e != hipSuccess) return hip_err(ctx, e, what);
@@ -286,7 +288,7 @@ re_status trace_common(re_ctx* ctx, int32_t scene_id, const float* ray_o, const 
   if (params) p = *params;
   if (!(p.block == 0 || p.block == 64 || p.block == 128 || p.block == 256))
     return set_err(ctx, RE_ERR_INVALID, "re_trace_closest: block must be 0, 64, 128 or 256");
-  if (p.variant != 0 && p.variant != 1) return set_err(ctx, RE_ERR_INVALID, "re_trace_closest: variant must be 0 or 1");
+  if (p.variant < 0 || p.variant > 3) return set_err(ctx, RE_ERR_INVALID, "re_trace_closest: variant must be 0..3");
   if (n == 0) return RE_OK;
   HIP_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
   TraceLaunch L;

@@ -13,6 +13,11 @@ using namespace re;
 
 namespace {
 
+// Switched off: the first run of a batch on an MI355X (creation, capture and
+// replay) did not complete, and the cause has not been identified.  Arguments
+// are still validated; a valid batch is refused until the cause is fixed.
+constexpr bool kBatchEnabled = false;
+
 void free_batch(DevBatch& b) {
   if (b.exec) (void)hipGraphExecDestroy(b.exec);
   if (b.graph) (void)hipGraphDestroy(b.graph);
@@ -63,6 +68,9 @@ RE_API re_status re_batch_create(re_ctx* ctx, const re_batch_scene* scenes, int3
   }
   if (any_surrogate && ctx->mlps.find(cfg.mlp_id) == ctx->mlps.end())
     return set_err(ctx, RE_ERR_NOT_FOUND, "re_batch_create: surrogate tiles need a valid mlp_id");
+  if (!kBatchEnabled)
+    return set_err(ctx, RE_ERR_UNSUPPORTED,
+                   "re_batch_create: batched launches are disabled (a batch run did not complete; cause not identified)");
   HIP_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
 
   DevBatch b;

@@ -50,6 +50,41 @@ struct RayCounts {
   unsigned ent = 0, leaf = 0, tri = 0;
 };
 
+// Reads the kNodeWindow entries starting at c (one 32-byte fetch; the entry
+// array is padded so a window at the last entry stays in bounds).
+__device__ __forceinline__ void load_window(const NodeEntry* __restrict__ nodes, int32_t c,
+                                            NodeEntry (&w)[kNodeWindow]) {
+#pragma unroll
+  for (int k = 0; k < kNodeWindow; ++k) w[k] = nodes[c + k];
+}
+
+// Descends from entry cur, whose window w is loaded, to the leaf containing
+// pe (the down-walk of ray_functions.cuh:289-291: below if pe < split, else
+// above).  The below child of an entry is the next entry, so a run of
+// "below" steps stays inside the window; a step above, or the end of the
+// window, starts a new fetch.  Returns the leaf id.
+template <bool COUNT>
+__device__ __forceinline__ uint32_t descend(const NodeEntry* __restrict__ nodes, int32_t cur,
+                                            NodeEntry (&w)[kNodeWindow], f3 pe, unsigned& c_ent) {
+  for (;;) {
+#pragma unroll
+    for (int k = 0; k < kNodeWindow; ++k) {
+      const NodeEntry e = w[k];
+      if (COUNT) ++c_ent;
+      const uint32_t ax = e.meta >> 30;
+      const uint32_t other = e.meta & 0x3fffffffu;
+      if (ax == 3u) return other;
+      if (get(pe, (int)ax) < e.split) {
+        ++cur;
+      } else {
+        cur = (int32_t)other;
+        break;
+      }
+    }
+    load_window(nodes, cur, w);
+  }
+}
+
 // Closest hit of one ray over objects [k0, k1): stackless_kdtree_traversal and
 // trace_scene, RayEngine/src/cuda-src/ray_functions.cuh:230-396.
 template <bool COUNT, bool ONE_SIDED>
@@ -71,30 +106,33 @@ __device__ __forceinline__ void trace_ray(const NodeEntry* __restrict__ nodes, c
     float hit_t = kInf;
     int32_t hit_prim = -1;
     int32_t visits = 0;
+    NodeEntry w[kNodeWindow];
+    bool have_window = false;
     while (t_entry < t_exit && visits < cap) {
       ++visits;
       const f3 pe = add(ro, smul(t_entry, rd));
-      while (!ref_is_leaf(cur)) {
-        const NodeEntry e = nodes[cur];
-        if (COUNT) ++rc.ent;
-        const uint32_t ax = e.meta >> 30;
-        const int32_t other = (int32_t)(e.meta & 0x3fffffffu);
-        if (ax == 3u) {
-          cur = leaf_ref((uint32_t)other);
-          break;
-        }
-        cur = (get(pe, (int)ax) < e.split) ? cur + 1 : other;
+      uint32_t leaf_id;
+      if (ref_is_leaf(cur)) {
+        leaf_id = ref_leaf_id(cur);
+      } else {
+        if (!have_window) load_window(nodes, cur, w);
+        leaf_id = descend<COUNT>(nodes, cur, w, pe, rc.ent);
       }
-      const LeafRec L = leaves[ref_leaf_id(cur)];
+      const LeafRec L = leaves[leaf_id];
       if (COUNT) ++rc.leaf;
-      test_leaf<ONE_SIDED, COUNT>(tris + L.first, L.count, ro, rd, t_exit, found, hit_t, hit_prim, rc.tri);
+      // The exit face and its rope depend on the leaf box alone, so they are
+      // found before the triangle tests, and the first window of the next
+      // descent is fetched while the triangles are.  Results are unchanged:
+      // the reference's order (triangles, then exit, :295-347) only matters
+      // for what the loop does next, which still waits for the tests.
       float tn, tf;
-      if (!ray_box(L.bmin, L.bmax, ro, rd, tn, tf)) break;
+      const bool box_ok = ray_box(L.bmin, L.bmax, ro, rd, tn, tf);
+      const int32_t r = box_ok ? exit_rope(L, add(ro, smul(tf, rd))) : kNoRope;
+      have_window = r >= 0;
+      if (have_window) load_window(nodes, r, w);
+      test_leaf<ONE_SIDED, COUNT>(tris + L.first, L.count, ro, rd, t_exit, found, hit_t, hit_prim, rc.tri);
+      if (!box_ok) break;
       t_entry = tf;
-      const f3 px = add(ro, smul(t_entry, rd));
-      const int face = exit_face(L.bmin, L.bmax, px);
-      if (face < 0) break;
-      const int32_t r = L.rope[face];
       if (r == kNoRope) break;
       cur = r;
     }

@@ -24,12 +24,15 @@ __device__ __forceinline__ f32x4 mfma16(bf16x8 a, bf16x8 b, f32x4 c) {
 __device__ __forceinline__ float sin_rev(float x) { return __builtin_amdgcn_sinf(__builtin_amdgcn_fractf(x)); }
 __device__ __forceinline__ float cos_rev(float x) { return __builtin_amdgcn_cosf(__builtin_amdgcn_fractf(x)); }
 
-__device__ __forceinline__ float sigmoid_f(float z) { return 1.0f / (1.0f + __expf(-z)); }
+// Hidden activation, hard SiLU: z * clamp(z/4 + 1/2, 0, 1).  It has SiLU's
+// value, slope and curvature at 0, is z above 2 and 0 below -2, and costs
+// two or three full-rate VALU instructions (SiLU's exp and divide are
+// quarter-rate transcendentals plus a division sequence).
+__device__ __forceinline__ float hsilu(float z) { return z * __builtin_amdgcn_fmed3f(fmaf(z, 0.25f, 0.5f), 0.0f, 1.0f); }
 
-// f_k = 2^(k/2).
-__device__ __forceinline__ float freq(int k) {
-  float base = (float)(1 << (k >> 1));
-  return (k & 1) ? base * 1.41421356237f : base;
+// Its derivative: z/2 + 1/2 on (-2, 2), 1 above, 0 below.
+__device__ __forceinline__ float dhsilu(float z) {
+  return fabsf(z) < 2.0f ? fmaf(z, 0.5f, 0.5f) : (z > 0.0f ? 1.0f : 0.0f);
 }
 
 // Device PCG32 (same generator as the host scene generators).
@@ -55,18 +58,26 @@ struct DevPcg {
 // ----------------------------------------------------------------- features
 
 // The B operands of layer 0 for this lane's point: K-step c < 3 holds the
-// sin/cos pairs of frequencies 4g..4g+3 of coordinate c; K-step 3 holds the
-// raw coordinates (g = 0, s < 3) and zero padding.
+// sin/cos pairs of frequencies 4g..4g+3 of coordinate c (f_k = 2^(k/2), so
+// f_{4g+2} = 2 f_{4g} and f_{4g+3} = 2 f_{4g+1}: those two pairs come from
+// the double-angle identities, which keeps 4 of the 8 transcendentals);
+// K-step 3 holds the raw coordinates (g = 0, s < 3) and zero padding.
 __device__ __forceinline__ void features(f3 p, int g, bf16x8 xb[4]) {
   const float pc[3] = {p.x, p.y, p.z};
+  const float f0 = (float)(1 << (2 * g));
+  const float f1 = f0 * 1.41421356237f;
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const float x = freq(4 * g + q) * pc[c];
-      xb[c][2 * q] = (__bf16)sin_rev(x);
-      xb[c][2 * q + 1] = (__bf16)cos_rev(x);
-    }
+    const float s0 = sin_rev(f0 * pc[c]), c0 = cos_rev(f0 * pc[c]);
+    const float s1 = sin_rev(f1 * pc[c]), c1 = cos_rev(f1 * pc[c]);
+    xb[c][0] = (__bf16)s0;
+    xb[c][1] = (__bf16)c0;
+    xb[c][2] = (__bf16)s1;
+    xb[c][3] = (__bf16)c1;
+    xb[c][4] = (__bf16)(2.0f * s0 * c0);
+    xb[c][5] = (__bf16)((c0 - s0) * (c0 + s0));
+    xb[c][6] = (__bf16)(2.0f * s1 * c1);
+    xb[c][7] = (__bf16)((c1 - s1) * (c1 + s1));
   }
 #pragma unroll
   for (int s = 0; s < 8; ++s) xb[3][s] = (__bf16)((g == 0 && s < 3) ? pc[s] : 0.0f);
@@ -90,7 +101,7 @@ __device__ __forceinline__ float forward_tile(const bf16x8* __restrict__ packF, 
     f32x4 acc[4];
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
-      acc[m] = f32x4{0.f, 0.f, 0.f, 0.f};
+      acc[m] = *reinterpret_cast<const f32x4*>(&small[16 * m + 4 * g]);  // the bias starts the sum
 #pragma unroll
       for (int c = 0; c < 4; ++c) acc[m] = mfma16(packF[(oPackF0 >> 3) + (m * 4 + c) * 64 + lane], xb[c], acc[m]);
     }
@@ -98,10 +109,9 @@ __device__ __forceinline__ float forward_tile(const bf16x8* __restrict__ packF, 
     for (int m = 0; m < 4; ++m)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const float z = acc[m][r] + small[16 * m + 4 * g + r];
+        const float z = acc[m][r];
         if (KEEP) T.z[0][m][r] = z;
-        const float h = z * sigmoid_f(z);
-        T.hb[0][m >> 1][(m & 1) * 4 + r] = (__bf16)h;
+        T.hb[0][m >> 1][(m & 1) * 4 + r] = (__bf16)hsilu(z);
       }
   }
   // layers 1..3: 2 K-steps of 32 neurons, chained in registers
@@ -113,7 +123,7 @@ __device__ __forceinline__ float forward_tile(const bf16x8* __restrict__ packF, 
     f32x4 acc[4];
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
-      acc[m] = f32x4{0.f, 0.f, 0.f, 0.f};
+      acc[m] = *reinterpret_cast<const f32x4*>(&small[64 * L + 16 * m + 4 * g]);
 #pragma unroll
       for (int c = 0; c < 2; ++c) acc[m] = mfma16(packF[off + (m * 2 + c) * 64 + lane], in[c], acc[m]);
     }
@@ -121,9 +131,9 @@ __device__ __forceinline__ float forward_tile(const bf16x8* __restrict__ packF, 
     for (int m = 0; m < 4; ++m)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const float z = acc[m][r] + small[64 * L + 16 * m + 4 * g + r];
+        const float z = acc[m][r];
         if (KEEP) T.z[L][m][r] = z;
-        const float h = z * sigmoid_f(z);
+        const float h = hsilu(z);
         T.hb[L][m >> 1][(m & 1) * 4 + r] = (__bf16)h;
         if (L == 3) T.h3[m][r] = h;
       }
@@ -155,13 +165,14 @@ __global__ void __launch_bounds__(256) mlp_infer_kernel(const bf16x8* __restrict
                                                         const float* __restrict__ pts, int64_t n,
                                                         float* __restrict__ out) {
   __shared__ bf16x8 spack[oPackB1 >> 3];
-  __shared__ float ssmall[kSmall];
+  __shared__ __align__(16) float ssmall[kSmall];
   load_weights(gpack, gsmall, spack, ssmall, oPackB1 >> 3);
   const int lane = threadIdx.x & 63;
   const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int64_t n_waves = ((int64_t)gridDim.x * blockDim.x) >> 6;
   const int64_t tiles = (n + 15) / 16;
   for (int64_t tile = wave; tile < tiles; tile += n_waves) {
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");  // weight reads stay in the loop
     int64_t i = tile * 16 + (lane & 15);
     const int64_t ic = i < n ? i : n - 1;
     const f3 p{pts[3 * ic], pts[3 * ic + 1], pts[3 * ic + 2]};
@@ -189,11 +200,6 @@ struct TrainLds {
   __bf16 hT[4][kH][kChunkPts];        // 32 KB (h0..h3)
   float dout[kChunkPts];
 };
-
-__device__ __forceinline__ float dsilu(float z) {
-  const float s = sigmoid_f(z);
-  return s * (1.0f + z * (1.0f - s));
-}
 
 __global__ void __launch_bounds__(256) mlp_train_kernel(const bf16x8* __restrict__ gpack,
                                                         const float* __restrict__ gsmall,
@@ -235,7 +241,7 @@ __global__ void __launch_bounds__(256) mlp_train_kernel(const bf16x8* __restrict
 #pragma unroll
     for (int m = 0; m < 4; ++m)
 #pragma unroll
-      for (int r2 = 0; r2 < 4; ++r2) dz[m][r2] = dout * S.small[256 + 16 * m + 4 * g + r2] * dsilu(T.z[3][m][r2]);
+      for (int r2 = 0; r2 < 4; ++r2) dz[m][r2] = dout * S.small[256 + 16 * m + 4 * g + r2] * dhsilu(T.z[3][m][r2]);
 #pragma unroll
     for (int L = 3; L >= 0; --L) {
       // stage dZ_L and the layer's input activations
@@ -260,7 +266,7 @@ __global__ void __launch_bounds__(256) mlp_train_kernel(const bf16x8* __restrict
 #pragma unroll
       for (int m = 0; m < 4; ++m)
 #pragma unroll
-        for (int r2 = 0; r2 < 4; ++r2) dz[m][r2] = dh[m][r2] * dsilu(T.z[L - 1][m][r2]);
+        for (int r2 = 0; r2 < 4; ++r2) dz[m][r2] = dh[m][r2] * dhsilu(T.z[L - 1][m][r2]);
     }
     // stage activations: features, h0..h3, and dout
 #pragma unroll
@@ -480,25 +486,44 @@ __global__ void sample_kernel(VolView edt, float lo, float hi, int64_t n, uint64
 
 // ----------------------------------------------------------------- hybrid tracer
 
-// Surrogate distance at every lane's point: the wave evaluates the network
-// in 16-point tiles; tiles whose lanes do not need it are skipped.
-__device__ __forceinline__ float wave_surrogate(const bf16x8* spack, const float* ssmall, f3 p, bool need) {
+// Surrogate distance at the points of the lanes that need it.  The wave packs
+// those points densely into its LDS slots (lane order), evaluates
+// ceil(count / 16) tiles of 16 points, and each lane reads its result back,
+// so a partly idle wave evaluates no empty tile rows.  LDS is in order within
+// a wave; the fences keep the compiler from moving the slot reads and writes
+// across each other.
+__device__ __forceinline__ float wave_surrogate(const bf16x8* spack, const float* ssmall, f3 p, bool need,
+                                                float* __restrict__ wpts, float* __restrict__ wout) {
   const int lane = threadIdx.x & 63;
-  float mine = 0.0f;
   const unsigned long long mask = __ballot(need);
+  const int count = __popcll(mask);
+  const int slot = __popcll(mask & ((1ull << lane) - 1ull));
+  if (need) {
+    wpts[slot] = p.x;
+    wpts[64 + slot] = p.y;
+    wpts[128 + slot] = p.z;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll 1
-  for (int q = 0; q < 4; ++q) {
-    if (((mask >> (16 * q)) & 0xffffull) == 0ull) continue;
-    const int src = 16 * q + (lane & 15);
-    const f3 pq{__shfl(p.x, src), __shfl(p.y, src), __shfl(p.z, src)};
+  for (int q = 0; q < count; q += 16) {
+    // keeps the weight reads inside the loop (hoisted, they would hold ~160
+    // VGPRs for the whole kernel and cut occupancy to one wave per SIMD)
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int idx = q + (lane & 15);
+    const int src = idx < count ? idx : count - 1;
+    const f3 pq{wpts[src], wpts[64 + src], wpts[128 + src]};
     bf16x8 xb[4];
     features(pq, lane >> 4, xb);
     TileAct T;
     const float o = forward_tile<false>(spack, ssmall, xb, lane, T);
-    const float v = __shfl(o, lane & 15);
-    if ((lane >> 4) == q) mine = v;
+    if (lane < 16 && idx < count) wout[idx] = o;
   }
-  return mine;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  return need ? wout[slot] : 0.0f;
 }
 
 // Exact-phase march: h = max(f * k_exact, h_min) from a point with f > 0;
@@ -514,7 +539,8 @@ __global__ void __launch_bounds__(256) hybrid_trace_kernel(
     float* __restrict__ t_out, uint8_t* __restrict__ hit_out, int32_t* __restrict__ evals_out,
     int32_t* __restrict__ samples_out) {
   __shared__ bf16x8 spack[USE_MLP ? (oPackB1 >> 3) : 1];
-  __shared__ float ssmall[kSmall];
+  __shared__ __align__(16) float ssmall[kSmall];
+  __shared__ float slots[USE_MLP ? 4 : 1][4 * 64];  // per wave: x, y, z, result
   if (USE_MLP) load_weights(gpack, gsmall, spack, ssmall, oPackB1 >> 3);
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t ic = i < n ? i : n - 1;
@@ -556,7 +582,8 @@ __global__ void __launch_bounds__(256) hybrid_trace_kernel(
     if (USE_MLP) {
       const bool need = phase == SURR;
       if (__any(need)) {
-        const float s = wave_surrogate(spack, ssmall, add(o, smul(t, d)), need);
+        float* ws = slots[threadIdx.x >> 6];
+        const float s = wave_surrogate(spack, ssmall, add(o, smul(t, d)), need, ws, ws + 192);
         if (need) {
           ++evals;
           if (s > hp.tau) {

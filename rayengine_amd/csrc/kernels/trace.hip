@@ -16,14 +16,20 @@
 namespace re {
 namespace {
 
-template <bool COUNT, bool ONE_SIDED>
-__global__ void __launch_bounds__(256) trace_closest_kernel(
-    const NodeEntry* __restrict__ nodes, const LeafRec* __restrict__ leaves,
-    const TriRec* __restrict__ tris, const ObjXform* __restrict__ objs, int32_t n_obj,
-    const float* __restrict__ ray_o, const float* __restrict__ ray_d, int64_t n, int32_t cap,
-    int32_t secondary, int32_t* __restrict__ prim_out, float* __restrict__ t_out,
-    int32_t* __restrict__ obj_out, TraceCounts* __restrict__ counts) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// One ray per lane.  REVERSE hands the last block of rays to the first
+// dispatched work-group: when costly rays sit at the end of the array (the
+// secondary rays a renderer appends after its primaries) they start first and
+// the cheap ones fill in behind them instead of forming the kernel's tail.
+template <bool COUNT, bool ONE_SIDED, bool REVERSE>
+__device__ __forceinline__ void closest_body(const NodeEntry* __restrict__ nodes, const LeafRec* __restrict__ leaves,
+                                             const TriRec* __restrict__ tris, const ObjXform* __restrict__ objs,
+                                             int32_t n_obj, const float* __restrict__ ray_o,
+                                             const float* __restrict__ ray_d, int64_t n, int32_t cap,
+                                             int32_t secondary, int32_t* __restrict__ prim_out,
+                                             float* __restrict__ t_out, int32_t* __restrict__ obj_out,
+                                             TraceCounts* __restrict__ counts) {
+  const int64_t blk = REVERSE ? (int64_t)(gridDim.x - 1 - blockIdx.x) : (int64_t)blockIdx.x;
+  const int64_t i = blk * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const f3 o{ray_o[3 * i], ray_o[3 * i + 1], ray_o[3 * i + 2]};
   const f3 d{ray_d[3 * i], ray_d[3 * i + 1], ray_d[3 * i + 2]};
@@ -41,6 +47,28 @@ __global__ void __launch_bounds__(256) trace_closest_kernel(
     atomicAdd(&counts->tris, (unsigned long long)rc.tri);
     if (best_prim >= 0) atomicAdd(&counts->rays_hit, 1ull);
   }
+}
+
+#define RE_CLOSEST_PARAMS                                                                                  \
+  const NodeEntry *__restrict__ nodes, const LeafRec *__restrict__ leaves, const TriRec *__restrict__ tris, \
+      const ObjXform *__restrict__ objs, int32_t n_obj, const float *__restrict__ ray_o,                    \
+      const float *__restrict__ ray_d, int64_t n, int32_t cap, int32_t secondary,                          \
+      int32_t *__restrict__ prim_out, float *__restrict__ t_out, int32_t *__restrict__ obj_out,            \
+      TraceCounts *__restrict__ counts
+#define RE_CLOSEST_ARGS \
+  nodes, leaves, tris, objs, n_obj, ray_o, ray_d, n, cap, secondary, prim_out, t_out, obj_out, counts
+
+template <bool COUNT, bool ONE_SIDED, bool REVERSE>
+__global__ void __launch_bounds__(256) trace_closest_kernel(RE_CLOSEST_PARAMS) {
+  closest_body<COUNT, ONE_SIDED, REVERSE>(RE_CLOSEST_ARGS);
+}
+
+// The same with the register budget cut to 80 VGPRs (6 waves per SIMD
+// instead of 5); the compiler spills a few values to scratch for it.
+template <bool COUNT, bool ONE_SIDED>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6)))
+trace_closest_o6_kernel(RE_CLOSEST_PARAMS) {
+  closest_body<COUNT, ONE_SIDED, true>(RE_CLOSEST_ARGS);
 }
 
 // ---------------------------------------------------------------------------
@@ -127,18 +155,15 @@ __global__ void __launch_bounds__(256) trace_persistent_kernel(
     if (live) {
       ++visits;
       const f3 pe = add(ro, smul(t_entry, rd));
-      while (!ref_is_leaf(cur)) {
-        const NodeEntry e = nodes[cur];
-        if (COUNT) ++c_ent;
-        const uint32_t ax = e.meta >> 30;
-        const int32_t other = (int32_t)(e.meta & 0x3fffffffu);
-        if (ax == 3u) {
-          cur = leaf_ref((uint32_t)other);
-          break;
-        }
-        cur = (get(pe, (int)ax) < e.split) ? cur + 1 : other;
+      uint32_t leaf_id;
+      if (ref_is_leaf(cur)) {
+        leaf_id = ref_leaf_id(cur);
+      } else {
+        NodeEntry w[kNodeWindow];
+        load_window(nodes, cur, w);
+        leaf_id = descend<COUNT>(nodes, cur, w, pe, c_ent);
       }
-      const LeafRec L = leaves[ref_leaf_id(cur)];
+      const LeafRec L = leaves[leaf_id];
       if (COUNT) ++c_leaf;
       test_leaf<ONE_SIDED, COUNT>(tris + L.first, L.count, ro, rd, t_exit, found, hit_t, hit_prim, c_tri);
       float tn, tf;
@@ -146,9 +171,7 @@ __global__ void __launch_bounds__(256) trace_persistent_kernel(
         live = false;
       } else {
         t_entry = tf;
-        const f3 px = add(ro, smul(t_entry, rd));
-        const int face = exit_face(L.bmin, L.bmax, px);
-        const int32_t r = face < 0 ? kNoRope : L.rope[face];
+        const int32_t r = exit_rope(L, add(ro, smul(t_entry, rd)));
         if (r == kNoRope) {
           live = false;
         } else {
@@ -219,15 +242,23 @@ hipError_t launch_trace_closest(const TraceLaunch& L, hipStream_t s) {
     return hipGetLastError();
   }
   const int64_t grid = (L.n + block - 1) / block;
-#define RE_LAUNCH(C, O)                                                                      \
-  hipLaunchKernelGGL((trace_closest_kernel<C, O>), dim3((unsigned)grid), dim3(block), 0, s,  \
-                     S.nodes, S.leaves, S.tris, S.objs, S.n_obj, L.ray_o, L.ray_d, L.n, cap, \
-                     L.secondary, L.prim, L.t, L.obj, L.counts)
-  if (L.counts) {
-    if (L.one_sided) RE_LAUNCH(true, true); else RE_LAUNCH(true, false);
-  } else {
-    if (L.one_sided) RE_LAUNCH(false, true); else RE_LAUNCH(false, false);
-  }
+#define RE_LAUNCH(K, ...)                                                                                       \
+  hipLaunchKernelGGL((K<__VA_ARGS__>), dim3((unsigned)grid), dim3(block), 0, s, S.nodes, S.leaves, S.tris, S.objs, \
+                     S.n_obj, L.ray_o, L.ray_d, L.n, cap, L.secondary, L.prim, L.t, L.obj, L.counts)
+#define RE_LAUNCH_ALL(K, ...)                                             \
+  do {                                                                    \
+    if (L.counts) {                                                       \
+      if (L.one_sided) RE_LAUNCH(K, true, true __VA_ARGS__);              \
+      else RE_LAUNCH(K, true, false __VA_ARGS__);                         \
+    } else {                                                              \
+      if (L.one_sided) RE_LAUNCH(K, false, true __VA_ARGS__);             \
+      else RE_LAUNCH(K, false, false __VA_ARGS__);                        \
+    }                                                                     \
+  } while (0)
+  if (L.variant == 2) RE_LAUNCH_ALL(trace_closest_kernel, , true);
+  else if (L.variant == 3) RE_LAUNCH_ALL(trace_closest_o6_kernel);
+  else RE_LAUNCH_ALL(trace_closest_kernel, , false);
+#undef RE_LAUNCH_ALL
 #undef RE_LAUNCH
   return hipGetLastError();
 }

@@ -167,6 +167,11 @@ constexpr uint32_t kLeafMeta = 3u << 30;
 // Ceiling of the per-ray leaf-visit cap (a cap <= 0 means this ceiling).
 constexpr int32_t kMaxLeafVisits = 1 << 20;
 constexpr int32_t kNoRope = -1;
+// Node entries are read kNodeWindow at a time (an inner node's below child is
+// the next entry), so the uploaded entry array carries kNodeWindow - 1 or more
+// trailing pad entries that a window starting at the last entry may read.
+constexpr int kNodeWindow = 4;
+constexpr int kNodePad = 4;
 
 struct NodeEntry {
   float split;
@@ -188,6 +193,18 @@ struct TriRec {
   float e2[3];
   int32_t prim;
 };
+
+// exit_face followed by the rope of that face, with constant indices only, so
+// the leaf record stays in registers.  kNoRope when no face matches.
+RE_HD int32_t exit_rope(const LeafRec& L, f3 p) {
+  if (fabsf(p.x - L.bmin[0]) < kEps) return L.rope[LEFT];
+  if (fabsf(p.z - L.bmax[2]) < kEps) return L.rope[FRONT];
+  if (fabsf(p.x - L.bmax[0]) < kEps) return L.rope[RIGHT];
+  if (fabsf(p.z - L.bmin[2]) < kEps) return L.rope[BACK];
+  if (fabsf(p.y - L.bmax[1]) < kEps) return L.rope[TOP];
+  if (fabsf(p.y - L.bmin[1]) < kEps) return L.rope[BOTTOM];
+  return kNoRope;
+}
 
 static_assert(sizeof(NodeEntry) == 8, "node entry is 8 bytes");
 static_assert(sizeof(LeafRec) == 64, "leaf record is 64 bytes");

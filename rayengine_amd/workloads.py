@@ -36,10 +36,29 @@ def cluster_soup_default(n: int = CLUSTER_TRIS, seed: int = 1) -> np.ndarray:
     return cluster_soup(seed, n, 64, -10.0, 10.0, 0.3, -3.0, 0.5)
 
 
-def cluster_rays(tri_v, primary_prim, primary_t, width=1920, height=1080):
-    """Primary rays plus one bounce ray per primary hit, concatenated."""
+def tile_order(width: int, height: int, tile: int = 8) -> np.ndarray:
+    """Pixel indices (y * width + x) visited tile by tile: tile x tile pixel
+    blocks in row order, pixels inside a block in row order; partial blocks
+    at the right and bottom edges keep the same scheme."""
+    ys, xs = np.divmod(np.arange(width * height, dtype=np.int64), width)
+    tiles_x = (width + tile - 1) // tile
+    key = ((ys // tile) * tiles_x + xs // tile) * (tile * tile) + (ys % tile) * tile + xs % tile
+    return np.argsort(key, kind="stable")
+
+
+def cluster_rays(tri_v, primary_prim, primary_t, width=1920, height=1080, order=None):
+    """Primary rays (in pixel order, or permuted by `order`, e.g. tile_order)
+    plus one bounce ray per primary hit, in the primaries' order.  The
+    primary hits passed in are in pixel order."""
     o, d = camera_rays(CLUSTER_CAMERA, width, height)
     bo, bd = bounce_rays(tri_v, o, d, primary_prim, primary_t, BOUNCE_SEED)
+    if order is not None:
+        o, d = o[order], d[order]
+        hit = np.flatnonzero(primary_prim >= 0)          # bounce k belongs to pixel hit[k]
+        rank = np.empty(len(primary_prim), np.int64)
+        rank[order] = np.arange(len(order))
+        bsel = np.argsort(rank[hit], kind="stable")
+        bo, bd = bo[bsel], bd[bsel]
     return np.concatenate([o, bo]), np.concatenate([d, bd])
 
 

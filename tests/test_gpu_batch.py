@@ -30,6 +30,8 @@ def _bits(x):
     return np.ascontiguousarray(x).view(np.uint32)
 
 
+@pytest.mark.skip(reason="batched launches are switched off: the first GPU run of this test did not complete and "
+                         "the cause is not identified; re_batch_create returns RE_ERR_UNSUPPORTED until it is")
 @pytest.mark.parametrize("item_rays,block", [(256, 256), (64, 128), (1000, 64)])
 def test_batch_equals_single_calls(gpu_ctx, small_batch, item_rays, block):
     trees, vols, ro, rd, sr, tr = small_batch
@@ -68,3 +70,9 @@ def test_batch_rejects_bad_ranges(gpu_ctx, small_batch):
         re.Batch(gpu_ctx, [(trees[0], len(ro) - 5, 10)], [], ro, rd)
     with pytest.raises(re.ReError):
         re.Batch(gpu_ctx, [], [(vols[0], 2, 0, 10)], ro, rd)          # surrogate without a network
+
+
+def test_batch_is_refused_while_switched_off(gpu_ctx, small_batch):
+    trees, vols, ro, rd, sr, tr = small_batch
+    with pytest.raises(re.ReError, match="disabled"):
+        re.Batch(gpu_ctx, [(trees[0], sr[0][0], sr[0][1])], [], ro, rd)

@@ -24,7 +24,7 @@ def _parity(p, t, po, to):
     assert np.array_equal(t.view(np.uint32), to.view(np.uint32)), f"{int((t.view(np.uint32) != to.view(np.uint32)).sum())} t differ"
 
 
-@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
 @pytest.mark.parametrize("cap", [50, 1, 4096])
 def test_small_scene_bit_exact(gpu_ctx, small, cap, variant):
     say(f"small scene, cap {cap}, variant {variant}")
@@ -52,13 +52,13 @@ def test_two_sided_and_block_shapes(gpu_ctx, small):
     scene = gpu_ctx.upload_scene([re.SceneObject(tree)])
     po, to = oracle.trace(*tree.export(), tree.identity_object(), o, d, two_sided=True)
     for block in (64, 128, 256):
-        for variant in (0, 1):
+        for variant in (0, 1, 2, 3):
             say(f"two-sided, block {block}, variant {variant}")
             p, t = scene.trace_closest(o, d, one_sided=False, block=block, variant=variant)
             _parity(p, t, po, to)
 
 
-@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
 def test_counters_match_oracle(gpu_ctx, small, variant):
     tri_v, tree, o, d = small
     scene = gpu_ctx.upload_scene([re.SceneObject(tree)])
@@ -84,7 +84,7 @@ def test_multi_object_transforms(gpu_ctx):
     o, d = re.camera_rays(re.Camera(pos=(0.3, 0.5, 6.0), view=(0.0, -0.1, -1.0)), 160, 120)
     for secondary in (False, True):
         po, to = oracle.trace(nodes, leaves, tris, obs, o, d, secondary=secondary)
-        for variant in (0, 1):
+        for variant in (0, 1, 2, 3):
             say(f"multi-object, secondary={secondary}, variant {variant}")
             p, t, ob = scene.trace_closest(o, d, with_obj=True, secondary=secondary, variant=variant)
             _parity(p, t, po, to)
