@@ -3,9 +3,10 @@
 Workload (the cluster scene): 2^20 FP32 triangles in 64 Gaussian clusters
 (sigma 0.3, centres U[-10,10]^3, edge lengths LogNormal(-3, 0.5), seed 1), SAH
 kd-tree with ropes (leaf <= 8, depth <= 24); 1920x1080 primary rays from
-(0,0,30) towards the origin (FOV 60) plus one cosine-weighted diffuse bounce
-ray per primary hit.  One step = one closest-hit launch over every ray, with
-rays and results resident in HBM.
+(0,0,30) towards the origin (FOV 60), issued in 8x8 pixel tiles, plus one
+cosine-weighted diffuse bounce ray per primary hit, in the order of their
+primaries.  One step = one closest-hit launch over every ray, with rays and
+results resident in HBM.
 
 Prints ONE JSON line.  `value` is rays traced per second (Mrays/s) over the
 timed steps.  `roofline` uses the algorithmic bytes of a launch, from the
@@ -13,12 +14,15 @@ kernel's own work counters: per ray 24 B in + 8 B out, 8 B per node entry,
 56 B per leaf record, 40 B per triangle record; divided by the mean kernel
 duration from HIP events recorded on the launch stream during the timed
 steps.  `cpu_baseline` times the CPU oracle on a bounded sample of the same
-rays.
+rays.  `terrain` reports the terrain path (noise volume, surrogate training
+and inference, and Mrays/s with hit depth error of the reference sphere
+tracer, the exact-field march and the surrogate hybrid).
 """
 from __future__ import annotations
 
 import argparse
 import json
+import re as re_mod
 import os
 import sys
 import time
@@ -64,14 +68,125 @@ def cpu_baseline(tree, ro, rd, n_primary, threads, budget_s=12.0):
                       f"{dt:.1f} s"}
 
 
+# Terrain settings (hit depth error is measured against REFERENCE_DEPTH, the
+# fine exact-field march that tests/test_gpu_terrain.py pins to the FP64
+# first crossing within 0.02 voxel at the 99th percentile).
+REFERENCE_DEPTH = dict(k_exact=0.02, h_min_voxels=0.25, n_bisect=20, max_iters=1 << 16)
+TERRAIN_METHODS = {
+    # the reference sphere tracer (renderer.cu:142-235) at its own settings
+    "reference_march": ("plain", dict(step_scale=0.8, eps=1e-3, max_steps=256)),
+    # the same tracer at the smallest step and tolerance tried: its most accurate setting
+    "reference_march_fine": ("plain", dict(step_scale=0.02, eps=1e-5, max_steps=8192)),
+    # exact-field bracketing march (no network)
+    "exact_march": ("exact", dict(k_exact=0.05, h_min_voxels=1.0, n_bisect=1)),
+    # surrogate far field + exact near field
+    "surrogate_hybrid": ("hybrid", dict(tau_voxels=4.0, margin_voxels=1.0, k_exact=0.1, n_bisect=4)),
+}
+MLP_FLOP_PER_EVAL = 2 * (99 * 64 + 3 * 64 * 64 + 64)
+BF16_PEAK_TFLOPS = 2500.0
+
+
+def terrain_section(re, W, torch, ctx):
+    """Terrain path on one GPU: the 256^3 fBm volume (seed 2016) over the unit
+    cube, 1920x1080 rays from the terrain camera; noise generation, training,
+    network throughput, and hit depth error / Mrays/s of each tracer."""
+    dev = torch.device("cuda:0")
+    out = {"workload": "256^3 fBm terrain (5 octaves, seed 2016) on the unit cube, 1920x1080 rays"}
+    vol = ctx.noise_volume()
+    noise = []
+    for _ in range(3):
+        v2 = ctx.noise_volume()
+        noise.append(ctx.last_kernel_ms())
+        v2.release()
+    out["noise_ms"] = round(float(np.mean(noise)), 4)
+    out["noise_gvox_s"] = round(256 ** 3 / (np.mean(noise) * 1e-3) / 1e9, 2)
+    dv = re.distance_volume(vol)
+    net = re.Surrogate(ctx, seed=1)
+    t0 = time.perf_counter()
+    hist = net.train(dv, n_samples=1 << 22, batch=65536, steps=4000, lr=1e-3, seed=1, over_weight=4.0)
+    out["train"] = {"steps": 4000, "batch": 65536, "ms_per_step": round((time.perf_counter() - t0) * 1e3 / 4000, 4),
+                    "loss_first": float(hist[:10].mean()), "loss_last": float(hist[-100:].mean())}
+    # network throughput (re_mlp_forward on 4M device points)
+    npts = 1 << 22
+    pts = torch.rand(npts, 3, device=dev)
+    val = torch.empty(npts, device=dev)
+    torch.cuda.synchronize()
+    for _ in range(2):
+        net.forward_device(pts.data_ptr(), npts, val.data_ptr())
+    for _ in range(5):
+        net.forward_device(pts.data_ptr(), npts, val.data_ptr(), sync=False)
+    torch.cuda.synchronize()
+    fms = float(np.mean(ctx.kernel_times(5)))
+    tf = npts * MLP_FLOP_PER_EVAL / (fms * 1e-3) / 1e12
+    out["mlp_forward"] = {"kernel": "mlp_infer_kernel", "gevals_s": round(npts / (fms * 1e-3) / 1e9, 3),
+                          "tflops": round(tf, 2), "bound": "mfma", "peak_tflops": BF16_PEAK_TFLOPS,
+                          "frac": round(tf / BF16_PEAK_TFLOPS, 4), "flop_per_eval": MLP_FLOP_PER_EVAL}
+    o, d = W.terrain_rays(1920, 1080)
+    n = len(o)
+    do, dd = torch.from_numpy(o).to(dev), torch.from_numpy(d).to(dev)
+    dt = torch.empty(n, dtype=torch.float32, device=dev)
+    dh = torch.empty(n, dtype=torch.uint8, device=dev)
+    torch.cuda.synchronize()
+
+    def run(kind, params, sync=True):
+        if kind == "plain":
+            vol.trace_device(do.data_ptr(), dd.data_ptr(), n, dt.data_ptr(), dh.data_ptr(), sync=sync, **params)
+        else:
+            re.hybrid_trace_device(vol, net if kind == "hybrid" else None, do.data_ptr(), dd.data_ptr(), n,
+                                   dt.data_ptr(), dh.data_ptr(), sync=sync, **params)
+
+    run("exact", REFERENCE_DEPTH)
+    ref_t, ref_h = dt.cpu().numpy().astype(np.float64), dh.cpu().numpy()
+    vox = 1.0 / 256
+    methods = {}
+    for name, (kind, params) in TERRAIN_METHODS.items():
+        run(kind, params)
+        t, h = dt.cpu().numpy().astype(np.float64), dh.cpu().numpy()
+        both = (h == 1) & (ref_h == 1)
+        e = np.abs(t[both] - ref_t[both]) / vox
+        for _ in range(5):
+            run(kind, params, sync=False)
+        torch.cuda.synchronize()
+        ms = float(np.mean(ctx.kernel_times(5)))
+        methods[name] = {"mrays_s": round(n / (ms * 1e-3) / 1e6, 1), "kernel_ms": round(ms, 4),
+                         "err_p50_vox": round(float(np.median(e)), 4), "err_p99_vox": round(float(np.percentile(e, 99)), 4),
+                         "frac_err_gt_half_vox": round(float(np.mean(e > 0.5)), 5),
+                         "hit_agree": round(float(np.mean(h == ref_h)), 5), "params": params}
+    out["methods"] = methods
+    hy, fine, ex = methods["surrogate_hybrid"], methods["reference_march_fine"], methods["exact_march"]
+    out["surrogate_vs_reference_march"] = round(hy["mrays_s"] / fine["mrays_s"], 3)
+    out["surrogate_vs_exact_march"] = round(hy["mrays_s"] / ex["mrays_s"], 3)
+    reach = "reaches" if fine["err_p99_vox"] <= 0.5 else "does not reach"
+    out["note"] = (f"reference_march_fine is the reference tracer's most accurate setting tried; it {reach} 0.5 voxel "
+                   f"at p99. exact_march (no network) is the ablation of the surrogate.")
+    for x in (net, dv, vol):
+        x.release()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-terrain", action="store_true", help="skip the terrain section")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="only run 5 device copies of BYTES bytes (counter calibration)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     args = ap.parse_args()
+    if args.calibrate_copy:
+        # A device-to-device copy of known size: the reference point that turns
+        # the FETCH_SIZE / WRITE_SIZE counters into bytes.
+        import torch
+        n = args.calibrate_copy // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": n * 4, "launches": 5}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               f"(rays shard across GPUs with no exchange, see DESIGN.md)", file=sys.stderr)
@@ -90,7 +205,10 @@ def main():
     scene = ctx.upload_scene([re.SceneObject(tree)])
     po, pd = re.camera_rays(W.CLUSTER_CAMERA, 1920, 1080)
     pp, pt = scene.trace_closest(po, pd)
-    ro, rd = W.cluster_rays(tri_v, pp, pt)
+    # Primary rays in 8x8 pixel tiles (bounces follow their primaries): the
+    # order a tiled renderer issues them in, and the ray-sort granularity
+    # measured best among 4, 8 and 16.
+    ro, rd = W.cluster_rays(tri_v, pp, pt, order=W.tile_order(1920, 1080, 8))
     n = len(ro)
     n_primary = len(po)
     log(f"rays: {n} ({n_primary} primary + {n - n_primary} bounce)")
@@ -116,13 +234,14 @@ def main():
     kms = float(np.mean(ktimes))
     achieved = alg_bytes / (kms * 1e-3) / 1e9
 
+    # HBM bytes per launch of the traversal kernel, from the FETCH_SIZE and
+    # WRITE_SIZE counter runs recorded in the profile summary (calibrated
+    # against a copy kernel of known size).
     traffic = None
-    tfile = ROOT / "profiles" / "trace_closest_cluster_traffic.json"
+    tfile = ROOT / "profiles" / "trace_closest_cluster_traffic.md"
     if tfile.exists():
-        try:
-            traffic = json.loads(tfile.read_text()).get("hbm_bytes_per_launch")
-        except Exception:
-            traffic = None
+        m = re_mod.search(r"hbm_bytes_per_launch:\s*([0-9]+)", tfile.read_text())
+        traffic = int(m.group(1)) if m else None
 
     value = n * args.steps / wall / 1e6
     res = {
@@ -139,17 +258,22 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": "cluster scene: 2^20-triangle Gaussian-cluster soup (64 clusters, seed 1), "
-                               "SAH rope kd-tree leaf<=8 depth<=24, 1920x1080 primary rays + 1 diffuse bounce "
-                               "per hit, closest hit, leaf-visit cap 50",
+                               "SAH rope kd-tree leaf<=8 depth<=24, 1920x1080 primary rays (8x8 pixel tiles) + "
+                               "1 diffuse bounce per hit, closest hit, leaf-visit cap 50",
                    "rays": n, "primary_rays": n_primary, "bounce_rays": n - n_primary,
                    "triangles": int(tri_v.shape[0]), "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "trace_closest_kernel", "kernel_ms": round(kms, 4),
+                     "kernel": "trace_closest_o6_kernel", "kernel_ms": round(kms, 4),
                      "algorithmic_bytes": int(alg_bytes),
                      "work": {"node_entries": counts[0], "leaves": counts[1], "tri_tests": counts[2],
                               "rays_hit": counts[3]}},
     }
+    if not args.no_terrain:
+        try:
+            res["terrain"] = terrain_section(re, W, torch, ctx)
+        except Exception as e:  # the closest-hit line is still reported
+            res["terrain"] = {"error": f"{type(e).__name__}: {e}"}
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         res["cpu_baseline"] = cpu_baseline(tree, ro, rd, n_primary, threads)

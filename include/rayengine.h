@@ -185,11 +185,11 @@ typedef struct re_trace_params {
   int32_t max_leaf_visits; /* reference cap 50 (ray_functions.cuh:284); <= 0 = unlimited */
   int32_t one_sided;       /* 1: reference one-sided test (ray_functions.cuh:182) */
   int32_t secondary;       /* is_secondary of stackless_kdtree_traversal */
-  int32_t block;           /* threads per block, 0 = tuned default */
-  int32_t variant;         /* 0: one ray per thread, blocks in ray order (default);
+  int32_t block;           /* threads per block, 0 = default (64; 256 for variant 1) */
+  int32_t variant;         /* 0: one ray per thread, blocks in ray order;
                               1: persistent waves with per-lane ray refill;
                               2: one ray per thread, last block of rays first;
-                              3: as 2 with a 6-wave register budget.
+                              3: as 2 with a 6-wave register budget (default).
                               Results are identical for every variant. */
 } re_trace_params;
 

@@ -346,6 +346,11 @@ class Surrogate:
         check(lib().re_mlp_forward(self.ctx.handle, self.id, ptr(pts), n, ptr(out), 0), self.ctx.handle)
         return out
 
+    def forward_device(self, pts_ptr: int, n: int, out_ptr: int, sync=True):
+        """Forward pass over n device-resident points (float32 x, y, z)."""
+        flags = RE_DEVICE_PTRS | (0 if sync else RE_ASYNC)
+        check(lib().re_mlp_forward(self.ctx.handle, self.id, pts_ptr, n, out_ptr, flags), self.ctx.handle)
+
     def batch_step(self, samples, over_weight=4.0, adam=False, lr=1e-3):
         """(loss, grad) of one batch of (x, y, z, target) samples; optionally one Adam step."""
         s = _f32c(samples, 4)
@@ -474,7 +479,7 @@ class Batch:
             self.id = None
 
 
-def _trace_params(max_leaf_visits=50, one_sided=True, secondary=False, block=0, variant=0):
+def _trace_params(max_leaf_visits=50, one_sided=True, secondary=False, block=0, variant=3):
     return _lib.re_trace_params(int(max_leaf_visits), int(bool(one_sided)), int(bool(secondary)), int(block),
                                 int(variant))
 

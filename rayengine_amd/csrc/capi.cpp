@@ -246,7 +246,7 @@ RE_API void re_trace_default_params(re_trace_params* p) {
   p->one_sided = 1;
   p->secondary = 0;
   p->block = 0;
-  p->variant = 0;
+  p->variant = 3;
 }
 
 RE_API re_status re_kernel_times(re_ctx* ctx, float* ms, int32_t max, int32_t* count) {

@@ -217,7 +217,10 @@ int64_t trace_persistent_grid(int block) {
 
 hipError_t launch_trace_closest(const TraceLaunch& L, hipStream_t s) {
   if (L.n <= 0) return hipSuccess;
-  const int block = L.block > 0 ? L.block : 256;
+  // Default work-group: 256 threads for the persistent kernel, one wave for
+  // the one-ray-per-thread kernels (finer dispatch granularity at the tail;
+  // 64 and 128 measured alike and ahead of 256 on the cluster workload).
+  const int block = L.block > 0 ? L.block : (L.variant == 1 ? 256 : 64);
   const DevScene& S = *L.scene;
   // A ray can alternate between two leaves when its exit point lies within
   // the rope tolerance of two faces; the visit cap is what ends it, so

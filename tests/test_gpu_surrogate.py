@@ -170,3 +170,34 @@ def test_training_reduces_loss(gpu_ctx):
     assert np.isfinite(hist).all()
     assert hist[-20:].mean() < 0.25 * hist[:5].mean()
     net.release()
+
+
+# Hybrid settings of the terrain benchmark.
+HYBRID = dict(tau_voxels=4.0, margin_voxels=1.0, k_exact=0.1, n_bisect=4)
+
+
+def test_hybrid_accuracy_and_what_the_network_saves(gpu_ctx):
+    """Against the FP64 first crossing on a 480x270 frame: the trained
+    surrogate's hybrid trace stays within 0.5 voxel at the 99th percentile,
+    and it takes fewer exact-field samples per ray than the exact march with
+    the same near-surface settings (the network replaces the far-field
+    march)."""
+    vol = gpu_ctx.noise_volume()
+    dv = re.distance_volume(vol)
+    net = re.Surrogate(gpu_ctx, seed=1)
+    hist = net.train(dv, n_samples=1 << 22, batch=65536, steps=4000, lr=1e-3, seed=1, over_weight=4.0)
+    o, d = W.terrain_rays(480, 270)
+    g64 = oracle.noise_fill_f64(re.perlin_table(2016))
+    tt, th = oracle.truth_trace(g64, W.TERRAIN_SPACING, o, d)
+    th_, hh, ev, sm = re.hybrid_trace(vol, net, o, d, **HYBRID)
+    te, he, _, se = re.hybrid_trace(vol, None, o, d, k_exact=HYBRID["k_exact"], n_bisect=HYBRID["n_bisect"])
+    both = (hh == 1) & (th == 1)
+    e = np.abs(th_[both].astype(np.float64) - tt[both]) * 256
+    say(f"loss {hist[-100:].mean():.2e}; hybrid agree {np.mean(hh == th):.5f} p99 {np.percentile(e, 99):.3f} voxel, "
+        f"evals/ray {ev.mean():.2f}, samples/ray {sm.mean():.2f} vs exact march {se.mean():.2f}")
+    assert np.mean(hh == th) >= 0.999
+    assert np.percentile(e, 99) <= 0.5
+    assert sm.mean() < 0.9 * se.mean()
+    net.release()
+    dv.release()
+    vol.release()

@@ -113,3 +113,23 @@ def test_fp32_hit_depth_error_vs_fp64(terrain, perm):
         f"p99 {np.percentile(err, 99):.3e}, max {err.max():.3e} (voxel = {1 / 256:.3e})")
     assert np.mean(hit == h64) > 0.999
     assert np.percentile(err, 99) < 1e-3
+
+
+# The depth reference of the terrain benchmark: the exact-field march with
+# small steps and many bisections, pinned here against the FP64 first
+# crossing of the oracle.
+REFERENCE_DEPTH = dict(k_exact=0.02, h_min_voxels=0.25, n_bisect=20, max_iters=1 << 16)
+
+
+def test_fine_exact_march_is_a_depth_reference(terrain, perm):
+    vol, grid = terrain
+    o, d = W.terrain_rays(480, 270)
+    t, h, _, samples = re.hybrid_trace(vol, None, o, d, **REFERENCE_DEPTH)
+    g64 = oracle.noise_fill_f64(perm)
+    tt, th = oracle.truth_trace(g64, W.TERRAIN_SPACING, o, d)
+    both = (h == 1) & (th == 1)
+    e = np.abs(t[both].astype(np.float64) - tt[both]) * 256
+    say(f"fine exact march vs FP64 crossing: agree {np.mean(h == th):.5f}, p50 {np.median(e):.4f}, "
+        f"p99 {np.percentile(e, 99):.4f} voxel, samples/ray {samples.mean():.1f}")
+    assert np.mean(h == th) >= 0.999
+    assert np.percentile(e, 99) <= 0.02
