@@ -234,7 +234,7 @@ hipError_t launch_mlp_infer(const void* pack, const float* small, const float* p
 hipError_t launch_mlp_train(const void* pack, const float* small, const float4* samples, int64_t n_samples,
                             int64_t batch_start, int32_t batch, float over_weight, float* partial, int32_t blocks,
                             hipStream_t s);
-hipError_t launch_mlp_update(float* params, float* m1, float* m2, const float* partial, int32_t n_blocks,
+hipError_t launch_mlp_update(float* params, float* m1, float* m2, float* partial, int32_t n_blocks,
                              int32_t step, float lr, float b1, float b2, float eps, int32_t do_adam, float* grad_out,
                              void* pack, float* small, float* loss_out, hipStream_t s);
 hipError_t launch_edt(const DevVolume& field, float* scratch_a, float* scratch_b, float* out_bricked, hipStream_t s);

@@ -376,6 +376,21 @@ __device__ __forceinline__ void pack_param(int p, float wv, __bf16* pack, float*
   else if (p == ob4) small[320] = wv;
 }
 
+// First stage of the sum over blocks: row kFold * s of the partial-gradient
+// array receives the sum of rows kFold * s .. kFold * s + kFold - 1, each
+// column by one thread (fixed order, so the result is deterministic).
+constexpr int kFold = 16;
+
+__global__ void partial_fold_kernel(float* __restrict__ partial, int32_t n_blocks) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p > kParams) return;
+  const int b0 = blockIdx.y * kFold;
+  const int b1 = b0 + kFold < n_blocks ? b0 + kFold : n_blocks;
+  float acc = 0.0f;
+  for (int b = b0; b < b1; ++b) acc += partial[(int64_t)b * (kParams + 1) + p];
+  partial[(int64_t)b0 * (kParams + 1) + p] = acc;
+}
+
 __global__ void mlp_update_kernel(float* __restrict__ params, float* __restrict__ m1, float* __restrict__ m2,
                                   const float* __restrict__ partial, int32_t n_blocks, int32_t step, float lr,
                                   float b1, float b2, float eps, int32_t do_adam, float* __restrict__ grad_out,
@@ -384,7 +399,7 @@ __global__ void mlp_update_kernel(float* __restrict__ params, float* __restrict_
   if (p > kParams) return;
   float gsum = 0.0f;
   if (partial)
-    for (int b = 0; b < n_blocks; ++b) gsum += partial[(int64_t)b * (kParams + 1) + p];
+    for (int b = 0; b < n_blocks; b += kFold) gsum += partial[(int64_t)b * (kParams + 1) + p];
   if (p == kParams) {
     if (loss_out) *loss_out = gsum;
     return;
@@ -677,11 +692,14 @@ hipError_t launch_mlp_train(const void* pack, const float* small, const float4* 
   return hipGetLastError();
 }
 
-hipError_t launch_mlp_update(float* params, float* m1, float* m2, const float* partial, int32_t n_blocks,
+hipError_t launch_mlp_update(float* params, float* m1, float* m2, float* partial, int32_t n_blocks,
                              int32_t step, float lr, float b1, float b2, float eps, int32_t do_adam, float* grad_out,
                              void* pack, float* small, float* loss_out, hipStream_t s) {
   const int threads = 256;
   const int blocks = (kParams + 1 + threads - 1) / threads;
+  if (partial && n_blocks > 0)
+    hipLaunchKernelGGL(partial_fold_kernel, dim3(blocks, (n_blocks + kFold - 1) / kFold), dim3(threads), 0, s,
+                       partial, n_blocks);
   hipLaunchKernelGGL(mlp_update_kernel, dim3(blocks), dim3(threads), 0, s, params, m1, m2, partial, n_blocks, step,
                      lr, b1, b2, eps, do_adam, grad_out, (__bf16*)pack, small, loss_out);
   return hipGetLastError();
