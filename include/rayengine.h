@@ -272,7 +272,7 @@ RE_API re_status re_sdf_trace(re_ctx* ctx, int32_t vol_id, const re_sdf_params* 
 /* ------------------------------------------------------ SDF surrogate
  *
  * A learned stand-in for the terrain field with no counterpart in the
- * reference: 16-frequency Fourier features of p, 4 hidden hard-SiLU layers of 64,
+ * reference: 16-frequency Fourier features of p, 4 hidden SiLU layers of 64,
  * scalar output, BF16 MFMA with FP32 accumulation.  Parameters are one flat
  * float vector of re_mlp_param_count() values:
  *   W0[64][99] b0[64] W1[64][64] b1[64] W2[64][64] b2[64] W3[64][64] b3[64] W4[64] b4
@@ -331,8 +331,8 @@ RE_API re_status re_mlp_train(re_ctx* ctx, int32_t mlp_id, int32_t dist_vol_id,
  * entry: the exact-field tracer at the same accuracy.  Distances are in
  * voxels of the volume. */
 typedef struct re_hybrid_params {
-  float tau_voxels;    /* default 2 */
-  float margin_voxels; /* default 1 */
+  float tau_voxels;    /* default 2; the surrogate steps while its distance exceeds tau */
+  float margin_voxels; /* default 1; subtracted from each surrogate step, must be < tau */
   float k_exact;       /* default 0.05 */
   float h_min_voxels;  /* default 0.5 */
   float rejoin_voxels; /* default 4 */

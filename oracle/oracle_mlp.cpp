@@ -5,8 +5,8 @@
 //   features(p) = [sin(2 pi f_k p_d), cos(2 pi f_k p_d)] for d < 3, k < 16,
 //                 feature index 32 d + 2 k + {0: sin, 1: cos}, f_k = 2^(k/2);
 //                 then p.x, p.y, p.z (indices 96..98)
-//   h_0 = a(W0 x + b0), h_{l} = a(W_l h_{l-1} + b_l) for l = 1..3,
-//   out = W4 . h_3 + b4,   a(z) = z * clamp(z / 4 + 1 / 2, 0, 1)  (hard SiLU)
+//   h_0 = silu(W0 x + b0), h_{l} = silu(W_l h_{l-1} + b_l) for l = 1..3,
+//   out = W4 . h_3 + b4,   silu(z) = z / (1 + exp(-z))
 // Parameters are one flat float vector:
 //   W0[64][99] b0[64] W1[64][64] b1[64] W2[64][64] b2[64] W3[64][64] b3[64] W4[64] b4[1]
 // Loss over a batch: mean_i w_i (out_i - y_i)^2 with w_i = over_weight when
@@ -40,9 +40,13 @@ void features(const double* p, double* x) {
   x[98] = p[2];
 }
 
-inline double act(double z) { return z * std::min(std::max(0.25 * z + 0.5, 0.0), 1.0); }
-// d act / dz: z/2 + 1/2 on (-2, 2), 1 above, 0 below.
-inline double dact(double z) { return std::fabs(z) < 2.0 ? 0.5 * z + 0.5 : (z > 0.0 ? 1.0 : 0.0); }
+inline double sigmoid(double z) { return 1.0 / (1.0 + std::exp(-z)); }
+inline double act(double z) { return z * sigmoid(z); }
+// d silu / dz = s (1 + z (1 - s)), s = sigmoid(z).
+inline double dact(double z) {
+  const double s = sigmoid(z);
+  return s * (1.0 + z * (1.0 - s));
+}
 
 struct Acts {
   double x[kIn];

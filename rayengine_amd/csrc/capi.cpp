@@ -142,6 +142,8 @@ RE_API re_status re_scene_upload(re_ctx* ctx, const re_object* objs, int32_t n_o
   s.n_trirecs = (int64_t)tris.size();
   // Trailing pad entries (leaf entries of leaf 0) for window reads at the end.
   nodes.resize(nodes.size() + kNodePad, NodeEntry{0.0f, kLeafMeta});
+  std::vector<DevLeaf> dleaves(leaves.size());
+  for (size_t i = 0; i < leaves.size(); ++i) dleaves[i] = dev_leaf(leaves[i]);
   auto up =[&](void** dst, const void* src, size_t bytes, const char* what) -> re_status {
     if (bytes == 0) bytes = 64;  // keep every pointer valid
     hipError_t e = hipMalloc(dst, bytes);
@@ -155,7 +157,7 @@ RE_API re_status re_scene_upload(re_ctx* ctx, const re_object* objs, int32_t n_o
   };
   re_status st;
   if ((st = up((void**)&s.nodes, nodes.empty() ? nullptr : nodes.data(), nodes.size() * sizeof(NodeEntry), "upload nodes")) ||
-      (st = up((void**)&s.leaves, leaves.empty() ? nullptr : leaves.data(), leaves.size() * sizeof(LeafRec), "upload leaves")) ||
+      (st = up((void**)&s.leaves, dleaves.empty() ? nullptr : dleaves.data(), dleaves.size() * sizeof(DevLeaf), "upload leaves")) ||
       (st = up((void**)&s.tris, tris.empty() ? nullptr : tris.data(), tris.size() * sizeof(TriRec), "upload tris")) ||
       (st = up((void**)&s.objs, xf.data(), xf.size() * sizeof(ObjXform), "upload objects"))) {
     (void)hipStreamSynchronize(ctx->stream);
@@ -167,8 +169,9 @@ RE_API re_status re_scene_upload(re_ctx* ctx, const re_object* objs, int32_t n_o
     free_scene(s);
     return hip_err(ctx, e, "re_scene_upload sync");
   }
+  s.h_leaves = std::move(leaves);
   int32_t id = ctx->next_id++;
-  ctx->scenes[id] = s;
+  ctx->scenes[id] = std::move(s);
   *scene_id = id;
   return RE_OK;
 }
@@ -230,8 +233,8 @@ RE_API re_status re_scene_export(re_ctx* ctx, int32_t scene_id, void* nodes, voi
   HIP_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
   if (nodes && s.n_nodes)
     HIP_TRY(ctx, hipMemcpyAsync(nodes, s.nodes, s.n_nodes * sizeof(NodeEntry), hipMemcpyDeviceToHost, ctx->stream), "export nodes");
-  if (leaves && s.n_leaves)
-    HIP_TRY(ctx, hipMemcpyAsync(leaves, s.leaves, s.n_leaves * sizeof(LeafRec), hipMemcpyDeviceToHost, ctx->stream), "export leaves");
+  // Leaf records in the flat field order (host copy; the device holds them as DevLeaf).
+  if (leaves && s.n_leaves) std::memcpy(leaves, s.h_leaves.data(), s.n_leaves * sizeof(LeafRec));
   if (tris && s.n_trirecs)
     HIP_TRY(ctx, hipMemcpyAsync(tris, s.tris, s.n_trirecs * sizeof(TriRec), hipMemcpyDeviceToHost, ctx->stream), "export tris");
   if (objs)

@@ -278,6 +278,10 @@ RE_API re_status re_hybrid_trace(re_ctx* ctx, int32_t vol_id, int32_t mlp_id, co
   if (p.max_iters < 1 || p.max_iters > (1 << 20) || p.n_bisect < 0 || p.n_bisect > 32 || !(p.h_min_voxels > 0.0f) ||
       !(p.k_exact >= 0.0f))
     return set_err(ctx, RE_ERR_INVALID, "re_hybrid_trace: max_iters in [1,2^20], n_bisect in [0,32], h_min > 0");
+  // A surrogate step is s - margin with s > tau: tau must exceed the margin
+  // or the steps shrink towards zero.
+  if (mlp_id >= 0 && !(p.tau_voxels > p.margin_voxels))
+    return set_err(ctx, RE_ERR_INVALID, "re_hybrid_trace: tau_voxels must exceed margin_voxels");
   if (n == 0) return RE_OK;
   const DevVolume& v = it->second;
   const float vox = v.spacing[0];

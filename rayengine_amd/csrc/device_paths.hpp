@@ -50,45 +50,46 @@ struct RayCounts {
   unsigned ent = 0, leaf = 0, tri = 0;
 };
 
-// Reads the kNodeWindow entries starting at c (one 32-byte fetch; the entry
-// array is padded so a window at the last entry stays in bounds).
-__device__ __forceinline__ void load_window(const NodeEntry* __restrict__ nodes, int32_t c,
-                                            NodeEntry (&w)[kNodeWindow]) {
-#pragma unroll
-  for (int k = 0; k < kNodeWindow; ++k) w[k] = nodes[c + k];
+// The two entries starting at c as one 16-byte load per lane (the entry
+// array is padded so a window at the last entry stays in bounds): .x/.y are
+// the split bits and meta of entry c, .z/.w those of entry c + 1.
+static_assert(kNodeWindow == 2, "the descent reads two-entry windows");
+__device__ __forceinline__ uint4 load_window(const NodeEntry* __restrict__ nodes, int32_t c) {
+  uint4 q;
+  __builtin_memcpy(&q, nodes + c, sizeof(q));
+  return q;
 }
 
-// Descends from entry cur, whose window w is loaded, to the leaf containing
+// Descends from entry cur, whose window q is loaded, to the leaf containing
 // pe (the down-walk of ray_functions.cuh:289-291: below if pe < split, else
-// above).  The below child of an entry is the next entry, so a run of
-// "below" steps stays inside the window; a step above, or the end of the
-// window, starts a new fetch.  Returns the leaf id.
+// above).  The below child of an entry is the next entry, so a "below" step
+// from the window's first entry needs no new load; any other step does.
+// Returns the leaf id.
 template <bool COUNT>
-__device__ __forceinline__ uint32_t descend(const NodeEntry* __restrict__ nodes, int32_t cur,
-                                            NodeEntry (&w)[kNodeWindow], f3 pe, unsigned& c_ent) {
+__device__ __forceinline__ uint32_t descend(const NodeEntry* __restrict__ nodes, int32_t cur, uint4 q, f3 pe,
+                                            unsigned& c_ent) {
   for (;;) {
-#pragma unroll
-    for (int k = 0; k < kNodeWindow; ++k) {
-      const NodeEntry e = w[k];
+    if (COUNT) ++c_ent;
+    uint32_t meta = q.y;
+    uint32_t ax = meta >> 30;
+    if (ax == 3u) return meta & 0x3fffffffu;
+    if (get(pe, (int)ax) < __uint_as_float(q.x)) {
       if (COUNT) ++c_ent;
-      const uint32_t ax = e.meta >> 30;
-      const uint32_t other = e.meta & 0x3fffffffu;
-      if (ax == 3u) return other;
-      if (get(pe, (int)ax) < e.split) {
-        ++cur;
-      } else {
-        cur = (int32_t)other;
-        break;
-      }
+      meta = q.w;
+      ax = meta >> 30;
+      if (ax == 3u) return meta & 0x3fffffffu;
+      cur = (get(pe, (int)ax) < __uint_as_float(q.z)) ? cur + 2 : (int32_t)(meta & 0x3fffffffu);
+    } else {
+      cur = (int32_t)(meta & 0x3fffffffu);
     }
-    load_window(nodes, cur, w);
+    q = load_window(nodes, cur);
   }
 }
 
 // Closest hit of one ray over objects [k0, k1): stackless_kdtree_traversal and
 // trace_scene, RayEngine/src/cuda-src/ray_functions.cuh:230-396.
 template <bool COUNT, bool ONE_SIDED>
-__device__ __forceinline__ void trace_ray(const NodeEntry* __restrict__ nodes, const LeafRec* __restrict__ leaves,
+__device__ __forceinline__ void trace_ray(const NodeEntry* __restrict__ nodes, const DevLeaf* __restrict__ leaves,
                                           const TriRec* __restrict__ tris, const ObjXform* __restrict__ objs,
                                           int32_t k0, int32_t k1, f3 o, f3 d, int32_t cap, bool secondary,
                                           float& best_t, int32_t& best_prim, int32_t& best_obj, RayCounts& rc) {
@@ -106,7 +107,7 @@ __device__ __forceinline__ void trace_ray(const NodeEntry* __restrict__ nodes, c
     float hit_t = kInf;
     int32_t hit_prim = -1;
     int32_t visits = 0;
-    NodeEntry w[kNodeWindow];
+    uint4 w = make_uint4(0u, 0u, 0u, 0u);
     bool have_window = false;
     while (t_entry < t_exit && visits < cap) {
       ++visits;
@@ -115,22 +116,28 @@ __device__ __forceinline__ void trace_ray(const NodeEntry* __restrict__ nodes, c
       if (ref_is_leaf(cur)) {
         leaf_id = ref_leaf_id(cur);
       } else {
-        if (!have_window) load_window(nodes, cur, w);
+        if (!have_window) w = load_window(nodes, cur);
         leaf_id = descend<COUNT>(nodes, cur, w, pe, rc.ent);
       }
-      const LeafRec L = leaves[leaf_id];
+      // The box and the triangle run (32 bytes); the exit face and its rope
+      // depend on the box alone, so they are found before the triangle tests
+      // and only that rope is loaded; the first window of the next descent is
+      // fetched while the triangles are.  Results are unchanged: the
+      // reference's order (triangles, then exit, :295-347) only matters for
+      // what the loop does next, which still waits for the tests.
+      const DevLeaf* lp = leaves + leaf_id;
+      const float4 h0 = *reinterpret_cast<const float4*>(lp->bmin);
+      const float4 h1 = *reinterpret_cast<const float4*>(lp->bmax + 1);
+      const float bmin[3] = {h0.x, h0.y, h0.z}, bmax[3] = {h0.w, h1.x, h1.y};
+      const uint32_t first = __float_as_uint(h1.z), count = __float_as_uint(h1.w);
       if (COUNT) ++rc.leaf;
-      // The exit face and its rope depend on the leaf box alone, so they are
-      // found before the triangle tests, and the first window of the next
-      // descent is fetched while the triangles are.  Results are unchanged:
-      // the reference's order (triangles, then exit, :295-347) only matters
-      // for what the loop does next, which still waits for the tests.
       float tn, tf;
-      const bool box_ok = ray_box(L.bmin, L.bmax, ro, rd, tn, tf);
-      const int32_t r = box_ok ? exit_rope(L, add(ro, smul(tf, rd))) : kNoRope;
+      const bool box_ok = ray_box(bmin, bmax, ro, rd, tn, tf);
+      const int face = box_ok ? exit_face(bmin, bmax, add(ro, smul(tf, rd))) : -1;
+      const int32_t r = face >= 0 ? lp->rope[face] : kNoRope;
       have_window = r >= 0;
-      if (have_window) load_window(nodes, r, w);
-      test_leaf<ONE_SIDED, COUNT>(tris + L.first, L.count, ro, rd, t_exit, found, hit_t, hit_prim, rc.tri);
+      if (have_window) w = load_window(nodes, r);
+      test_leaf<ONE_SIDED, COUNT>(tris + first, count, ro, rd, t_exit, found, hit_t, hit_prim, rc.tri);
       if (!box_ok) break;
       t_entry = tf;
       if (r == kNoRope) break;

@@ -20,15 +20,18 @@ const std::vector<NodeEntry>& kd_nodes(const re_kdtree* t);
 const std::vector<LeafRec>& kd_leaves(const re_kdtree* t);
 const std::vector<TriRec>& kd_tris(const re_kdtree* t);
 
-// A scene resident in HBM: the concatenated trees of its objects.
+// A scene resident in HBM: the concatenated trees of its objects (leaf
+// records in the device order of DevLeaf; the host keeps the flat records for
+// re_scene_export).
 struct DevScene {
   NodeEntry* nodes = nullptr;
-  LeafRec* leaves = nullptr;
+  DevLeaf* leaves = nullptr;
   TriRec* tris = nullptr;
   ObjXform* objs = nullptr;
   int32_t n_obj = 0;
   int64_t n_nodes = 0, n_leaves = 0, n_trirecs = 0;
   int64_t bytes = 0;
+  std::vector<LeafRec> h_leaves;
 };
 
 // Per-ray work counters of a traversal: node entries read, leaf records read,
@@ -168,7 +171,7 @@ struct BatchSdf {
 
 struct BatchLaunch {
   const NodeEntry* nodes;
-  const LeafRec* leaves;
+  const DevLeaf* leaves;
   const TriRec* tris;
   const ObjXform* objs;
   const BatchTile* tiles;

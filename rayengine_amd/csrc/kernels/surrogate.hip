@@ -12,6 +12,11 @@ namespace re {
 namespace {
 
 using namespace mlp;
+
+// Work-group size of the kernels that stage the network in LDS (42 KB per
+// group): 8 waves share one copy, so LDS allows 3 groups = 24 waves per CU
+// and the register budget, not LDS, sets the occupancy.
+constexpr int kSurrBlock = 512;
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
@@ -24,15 +29,19 @@ __device__ __forceinline__ f32x4 mfma16(bf16x8 a, bf16x8 b, f32x4 c) {
 __device__ __forceinline__ float sin_rev(float x) { return __builtin_amdgcn_sinf(__builtin_amdgcn_fractf(x)); }
 __device__ __forceinline__ float cos_rev(float x) { return __builtin_amdgcn_cosf(__builtin_amdgcn_fractf(x)); }
 
-// Hidden activation, hard SiLU: z * clamp(z/4 + 1/2, 0, 1).  It has SiLU's
-// value, slope and curvature at 0, is z above 2 and 0 below -2, and costs
-// two or three full-rate VALU instructions (SiLU's exp and divide are
-// quarter-rate transcendentals plus a division sequence).
-__device__ __forceinline__ float hsilu(float z) { return z * __builtin_amdgcn_fmed3f(fmaf(z, 0.25f, 0.5f), 0.0f, 1.0f); }
+// Hidden activation, SiLU: z * sigmoid(z), with sigmoid(z) =
+// 1 / (1 + 2^(-z log2 e)) from the hardware exp2 and reciprocal (about 1 ulp
+// each; the activations are rounded to BF16 anyway), not the IEEE division
+// sequence.
+__device__ __forceinline__ float sigmoid_fast(float z) {
+  return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(z * -1.44269504089f));
+}
+__device__ __forceinline__ float silu(float z) { return z * sigmoid_fast(z); }
 
-// Its derivative: z/2 + 1/2 on (-2, 2), 1 above, 0 below.
-__device__ __forceinline__ float dhsilu(float z) {
-  return fabsf(z) < 2.0f ? fmaf(z, 0.5f, 0.5f) : (z > 0.0f ? 1.0f : 0.0f);
+// Its derivative: s (1 + z (1 - s)) with s = sigmoid(z).
+__device__ __forceinline__ float dsilu(float z) {
+  const float s = sigmoid_fast(z);
+  return s * (1.0f + z * (1.0f - s));
 }
 
 // Device PCG32 (same generator as the host scene generators).
@@ -111,7 +120,7 @@ __device__ __forceinline__ float forward_tile(const bf16x8* __restrict__ packF, 
       for (int r = 0; r < 4; ++r) {
         const float z = acc[m][r];
         if (KEEP) T.z[0][m][r] = z;
-        T.hb[0][m >> 1][(m & 1) * 4 + r] = (__bf16)hsilu(z);
+        T.hb[0][m >> 1][(m & 1) * 4 + r] = (__bf16)silu(z);
       }
   }
   // layers 1..3: 2 K-steps of 32 neurons, chained in registers
@@ -133,7 +142,7 @@ __device__ __forceinline__ float forward_tile(const bf16x8* __restrict__ packF, 
       for (int r = 0; r < 4; ++r) {
         const float z = acc[m][r];
         if (KEEP) T.z[L][m][r] = z;
-        const float h = hsilu(z);
+        const float h = silu(z);
         T.hb[L][m >> 1][(m & 1) * 4 + r] = (__bf16)h;
         if (L == 3) T.h3[m][r] = h;
       }
@@ -160,7 +169,7 @@ __device__ __forceinline__ void load_weights(const bf16x8* __restrict__ gpack, c
 
 // ----------------------------------------------------------------- inference
 
-__global__ void __launch_bounds__(256) mlp_infer_kernel(const bf16x8* __restrict__ gpack,
+__global__ void __launch_bounds__(512) mlp_infer_kernel(const bf16x8* __restrict__ gpack,
                                                         const float* __restrict__ gsmall,
                                                         const float* __restrict__ pts, int64_t n,
                                                         float* __restrict__ out) {
@@ -241,7 +250,7 @@ __global__ void __launch_bounds__(256) mlp_train_kernel(const bf16x8* __restrict
 #pragma unroll
     for (int m = 0; m < 4; ++m)
 #pragma unroll
-      for (int r2 = 0; r2 < 4; ++r2) dz[m][r2] = dout * S.small[256 + 16 * m + 4 * g + r2] * dhsilu(T.z[3][m][r2]);
+      for (int r2 = 0; r2 < 4; ++r2) dz[m][r2] = dout * S.small[256 + 16 * m + 4 * g + r2] * dsilu(T.z[3][m][r2]);
 #pragma unroll
     for (int L = 3; L >= 0; --L) {
       // stage dZ_L and the layer's input activations
@@ -266,7 +275,7 @@ __global__ void __launch_bounds__(256) mlp_train_kernel(const bf16x8* __restrict
 #pragma unroll
       for (int m = 0; m < 4; ++m)
 #pragma unroll
-        for (int r2 = 0; r2 < 4; ++r2) dz[m][r2] = dh[m][r2] * dhsilu(T.z[L - 1][m][r2]);
+        for (int r2 = 0; r2 < 4; ++r2) dz[m][r2] = dh[m][r2] * dsilu(T.z[L - 1][m][r2]);
     }
     // stage activations: features, h0..h3, and dout
 #pragma unroll
@@ -548,14 +557,14 @@ __device__ __forceinline__ float wave_surrogate(const bf16x8* spack, const float
 // exceeds tau, then hand over to the exact phase; after rejoin of exact
 // travel without a crossing, return to the surrogate.
 template <bool USE_MLP>
-__global__ void __launch_bounds__(256) hybrid_trace_kernel(
+__global__ void __launch_bounds__(512) hybrid_trace_kernel(
     VolView V, float bx1, float by1, float bz1, const bf16x8* __restrict__ gpack, const float* __restrict__ gsmall,
     const float* __restrict__ ray_o, const float* __restrict__ ray_d, int64_t n, HybridParams hp,
     float* __restrict__ t_out, uint8_t* __restrict__ hit_out, int32_t* __restrict__ evals_out,
     int32_t* __restrict__ samples_out) {
   __shared__ bf16x8 spack[USE_MLP ? (oPackB1 >> 3) : 1];
   __shared__ __align__(16) float ssmall[kSmall];
-  __shared__ float slots[USE_MLP ? 4 : 1][4 * 64];  // per wave: x, y, z, result
+  __shared__ float slots[USE_MLP ? kSurrBlock / 64 : 1][4 * 64];  // per wave: x, y, z, result
   if (USE_MLP) load_weights(gpack, gsmall, spack, ssmall, oPackB1 >> 3);
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t ic = i < n ? i : n - 1;
@@ -658,14 +667,15 @@ hipError_t launch_hybrid_trace(const DevVolume& field, const void* pack, const f
                                const float* ray_d, int64_t n, const HybridParams& hp, bool use_mlp, float* t,
                                uint8_t* hit, int32_t* evals, int32_t* samples, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  const unsigned blocks = (unsigned)((n + 255) / 256);
   const VolView V = field.view();
   if (use_mlp)
-    hipLaunchKernelGGL(hybrid_trace_kernel<true>, dim3(blocks), dim3(256), 0, s, V, field.box_max[0],
+    hipLaunchKernelGGL(hybrid_trace_kernel<true>, dim3((unsigned)((n + kSurrBlock - 1) / kSurrBlock)),
+                       dim3(kSurrBlock), 0, s, V, field.box_max[0],
                        field.box_max[1], field.box_max[2], (const bf16x8*)pack, small, ray_o, ray_d, n, hp, t, hit,
                        evals, samples);
   else
-    hipLaunchKernelGGL(hybrid_trace_kernel<false>, dim3(blocks), dim3(256), 0, s, V, field.box_max[0],
+    hipLaunchKernelGGL(hybrid_trace_kernel<false>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, V,
+                       field.box_max[0],
                        field.box_max[1], field.box_max[2], (const bf16x8*)pack, small, ray_o, ray_d, n, hp, t, hit,
                        evals, samples);
   return hipGetLastError();
@@ -677,10 +687,10 @@ hipError_t launch_mlp_infer(const void* pack, const float* small, const float* p
                             hipStream_t s) {
   if (n <= 0) return hipSuccess;
   const int64_t tiles = (n + 15) / 16;
-  int64_t blocks = (tiles + 3) / 4;
+  int64_t blocks = (tiles + kSurrBlock / 64 - 1) / (kSurrBlock / 64);
   if (blocks > 8192) blocks = 8192;
-  hipLaunchKernelGGL(mlp_infer_kernel, dim3((unsigned)blocks), dim3(256), 0, s, (const bf16x8*)pack, small, pts, n,
-                     out);
+  hipLaunchKernelGGL(mlp_infer_kernel, dim3((unsigned)blocks), dim3(kSurrBlock), 0, s, (const bf16x8*)pack, small,
+                     pts, n, out);
   return hipGetLastError();
 }
 

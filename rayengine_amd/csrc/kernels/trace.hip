@@ -21,7 +21,7 @@ namespace {
 // secondary rays a renderer appends after its primaries) they start first and
 // the cheap ones fill in behind them instead of forming the kernel's tail.
 template <bool COUNT, bool ONE_SIDED, bool REVERSE>
-__device__ __forceinline__ void closest_body(const NodeEntry* __restrict__ nodes, const LeafRec* __restrict__ leaves,
+__device__ __forceinline__ void closest_body(const NodeEntry* __restrict__ nodes, const DevLeaf* __restrict__ leaves,
                                              const TriRec* __restrict__ tris, const ObjXform* __restrict__ objs,
                                              int32_t n_obj, const float* __restrict__ ray_o,
                                              const float* __restrict__ ray_d, int64_t n, int32_t cap,
@@ -50,7 +50,7 @@ __device__ __forceinline__ void closest_body(const NodeEntry* __restrict__ nodes
 }
 
 #define RE_CLOSEST_PARAMS                                                                                  \
-  const NodeEntry *__restrict__ nodes, const LeafRec *__restrict__ leaves, const TriRec *__restrict__ tris, \
+  const NodeEntry *__restrict__ nodes, const DevLeaf *__restrict__ leaves, const TriRec *__restrict__ tris, \
       const ObjXform *__restrict__ objs, int32_t n_obj, const float *__restrict__ ray_o,                    \
       const float *__restrict__ ray_d, int64_t n, int32_t cap, int32_t secondary,                          \
       int32_t *__restrict__ prim_out, float *__restrict__ t_out, int32_t *__restrict__ obj_out,            \
@@ -63,8 +63,7 @@ __global__ void __launch_bounds__(256) trace_closest_kernel(RE_CLOSEST_PARAMS) {
   closest_body<COUNT, ONE_SIDED, REVERSE>(RE_CLOSEST_ARGS);
 }
 
-// The same with the register budget cut to 80 VGPRs (6 waves per SIMD
-// instead of 5); the compiler spills a few values to scratch for it.
+// The same with the register budget held to 80 VGPRs (6 waves per SIMD).
 template <bool COUNT, bool ONE_SIDED>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6)))
 trace_closest_o6_kernel(RE_CLOSEST_PARAMS) {
@@ -83,7 +82,7 @@ constexpr int kChunk = 64;
 
 template <bool COUNT, bool ONE_SIDED>
 __global__ void __launch_bounds__(256) trace_persistent_kernel(
-    const NodeEntry* __restrict__ nodes, const LeafRec* __restrict__ leaves,
+    const NodeEntry* __restrict__ nodes, const DevLeaf* __restrict__ leaves,
     const TriRec* __restrict__ tris, const ObjXform* __restrict__ objs, int32_t n_obj,
     const float* __restrict__ ray_o, const float* __restrict__ ray_d, int64_t n, int32_t cap,
     int32_t secondary, int32_t* __restrict__ prim_out, float* __restrict__ t_out,
@@ -159,11 +158,10 @@ __global__ void __launch_bounds__(256) trace_persistent_kernel(
       if (ref_is_leaf(cur)) {
         leaf_id = ref_leaf_id(cur);
       } else {
-        NodeEntry w[kNodeWindow];
-        load_window(nodes, cur, w);
-        leaf_id = descend<COUNT>(nodes, cur, w, pe, c_ent);
+        leaf_id = descend<COUNT>(nodes, cur, load_window(nodes, cur), pe, c_ent);
       }
-      const LeafRec L = leaves[leaf_id];
+      const DevLeaf* lp = leaves + leaf_id;
+      const DevLeaf L = *lp;
       if (COUNT) ++c_leaf;
       test_leaf<ONE_SIDED, COUNT>(tris + L.first, L.count, ro, rd, t_exit, found, hit_t, hit_prim, c_tri);
       float tn, tf;
@@ -171,7 +169,8 @@ __global__ void __launch_bounds__(256) trace_persistent_kernel(
         live = false;
       } else {
         t_entry = tf;
-        const int32_t r = exit_rope(L, add(ro, smul(t_entry, rd)));
+        const int face = exit_face(L.bmin, L.bmax, add(ro, smul(t_entry, rd)));
+        const int32_t r = face >= 0 ? lp->rope[face] : kNoRope;
         if (r == kNoRope) {
           live = false;
         } else {

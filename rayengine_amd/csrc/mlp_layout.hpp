@@ -2,8 +2,7 @@
 // of its MFMA kernels.  The surrogate has no counterpart in the reference.
 //
 // Network: 99 inputs (16-frequency Fourier features of the 3 coordinates, then
-// the raw coordinates), 4 hidden layers of 64 hard-SiLU units
-// (z * clamp(z/4 + 1/2, 0, 1)), one linear output.
+// the raw coordinates), 4 hidden layers of 64 SiLU units, one linear output.
 //   feature 32 d + 2 k     = sin(2 pi f_k p_d),  f_k = 2^(k/2), k < 16, d < 3
 //   feature 32 d + 2 k + 1 = cos(2 pi f_k p_d)
 //   features 96..98        = p.x, p.y, p.z
