@@ -212,6 +212,16 @@ RE_API re_status re_trace_counts(re_ctx* ctx, int32_t scene_id, const float* ray
                                  const float* ray_d, int64_t n, const re_trace_params* params,
                                  int32_t* prim_id, float* t, uint64_t counts[4], uint32_t flags);
 
+/* Any-hit (shadow) query.  Replaces trace_shadow
+ * (RayEngine/src/cuda-src/ray_functions.cuh:365-374), whose callers
+ * (phong_light / shade, RayEngine/src/cuda-src/material_functions.cuh:68-119)
+ * use only its `hits` flag: occluded[i] = 1 when the closest-hit walk of some
+ * object detects an intersection (same walk, visit cap and acceptance rule as
+ * re_trace_closest; a walk stops at its first accepted hit).  params NULL
+ * means the defaults with secondary = 1, as trace_shadow traverses. */
+RE_API re_status re_trace_occluded(re_ctx* ctx, int32_t scene_id, const float* ray_o, const float* ray_d,
+                                   int64_t n, const re_trace_params* params, uint8_t* occluded, uint32_t flags);
+
 /* ------------------------------------------------------ terrain volume */
 
 /* Doubled permutation table (512 entries) of RPerlinNoise(seed)

@@ -22,6 +22,7 @@ _SIGS = {
     "orc_trace": (None, [_P, _P, _P, _P, C.c_int, _P, _P, C.c_int64, C.c_int, C.c_int, C.c_int, _P, _P, _P, _P,
                          C.c_int]),
     "orc_brute": (None, [_P, C.c_int64, _P, _P, _P, _P, C.c_int64, C.c_int, _P, _P, C.c_int]),
+    "orc_occluded": (None, [_P, _P, _P, _P, C.c_int, _P, _P, C.c_int64, C.c_int, C.c_int, C.c_int, _P, C.c_int]),
     "orc_brute_f64": (None, [_P, C.c_int64, _P, _P, C.c_int64, C.c_int, _P, _P, C.c_int]),
     "orc_ray_tri": (C.c_int, [_P, _P, _P, _P, _P, C.c_int, _P, _P]),
     "orc_ray_box": (C.c_int, [_P, _P, _P, _P, _P, _P]),
@@ -84,6 +85,16 @@ def trace(nodes, leaves, tris, objs, o, d, cap=50, two_sided=False, secondary=Fa
     lib().orc_trace(_p(nodes), _p(leaves), _p(tris), _p(objs), len(objs), _p(o), _p(d), n, cap, int(two_sided),
                     int(secondary), _p(prim), _p(t), None, _p(cnt), threads)
     return (prim, t, cnt) if counts else (prim, t)
+
+
+def occluded(nodes, leaves, tris, objs, o, d, cap=50, two_sided=False, secondary=True, threads=0):
+    """trace_shadow: uint8 per ray, 1 when some object's walk detects a hit."""
+    o, d = _f(o), _f(d)
+    n = o.size // 3
+    out = np.empty(n, np.uint8)
+    lib().orc_occluded(_p(nodes), _p(leaves), _p(tris), _p(objs), len(objs), _p(o), _p(d), n, cap, int(two_sided),
+                       int(secondary), _p(out), threads)
+    return out
 
 
 def brute(tri_v, root_min, root_max, o, d, two_sided=False, threads=0):

@@ -265,6 +265,39 @@ ORC_API void orc_trace(const void* nodes, const void* leaves, const void* tris, 
   }
 }
 
+// trace_shadow (RayEngine/src/cuda-src/ray_functions.cuh:365-374): the
+// closest-hit walk of every object (stackless_kdtree_traversal, is_secondary
+// as given; the reference passes true), each overwriting the shared
+// HitResult when it detects an intersection.  Its callers
+// (material_functions.cuh:68-119) read only `hits`, so the result is 1 when
+// some object's walk detects an intersection.  Pinned by its relation to
+// orc_trace (per object, detected <=> a closest hit) and by known answers.
+ORC_API void orc_occluded(const void* nodes, const void* leaves, const void* tris, const void* objs, int n_obj,
+                          const float* ray_o, const float* ray_d, int64_t n, int cap, int two_sided, int secondary,
+                          uint8_t* out, int threads) {
+  const Node* N = (const Node*)nodes;
+  const Leaf* L = (const Leaf*)leaves;
+  const Tri* T = (const Tri*)tris;
+  const Obj* O = (const Obj*)objs;
+  if (cap <= 0 || cap > (1 << 20)) cap = 1 << 20;
+#ifdef _OPENMP
+  if (threads <= 0) threads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 64) num_threads(threads)
+#endif
+  for (int64_t i = 0; i < n; ++i) {
+    V3<float> o{ray_o[3 * i], ray_o[3 * i + 1], ray_o[3 * i + 2]};
+    V3<float> d{ray_d[3 * i], ray_d[3 * i + 1], ray_d[3 * i + 2]};
+    bool hits = false;
+    for (int k = 0; k < n_obj; ++k) {
+      float t = K_INFINITY;
+      int32_t p = -1;
+      Counts c{0, 0, 0, 0};
+      if (traverse(N, L, T, O[k], o, d, cap, two_sided != 0, secondary != 0, t, p, c)) hits = true;
+    }
+    out[i] = hits ? 1 : 0;
+  }
+}
+
 // Brute force over a triangle soup (9 floats per triangle), the check the
 // reference's gpu_bruteforce_ray_cast performs (RayEngine/src/cuda-src/renderer.cu:328-382):
 // root-box gate, then every triangle in index order with a strict t < best.

@@ -516,6 +516,25 @@ class Scene:
                                      ptr(obj), 0), self.ctx.handle)
         return (prim, t, obj) if with_obj else (prim, t)
 
+    def occluded(self, o, d, **params):
+        """Any-hit (shadow) query: uint8 per ray, 1 when some object is hit
+        (trace_shadow; traverses with secondary=True unless given)."""
+        o, d = _f32c(o, 3), _f32c(d, 3)
+        n = o.size // 3
+        out = np.empty(n, np.uint8)
+        params.setdefault("secondary", True)
+        p = _trace_params(**params)
+        check(lib().re_trace_occluded(self.ctx.handle, self.id, ptr(o), ptr(d), n, C.byref(p), ptr(out), 0),
+              self.ctx.handle)
+        return out
+
+    def occluded_device(self, o_ptr: int, d_ptr: int, n: int, out_ptr: int, sync=True, **params):
+        params.setdefault("secondary", True)
+        p = _trace_params(**params)
+        flags = RE_DEVICE_PTRS | (0 if sync else RE_ASYNC)
+        check(lib().re_trace_occluded(self.ctx.handle, self.id, o_ptr, d_ptr, n, C.byref(p), out_ptr, flags),
+              self.ctx.handle)
+
     def trace_closest_device(self, o_ptr: int, d_ptr: int, n: int, prim_ptr: int, t_ptr: int, sync=True, **params):
         """Same on device-resident buffers (raw device pointers)."""
         p = _trace_params(**params)

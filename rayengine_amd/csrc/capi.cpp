@@ -382,4 +382,53 @@ RE_API re_status re_trace_counts(re_ctx* ctx, int32_t scene_id, const float* ray
   return st;
 }
 
+RE_API re_status re_trace_occluded(re_ctx* ctx, int32_t scene_id, const float* ray_o, const float* ray_d, int64_t n,
+                                   const re_trace_params* params, uint8_t* occluded, uint32_t flags) {
+  if (!ctx) return fail(RE_ERR_INVALID, "re_trace_occluded: null ctx");
+  if (n < 0 || (n > 0 && (!ray_o || !ray_d || !occluded)))
+    return set_err(ctx, RE_ERR_INVALID, "re_trace_occluded: bad arguments");
+  auto it = ctx->scenes.find(scene_id);
+  if (it == ctx->scenes.end()) return set_err(ctx, RE_ERR_NOT_FOUND, "re_trace_occluded: unknown scene");
+  re_trace_params p;
+  re_trace_default_params(&p);
+  p.secondary = 1;  // trace_shadow traverses with is_secondary = true
+  if (params) p = *params;
+  if (!(p.block == 0 || p.block == 64 || p.block == 128 || p.block == 256))
+    return set_err(ctx, RE_ERR_INVALID, "re_trace_occluded: block must be 0, 64, 128 or 256");
+  if (n == 0) return RE_OK;
+  HIP_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+  TraceLaunch L{};
+  L.scene = &it->second;
+  L.n = n;
+  L.cap = p.max_leaf_visits;
+  L.one_sided = p.one_sided;
+  L.secondary = p.secondary;
+  L.block = p.block;
+  const bool dev = (flags & RE_DEVICE_PTRS) != 0;
+  const size_t ray_bytes = (size_t)n * 12;
+  if (!dev) {
+    re_status st = ensure_stage(ctx, stage_size({ray_bytes, ray_bytes, (size_t)n}));
+    if (st) return st;
+    StageCursor c{(char*)ctx->stage};
+    float* d_o = c.take<float>(3 * n);
+    float* d_d = c.take<float>(3 * n);
+    L.occluded = c.take<uint8_t>(n);
+    HIP_TRY(ctx, hipMemcpyAsync(d_o, ray_o, ray_bytes, hipMemcpyHostToDevice, ctx->stream), "copy ray_o");
+    HIP_TRY(ctx, hipMemcpyAsync(d_d, ray_d, ray_bytes, hipMemcpyHostToDevice, ctx->stream), "copy ray_d");
+    L.ray_o = d_o;
+    L.ray_d = d_d;
+  } else {
+    L.ray_o = ray_o;
+    L.ray_d = ray_d;
+    L.occluded = occluded;
+  }
+  HIP_TRY(ctx, ctx->launch_begin(), "event");
+  HIP_TRY(ctx, launch_trace_occluded(L, ctx->stream), "trace_occluded launch");
+  HIP_TRY(ctx, ctx->launch_end(), "event");
+  if (!dev)
+    HIP_TRY(ctx, hipMemcpyAsync(occluded, L.occluded, (size_t)n, hipMemcpyDeviceToHost, ctx->stream), "copy occluded");
+  if (!(flags & RE_ASYNC) || !dev) HIP_TRY(ctx, hipStreamSynchronize(ctx->stream), "trace_occluded sync");
+  return RE_OK;
+}
+
 }  // extern "C"

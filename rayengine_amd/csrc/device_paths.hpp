@@ -20,11 +20,15 @@ __device__ __forceinline__ TriGeom load_geom(const TriRec* __restrict__ tp, uint
 // ray_functions.cuh:295-324).  The geometry of up to four records is loaded
 // before any of them is tested, so a lane has four independent loads in
 // flight instead of one; the prim id is read only for an accepted hit.
-template <bool ONE_SIDED, bool COUNT>
+//
+// ANY: stop at the first accepted hit (any-hit query; the reference's
+// intersection_detected becomes true there and never reverts).
+template <bool ONE_SIDED, bool COUNT, bool ANY = false>
 __device__ __forceinline__ void test_leaf(const TriRec* __restrict__ tp, uint32_t count, f3 ro, f3 rd,
                                           float& t_exit, bool& found, float& hit_t, int32_t& hit_prim,
                                           unsigned& c_tri) {
   for (uint32_t j = 0; j < count; j += 4) {
+    if (ANY && found) return;
     TriGeom g[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q)
@@ -41,6 +45,7 @@ __device__ __forceinline__ void test_leaf(const TriRec* __restrict__ tp, uint32_
         hit_t = t;
         hit_prim = tp[j + q].prim;
         t_exit = t;
+        if (ANY) break;
       }
     }
   }
@@ -87,8 +92,12 @@ __device__ __forceinline__ uint32_t descend(const NodeEntry* __restrict__ nodes,
 }
 
 // Closest hit of one ray over objects [k0, k1): stackless_kdtree_traversal and
-// trace_scene, RayEngine/src/cuda-src/ray_functions.cuh:230-396.
-template <bool COUNT, bool ONE_SIDED>
+// trace_scene, RayEngine/src/cuda-src/ray_functions.cuh:230-396.  With ANY the
+// walk of an object stops at its first accepted hit and the objects after the
+// first that reports one are skipped: best_prim >= 0 then says whether any
+// object detects an intersection, which is all trace_shadow
+// (ray_functions.cuh:365-374) delivers to its callers.
+template <bool COUNT, bool ONE_SIDED, bool ANY = false>
 __device__ __forceinline__ void trace_ray(const NodeEntry* __restrict__ nodes, const DevLeaf* __restrict__ leaves,
                                           const TriRec* __restrict__ tris, const ObjXform* __restrict__ objs,
                                           int32_t k0, int32_t k1, f3 o, f3 d, int32_t cap, bool secondary,
@@ -137,7 +146,8 @@ __device__ __forceinline__ void trace_ray(const NodeEntry* __restrict__ nodes, c
       const int32_t r = face >= 0 ? lp->rope[face] : kNoRope;
       have_window = r >= 0;
       if (have_window) w = load_window(nodes, r);
-      test_leaf<ONE_SIDED, COUNT>(tris + first, count, ro, rd, t_exit, found, hit_t, hit_prim, rc.tri);
+      test_leaf<ONE_SIDED, COUNT, ANY>(tris + first, count, ro, rd, t_exit, found, hit_t, hit_prim, rc.tri);
+      if (ANY && found) break;
       if (!box_ok) break;
       t_entry = tf;
       if (r == kNoRope) break;
@@ -148,6 +158,7 @@ __device__ __forceinline__ void trace_ray(const NodeEntry* __restrict__ nodes, c
       best_prim = hit_prim;
       best_obj = k;
     }
+    if (ANY && found) return;
   }
 }
 

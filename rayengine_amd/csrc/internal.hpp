@@ -53,6 +53,7 @@ struct TraceLaunch {
   float* t;
   int32_t* obj;
   TraceCounts* counts;  // device pointer or null
+  uint8_t* occluded;    // any-hit query: 1 when some object detects a hit
   int32_t variant;      // 0 persistent with ray refill, 1 one ray per thread
   int64_t grid_blocks;  // persistent grid size (blocks)
   unsigned long long* queue;  // persistent ray counter, zeroed before launch
@@ -63,6 +64,7 @@ struct TraceLaunch {
 int64_t trace_persistent_grid(int block);
 
 hipError_t launch_trace_closest(const TraceLaunch& L, hipStream_t s);
+hipError_t launch_trace_occluded(const TraceLaunch& L, hipStream_t s);
 
 // A volume resident in HBM: 8x8x8 Morton-ordered bricks.
 struct DevVolume {

@@ -70,6 +70,28 @@ trace_closest_o6_kernel(RE_CLOSEST_PARAMS) {
   closest_body<COUNT, ONE_SIDED, true>(RE_CLOSEST_ARGS);
 }
 
+// Any-hit query (trace_shadow, RayEngine/src/cuda-src/ray_functions.cuh:365-374):
+// one ray per lane, last block first, 6-wave budget; each ray stops at its
+// first accepted hit.
+template <bool ONE_SIDED>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6)))
+trace_occluded_kernel(const NodeEntry* __restrict__ nodes, const DevLeaf* __restrict__ leaves,
+                      const TriRec* __restrict__ tris, const ObjXform* __restrict__ objs, int32_t n_obj,
+                      const float* __restrict__ ray_o, const float* __restrict__ ray_d, int64_t n, int32_t cap,
+                      int32_t secondary, uint8_t* __restrict__ occluded) {
+  const int64_t n_blk = (n + blockDim.x - 1) / blockDim.x;
+  const int64_t i = (n_blk - 1 - (int64_t)blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const f3 o{ray_o[3 * i], ray_o[3 * i + 1], ray_o[3 * i + 2]};
+  const f3 d{ray_d[3 * i], ray_d[3 * i + 1], ray_d[3 * i + 2]};
+  float best_t;
+  int32_t best_prim, best_obj;
+  RayCounts rc;
+  trace_ray<false, ONE_SIDED, true>(nodes, leaves, tris, objs, 0, n_obj, o, d, cap, secondary != 0, best_t,
+                                    best_prim, best_obj, rc);
+  occluded[i] = best_prim >= 0 ? 1 : 0;
+}
+
 // ---------------------------------------------------------------------------
 // Persistent variant.  A wave keeps 64 rays in flight and refills each lane
 // as soon as its ray finishes, from a wave-private chunk of CHUNK
@@ -262,6 +284,21 @@ hipError_t launch_trace_closest(const TraceLaunch& L, hipStream_t s) {
   else RE_LAUNCH_ALL(trace_closest_kernel, , false);
 #undef RE_LAUNCH_ALL
 #undef RE_LAUNCH
+  return hipGetLastError();
+}
+
+hipError_t launch_trace_occluded(const TraceLaunch& L, hipStream_t s) {
+  if (L.n <= 0) return hipSuccess;
+  const int block = L.block > 0 ? L.block : 64;
+  const DevScene& S = *L.scene;
+  const int32_t cap = (L.cap > 0 && L.cap < kMaxLeafVisits) ? L.cap : kMaxLeafVisits;
+  const int64_t grid = (L.n + block - 1) / block;
+  if (L.one_sided)
+    hipLaunchKernelGGL(trace_occluded_kernel<true>, dim3((unsigned)grid), dim3(block), 0, s, S.nodes, S.leaves,
+                       S.tris, S.objs, S.n_obj, L.ray_o, L.ray_d, L.n, cap, L.secondary, L.occluded);
+  else
+    hipLaunchKernelGGL(trace_occluded_kernel<false>, dim3((unsigned)grid), dim3(block), 0, s, S.nodes, S.leaves,
+                       S.tris, S.objs, S.n_obj, L.ray_o, L.ray_d, L.n, cap, L.secondary, L.occluded);
   return hipGetLastError();
 }
 

@@ -62,6 +62,24 @@ def cluster_rays(tri_v, primary_prim, primary_t, width=1920, height=1080, order=
     return np.concatenate([o, bo]), np.concatenate([d, bd])
 
 
+# Shadow rays: from each primary hit, offset 1e-4 along the geometric normal
+# facing the incoming ray, towards a point light.
+CLUSTER_LIGHT = (20.0, 30.0, 25.0)
+
+
+def shadow_rays(tri_v, o, d, prim, t, light):
+    """(origin, direction) float32 of one shadow ray per hit (prim >= 0)."""
+    hit = np.flatnonzero(prim >= 0)
+    v = np.asarray(tri_v, np.float32).reshape(-1, 3, 3)[prim[hit]]
+    nrm = np.cross(v[:, 1] - v[:, 0], v[:, 2] - v[:, 0]).astype(np.float32)
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    facing = np.where(((nrm * d[hit]).sum(1) > 0)[:, None], -nrm, nrm)
+    org = (o[hit] + t[hit, None] * d[hit] + np.float32(1e-4) * facing).astype(np.float32)
+    dirv = np.asarray(light, np.float32) - org
+    dirv /= np.linalg.norm(dirv, axis=1, keepdims=True)
+    return org, dirv.astype(np.float32)
+
+
 # Terrain workload: 256^3 FP32 field over the unit cube (seed 2016), traced
 # with 1920x1080 rays from an elevated camera looking across the terrain.
 TERRAIN_CAMERA = Camera(pos=(0.5, 0.95, -0.35), view=(0.0, -0.55, 1.0))

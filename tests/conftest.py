@@ -38,3 +38,13 @@ def small():
     """Small icosphere workload: (tri_v, tree, ray_o, ray_d)."""
     from rayengine_amd import workloads as W
     return W.small_scene()
+
+
+@pytest.fixture(scope="session")
+def cluster():
+    """The 2^20-triangle clustered soup and its tree: (tri_v, tree)."""
+    import rayengine_amd as re
+    from rayengine_amd import workloads as W
+    print("building the 2^20-triangle cluster soup", flush=True)
+    tri_v = W.cluster_soup_default()
+    return tri_v, re.KDTree(tri_v)

@@ -149,14 +149,6 @@ def test_device_pointer_path(gpu_ctx, small):
     assert np.array_equal(tp.cpu().numpy(), p) and np.array_equal(tt.cpu().numpy().view(np.uint32), t.view(np.uint32))
 
 
-@pytest.fixture(scope="module")
-def cluster():
-    say("building the 2^20-triangle cluster soup")
-    tri_v = W.cluster_soup_default()
-    tree = re.KDTree(tri_v)
-    return tri_v, tree
-
-
 def test_cluster_scene_subset_bit_exact(gpu_ctx, cluster):
     """Full-size cluster workload on the GPU; a seeded subset of 65536 rays
     (primary and bounce) checked against the oracle."""
