@@ -20,7 +20,7 @@ LIB_PATH = HERE / "liboracle.so"
 _P = C.c_void_p
 _SIGS = {
     "orc_trace": (None, [_P, _P, _P, _P, C.c_int, _P, _P, C.c_int64, C.c_int, C.c_int, C.c_int, _P, _P, _P, _P,
-                         C.c_int]),
+                         C.c_int, C.c_int]),
     "orc_brute": (None, [_P, C.c_int64, _P, _P, _P, _P, C.c_int64, C.c_int, _P, _P, C.c_int]),
     "orc_occluded": (None, [_P, _P, _P, _P, C.c_int, _P, _P, C.c_int64, C.c_int, C.c_int, C.c_int, _P, C.c_int]),
     "orc_brute_f64": (None, [_P, C.c_int64, _P, _P, C.c_int64, C.c_int, _P, _P, C.c_int]),
@@ -72,18 +72,20 @@ def _f(a, k=None):
     return a
 
 
-def trace(nodes, leaves, tris, objs, o, d, cap=50, two_sided=False, secondary=False, counts=False, threads=0):
+def trace(nodes, leaves, tris, objs, o, d, cap=50, two_sided=False, secondary=False, counts=False, threads=0,
+          cut=False):
     """FP32 rope traversal; returns (prim, t[, counts]).  counts is (n, 4)
     uint64: node entries, leaf records, triangle tests, and how the walk ended
     (0 t_entry >= t_exit, 1 cap, 2 leaf slab miss, 3 no face within K_EPSILON,
-    4 rope -1, 5 root box missed)."""
+    4 rope -1, 5 root box missed, 6 walk state repeated).  cut stops a walk
+    whose state repeats (same results; the counters of the shortened walk)."""
     o, d = _f(o), _f(d)
     n = o.size // 3
     prim = np.empty(n, np.int32)
     t = np.empty(n, np.float32)
     cnt = np.empty((n, 4), np.uint64) if counts else None
     lib().orc_trace(_p(nodes), _p(leaves), _p(tris), _p(objs), len(objs), _p(o), _p(d), n, cap, int(two_sided),
-                    int(secondary), _p(prim), _p(t), None, _p(cnt), threads)
+                    int(secondary), _p(prim), _p(t), None, _p(cnt), threads, int(bool(cut)))
     return (prim, t, cnt) if counts else (prim, t)
 
 

@@ -149,13 +149,22 @@ void transform(const Obj& ob, V3<T> o, V3<T> d, bool secondary, V3<T>& no, V3<T>
 
 // Work counters of a ray and how its last traversal loop ended:
 // 0 t_entry >= t_exit, 1 visit cap, 2 leaf slab miss, 3 no face within
-// K_EPSILON, 4 rope -1 (left the root box), 5 root box missed.
+// K_EPSILON, 4 rope -1 (left the root box), 5 root box missed, 6 the walk
+// state repeated (only with cut).
 struct Counts { uint64_t entries, leaves, tris, reason; };
 
 // stackless_kdtree_traversal, RayEngine/src/cuda-src/ray_functions.cuh:230-360,
 // for one object.  Returns true with (t, prim) of the accepted hit.
+//
+// cut: stop when (cur, t_entry, t_exit) at the top of a visit equals its value
+// at one of the two visits before.  Nothing is accepted between two equal
+// states (an accepted hit lowers t_exit) and the next visit depends on that
+// state alone, so the uncut walk repeats them to the cap with the same result;
+// cut only changes the counters, to the work of the shortened walk the
+// library runs.  Equality of the results with and without cut is tested.
 bool traverse(const Node* nodes, const Leaf* leaves, const Tri* tris, const Obj& ob, V3<float> o, V3<float> d,
-              int cap, bool two_sided, bool secondary, float& t_hit, int32_t& prim, Counts& cnt) {
+              int cap, bool two_sided, bool secondary, float& t_hit, int32_t& prim, Counts& cnt,
+              bool cut = false) {
   V3<float> no, nd;
   transform(ob, o, d, secondary, no, nd);
   float t_entry, t_exit;
@@ -166,7 +175,22 @@ bool traverse(const Node* nodes, const Leaf* leaves, const Tri* tris, const Obj&
   bool detected = false;
   int limit = 0;
   int32_t cur = ob.root;  // current node as a ref
+  int32_t seen_c[2] = {-1, -1};  // a walk's cur is never -1
+  float seen_e[2] = {0.0f, 0.0f}, seen_x[2] = {0.0f, 0.0f};
   while (t_entry < t_exit && limit < cap) {
+    if (cut) {
+      for (int q = 0; q < 2; ++q)
+        if (seen_c[q] == cur && seen_e[q] == t_entry && seen_x[q] == t_exit) {
+          cnt.reason = 6;
+          return detected;
+        }
+      seen_c[1] = seen_c[0];
+      seen_e[1] = seen_e[0];
+      seen_x[1] = seen_x[0];
+      seen_c[0] = cur;
+      seen_e[0] = t_entry;
+      seen_x[0] = t_exit;
+    }
     ++limit;
     V3<float> p_entry = no + (t_entry * nd);
     // Down traversal to a leaf (:289-291).
@@ -225,10 +249,11 @@ bool traverse(const Node* nodes, const Leaf* leaves, const Tri* tris, const Obj&
 // RayEngine/src/Engine/RayTracer.cpp:154.  counts (nullable) gets 3 per ray.
 // cap <= 0 means the ceiling 2^20 (a ray whose exit point lies within
 // K_EPSILON of two faces can alternate between two leaves until the cap).  Pinned by brute-force equality at cap = inf
-// (orc_brute), the icosphere closed form and the primitive KATs.
+// (orc_brute), the icosphere closed form and the primitive KATs.  cut: see
+// traverse.
 ORC_API void orc_trace(const void* nodes, const void* leaves, const void* tris, const void* objs, int n_obj,
                        const float* ray_o, const float* ray_d, int64_t n, int cap, int two_sided, int secondary,
-                       int32_t* prim_out, float* t_out, int32_t* obj_out, uint64_t* counts, int threads) {
+                       int32_t* prim_out, float* t_out, int32_t* obj_out, uint64_t* counts, int threads, int cut) {
   const Node* N = (const Node*)nodes;
   const Leaf* L = (const Leaf*)leaves;
   const Tri* T = (const Tri*)tris;
@@ -247,7 +272,7 @@ ORC_API void orc_trace(const void* nodes, const void* leaves, const void* tris, 
     for (int k = 0; k < n_obj; ++k) {
       float t = K_INFINITY;
       int32_t p = -1;
-      if (traverse(N, L, T, O[k], o, d, cap, two_sided != 0, secondary != 0, t, p, c) && t < best) {
+      if (traverse(N, L, T, O[k], o, d, cap, two_sided != 0, secondary != 0, t, p, c, cut != 0) && t < best) {
         best = t;
         bp = p;
         bo = k;

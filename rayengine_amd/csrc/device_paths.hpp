@@ -91,6 +91,29 @@ __device__ __forceinline__ uint32_t descend(const NodeEntry* __restrict__ nodes,
   }
 }
 
+// The state that decides a walk's next visit is (cur, t_entry, t_exit).
+// When it equals the state of one of the last two visits, no hit was
+// accepted in between (an accepted hit lowers t_exit), so from here the walk
+// repeats the same visits, accepting nothing, until the visit cap: the
+// result is final and the walk stops.  This happens when a leaf's exit point
+// lies within the face tolerance of two faces and the ropes lead back and
+// forth between two leaves at the same t_entry (ray_functions.cuh:341-347);
+// the reference walks on to the cap.
+struct WalkMemo {
+  int32_t c1 = kNoRope, c2 = kNoRope;  // cur is never kNoRope inside a walk
+  float e1 = 0.0f, e2 = 0.0f, x1 = 0.0f, x2 = 0.0f;
+  __device__ __forceinline__ bool repeats(int32_t cur, float te, float tx) {
+    if ((cur == c1 && te == e1 && tx == x1) || (cur == c2 && te == e2 && tx == x2)) return true;
+    c2 = c1;
+    e2 = e1;
+    x2 = x1;
+    c1 = cur;
+    e1 = te;
+    x1 = tx;
+    return false;
+  }
+};
+
 // Closest hit of one ray over objects [k0, k1): stackless_kdtree_traversal and
 // trace_scene, RayEngine/src/cuda-src/ray_functions.cuh:230-396.  With ANY the
 // walk of an object stops at its first accepted hit and the objects after the
@@ -118,7 +141,9 @@ __device__ __forceinline__ void trace_ray(const NodeEntry* __restrict__ nodes, c
     int32_t visits = 0;
     uint4 w = make_uint4(0u, 0u, 0u, 0u);
     bool have_window = false;
+    WalkMemo memo;
     while (t_entry < t_exit && visits < cap) {
+      if (memo.repeats(cur, t_entry, t_exit)) break;
       ++visits;
       const f3 pe = add(ro, smul(t_entry, rd));
       uint32_t leaf_id;

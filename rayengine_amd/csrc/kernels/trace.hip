@@ -120,6 +120,7 @@ __global__ void __launch_bounds__(256) trace_persistent_kernel(
   float t_entry = 0.0f, t_exit = 0.0f, hit_t = kInf, best_t = kInf;
   int32_t cur = 0, visits = 0, k_obj = 0, hit_prim = -1, best_prim = -1, best_obj = -1;
   bool live = false, found = false;
+  WalkMemo memo;
   unsigned c_ent = 0, c_leaf = 0, c_tri = 0, c_hit = 0;
 
   for (;;) {
@@ -169,35 +170,40 @@ __global__ void __launch_bounds__(256) trace_persistent_kernel(
       found = false;
       hit_t = kInf;
       hit_prim = -1;
+      memo = WalkMemo{};
       live = (t_entry < t_exit && visits < cap);
     }
 
     // ---- one leaf visit ----
     if (live) {
-      ++visits;
-      const f3 pe = add(ro, smul(t_entry, rd));
-      uint32_t leaf_id;
-      if (ref_is_leaf(cur)) {
-        leaf_id = ref_leaf_id(cur);
-      } else {
-        leaf_id = descend<COUNT>(nodes, cur, load_window(nodes, cur), pe, c_ent);
-      }
-      const DevLeaf* lp = leaves + leaf_id;
-      const DevLeaf L = *lp;
-      if (COUNT) ++c_leaf;
-      test_leaf<ONE_SIDED, COUNT>(tris + L.first, L.count, ro, rd, t_exit, found, hit_t, hit_prim, c_tri);
-      float tn, tf;
-      if (!ray_box_inv(L.bmin, L.bmax, ro, fr, tn, tf)) {
+      if (memo.repeats(cur, t_entry, t_exit)) {
         live = false;
       } else {
-        t_entry = tf;
-        const int face = exit_face(L.bmin, L.bmax, add(ro, smul(t_entry, rd)));
-        const int32_t r = face >= 0 ? lp->rope[face] : kNoRope;
-        if (r == kNoRope) {
+        ++visits;
+        const f3 pe = add(ro, smul(t_entry, rd));
+        uint32_t leaf_id;
+        if (ref_is_leaf(cur)) {
+          leaf_id = ref_leaf_id(cur);
+        } else {
+          leaf_id = descend<COUNT>(nodes, cur, load_window(nodes, cur), pe, c_ent);
+        }
+        const DevLeaf* lp = leaves + leaf_id;
+        const DevLeaf L = *lp;
+        if (COUNT) ++c_leaf;
+        test_leaf<ONE_SIDED, COUNT>(tris + L.first, L.count, ro, rd, t_exit, found, hit_t, hit_prim, c_tri);
+        float tn, tf;
+        if (!ray_box_inv(L.bmin, L.bmax, ro, fr, tn, tf)) {
           live = false;
         } else {
-          cur = r;
-          live = (t_entry < t_exit && visits < cap);
+          t_entry = tf;
+          const int face = exit_face(L.bmin, L.bmax, add(ro, smul(t_entry, rd)));
+          const int32_t r = face >= 0 ? lp->rope[face] : kNoRope;
+          if (r == kNoRope) {
+            live = false;
+          } else {
+            cur = r;
+            live = (t_entry < t_exit && visits < cap);
+          }
         }
       }
       if (!live && found && hit_t < best_t) {
