@@ -223,6 +223,8 @@ def main():
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run 5 device copies of BYTES bytes (counter calibration)")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--autotune", action="store_true",
+                    help="pick leaf size, ray tile, block and variant by measurement before timing")
     args = ap.parse_args()
     if args.calibrate_copy:
         # A device-to-device copy of known size: the reference point that turns
@@ -245,20 +247,31 @@ def main():
     import rayengine_amd as re
     from rayengine_amd import workloads as W
 
-    t0 = time.time()
     tri_v = W.cluster_soup_default()
-    tree = re.KDTree(tri_v)
+    ctx = re.Context(0)
+    # Leaf size, ray-sort granularity (primary rays in tile x tile pixel
+    # blocks, bounces following their primaries: the order a tiled renderer
+    # issues them in), block and variant.  The defaults are within 1.3 % of
+    # the autotuner's best on this workload (leaf 4, same tile, block and
+    # variant), inside run-to-run noise; leaf 8 keeps the leaf bound of the
+    # small-scene configuration.  --autotune measures them again.
+    tune = {"max_leaf_tris": 8, "tile": 8, "block": 64, "variant": 3}
+    tuned = None
+    if args.autotune:
+        from rayengine_amd.autotune import tune_trace
+        tuned, _ = tune_trace(ctx, tri_v, lambda sc, tile: W.cluster_workload_rays(sc, tri_v, tile), 1920, 1080,
+                              device_name=torch.cuda.get_device_name(0), cache=None, log=log)
+        tune = {k: tuned[k] for k in tune}
+    t0 = time.time()
+    tree = re.KDTree(tri_v, max_leaf_tris=tune["max_leaf_tris"])
     log(f"scene: {tri_v.shape[0]} tris, tree {tree.info['n_leaves']} leaves, "
         f"{tree.info['n_trirecs']} tri records, build {time.time() - t0:.1f} s")
-    ctx = re.Context(0)
     scene = ctx.upload_scene([re.SceneObject(tree)])
     po, pd = re.camera_rays(W.CLUSTER_CAMERA, 1920, 1080)
     pp, pt = scene.trace_closest(po, pd)
-    # Primary rays in 8x8 pixel tiles (bounces follow their primaries): the
-    # order a tiled renderer issues them in, and the ray-sort granularity
-    # measured best among 4, 8 and 16.
-    order = W.tile_order(1920, 1080, 8)
+    order = W.tile_order(1920, 1080, tune["tile"])
     ro, rd = W.cluster_rays(tri_v, pp, pt, order=order)
+    tp = {"block": tune["block"], "variant": tune["variant"]}
     n = len(ro)
     n_primary = len(po)
     log(f"rays: {n} ({n_primary} primary + {n - n_primary} bounce)")
@@ -269,15 +282,17 @@ def main():
     d_p = torch.empty(n, dtype=torch.int32, device=dev)
     d_t = torch.empty(n, dtype=torch.float32, device=dev)
     torch.cuda.synchronize()
-    counts = scene.trace_counts_device(d_o.data_ptr(), d_d.data_ptr(), n, d_p.data_ptr(), d_t.data_ptr())
+    counts = scene.trace_counts_device(d_o.data_ptr(), d_d.data_ptr(), n, d_p.data_ptr(), d_t.data_ptr(), **tp)
     alg_bytes = 32 * n + 8 * counts[0] + 56 * counts[1] + 40 * counts[2]
 
     for _ in range(args.warmup):
-        scene.trace_closest_device(d_o.data_ptr(), d_d.data_ptr(), n, d_p.data_ptr(), d_t.data_ptr(), sync=False)
+        scene.trace_closest_device(d_o.data_ptr(), d_d.data_ptr(), n, d_p.data_ptr(), d_t.data_ptr(), sync=False,
+                                   **tp)
     torch.cuda.synchronize()
     t_start = time.perf_counter()
     for _ in range(args.steps):
-        scene.trace_closest_device(d_o.data_ptr(), d_d.data_ptr(), n, d_p.data_ptr(), d_t.data_ptr(), sync=False)
+        scene.trace_closest_device(d_o.data_ptr(), d_d.data_ptr(), n, d_p.data_ptr(), d_t.data_ptr(), sync=False,
+                                   **tp)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t_start
     ktimes = ctx.kernel_times(args.steps)
@@ -307,14 +322,18 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "cluster scene: 2^20-triangle Gaussian-cluster soup (64 clusters, seed 1), "
-                               "SAH rope kd-tree leaf<=8 depth<=24, 1920x1080 primary rays (8x8 pixel tiles) + "
-                               "1 diffuse bounce per hit, closest hit, leaf-visit cap 50",
+        "config": {"workload": f"cluster scene: 2^20-triangle Gaussian-cluster soup (64 clusters, seed 1), "
+                               f"SAH rope kd-tree leaf<={tune['max_leaf_tris']} depth<=24, 1920x1080 primary rays "
+                               f"({tune['tile']}x{tune['tile']} pixel tiles) + 1 diffuse bounce per hit, closest hit, "
+                               f"leaf-visit cap 50",
+                   "launch": {"block": tune["block"], "variant": tune["variant"], "autotuned": tuned is not None},
                    "rays": n, "primary_rays": n_primary, "bounce_rays": n - n_primary,
                    "triangles": int(tri_v.shape[0]), "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "trace_closest_o6_kernel", "kernel_ms": round(kms, 4),
+                     "kernel": "trace_closest_o6_kernel" if tune["variant"] == 3 else
+                               ("trace_persistent_kernel" if tune["variant"] == 1 else "trace_closest_kernel"),
+                     "kernel_ms": round(kms, 4),
                      "algorithmic_bytes": int(alg_bytes),
                      "work": {"node_entries": counts[0], "leaves": counts[1], "tri_tests": counts[2],
                               "rays_hit": counts[3]}},

@@ -62,6 +62,15 @@ def cluster_rays(tri_v, primary_prim, primary_t, width=1920, height=1080, order=
     return np.concatenate([o, bo]), np.concatenate([d, bd])
 
 
+def cluster_workload_rays(scene, tri_v, tile: int = 8, width: int = 1920, height: int = 1080):
+    """The cluster workload's rays for a scene holding the cluster soup:
+    primaries in tile x tile pixel blocks (1 = scanline order), then one
+    bounce ray per primary hit in the primaries' order."""
+    o, d = camera_rays(CLUSTER_CAMERA, width, height)
+    p, t = scene.trace_closest(o, d)
+    return cluster_rays(tri_v, p, t, width, height, order=tile_order(width, height, tile) if tile > 1 else None)
+
+
 # Shadow rays: from each primary hit, offset 1e-4 along the geometric normal
 # facing the incoming ray, towards a point light.
 CLUSTER_LIGHT = (20.0, 30.0, 25.0)

@@ -1,0 +1,40 @@
+"""The single-launch autotuner measures every candidate on the device and
+returns the fastest; the configuration it returns traces bit-exactly."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+import rayengine_amd as re
+from rayengine_amd import autotune as AT
+from rayengine_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def test_tune_trace_small_scene(gpu_ctx):
+    tri_v = W.small_soup()
+
+    def make_rays(scene, tile):
+        o, d = re.camera_rays(W.SMALL_CAMERA, 128, 128)
+        p, t = scene.trace_closest(o, d)
+        bo, bd = re.bounce_rays(tri_v, o, d, p, t, 5)
+        order = W.tile_order(128, 128, tile)
+        return np.concatenate([o[order], bo]), np.concatenate([d[order], bd])
+
+    logs = []
+    cfg, meas = AT.tune_trace(gpu_ctx, tri_v, make_rays, 128, 128, leaf_sizes=(4, 8), tiles=(1, 8),
+                              blocks=(64, 256), variants=(2, 3), reps=2, cache=None, log=logs.append)
+    print(cfg, flush=True)
+    sys.stdout.flush()
+    assert len(meas) == 16 and len(logs) == 16
+    assert cfg["ms"] == min(m["ms"] for m in meas) and cfg["ms"] > 0
+    assert cfg["max_leaf_tris"] in (4, 8) and cfg["tile"] in (1, 8) and cfg["block"] in (64, 256)
+    tree = re.KDTree(tri_v, max_leaf_tris=cfg["max_leaf_tris"])
+    scene = gpu_ctx.upload_scene([re.SceneObject(tree)])
+    o, d = make_rays(scene, cfg["tile"])
+    p, t = scene.trace_closest(o, d, block=cfg["block"], variant=cfg["variant"])
+    po, to = oracle.trace(*tree.export(), tree.identity_object(), o, d)
+    assert np.array_equal(p, po) and np.array_equal(t.view(np.uint32), to.view(np.uint32))
+    scene.release()
