@@ -41,6 +41,16 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(tree, ro, rd, n_primary, threads, budget_cpu_s=20.0):
     """Oracle traversal (the reference's walk, no shortcut) timed on the host
     cores: a stratified sample (primary and bounce rays in the workload's
@@ -97,6 +107,11 @@ TERRAIN_METHODS = {
     "surrogate_hybrid": ("hybrid", dict(tau_voxels=4.0, margin_voxels=1.0, k_exact=0.1, n_bisect=4)),
 }
 MLP_FLOP_PER_EVAL = 2 * (99 * 64 + 3 * 64 * 64 + 64)
+# forward + backward (input and weight gradients) per training sample, about
+# three forward passes
+MLP_TRAIN_FLOP_PER_SAMPLE = 3 * MLP_FLOP_PER_EVAL
+NOISE_FLOP_PER_VOXEL = 5 * 70
+FP32_PEAK_TFLOPS = 157.3
 # Surrogate training: 2^22 samples (half within 2 voxels of the surface), Adam
 # lr 1e-3, batch 65,536, 20k steps.
 TRAIN_STEPS = 20000
@@ -117,13 +132,23 @@ def terrain_section(re, W, torch, ctx):
         v2.release()
     out["noise_ms"] = round(float(np.mean(noise)), 4)
     out["noise_gvox_s"] = round(256 ** 3 / (np.mean(noise) * 1e-3) / 1e9, 2)
+    # VALU-bound: about 5 octaves x 70 FP32 operations per voxel, and a 4-byte write
+    nsec = float(np.mean(noise)) * 1e-3
+    out["noise_roofline"] = {"bound": "valu", "flop_per_voxel": NOISE_FLOP_PER_VOXEL,
+                             "achieved_tflops": round(256 ** 3 * NOISE_FLOP_PER_VOXEL / nsec / 1e12, 2),
+                             "peak_tflops": FP32_PEAK_TFLOPS,
+                             "frac": round(256 ** 3 * NOISE_FLOP_PER_VOXEL / nsec / 1e12 / FP32_PEAK_TFLOPS, 4),
+                             "write_gbs": round(256 ** 3 * 4 / nsec / 1e9, 1)}
     dv = re.distance_volume(vol)
     net = re.Surrogate(ctx, seed=1)
     t0 = time.perf_counter()
     hist = net.train(dv, n_samples=1 << 22, batch=65536, steps=TRAIN_STEPS, lr=1e-3, seed=1, over_weight=4.0)
-    out["train"] = {"steps": TRAIN_STEPS, "batch": 65536,
-                    "ms_per_step": round((time.perf_counter() - t0) * 1e3 / TRAIN_STEPS, 4),
-                    "loss_first": float(hist[:10].mean()), "loss_last": float(hist[-100:].mean())}
+    step_ms = (time.perf_counter() - t0) * 1e3 / TRAIN_STEPS
+    train_tf = MLP_TRAIN_FLOP_PER_SAMPLE * 65536 / (step_ms * 1e-3) / 1e12
+    out["train"] = {"steps": TRAIN_STEPS, "batch": 65536, "ms_per_step": round(step_ms, 4),
+                    "loss_first": float(hist[:10].mean()), "loss_last": float(hist[-100:].mean()),
+                    "flop_per_sample": MLP_TRAIN_FLOP_PER_SAMPLE, "tflops": round(train_tf, 2),
+                    "frac_of_bf16_peak": round(train_tf / BF16_PEAK_TFLOPS, 4)}
     # network throughput (re_mlp_forward on 4M device points)
     npts = 1 << 22
     pts = torch.rand(npts, 3, device=dev)
@@ -168,6 +193,7 @@ def terrain_section(re, W, torch, ctx):
         ms = float(np.mean(ctx.kernel_times(5)))
         methods[name] = {"mrays_s": round(n / (ms * 1e-3) / 1e6, 1), "kernel_ms": round(ms, 4),
                          "err_p50_vox": round(float(np.median(e)), 4), "err_p99_vox": round(float(np.percentile(e, 99)), 4),
+                         "err_max_vox": round(float(e.max()), 3) if e.size else None,
                          "frac_err_gt_half_vox": round(float(np.mean(e > 0.5)), 5),
                          "hit_agree": round(float(np.mean(h == ref_h)), 5), "params": params}
     out["methods"] = methods
@@ -284,6 +310,9 @@ def main():
     torch.cuda.synchronize()
     counts = scene.trace_counts_device(d_o.data_ptr(), d_d.data_ptr(), n, d_p.data_ptr(), d_t.data_ptr(), **tp)
     alg_bytes = 32 * n + 8 * counts[0] + 56 * counts[1] + 40 * counts[2]
+    # the same walks in the reference's layout: a 128-byte node per node visit
+    # (inner or leaf), 52 bytes per triangle test (index + 3 float4 fetches)
+    ref_layout_bytes = 32 * n + 128 * (counts[0] + counts[1]) + 52 * counts[2]
 
     for _ in range(args.warmup):
         scene.trace_closest_device(d_o.data_ptr(), d_d.data_ptr(), n, d_p.data_ptr(), d_t.data_ptr(), sync=False,
@@ -335,6 +364,8 @@ def main():
                                ("trace_persistent_kernel" if tune["variant"] == 1 else "trace_closest_kernel"),
                      "kernel_ms": round(kms, 4),
                      "algorithmic_bytes": int(alg_bytes),
+                     "reference_layout_bytes": int(ref_layout_bytes),
+                     "reference_layout_gbs": round(ref_layout_bytes / (kms * 1e-3) / 1e9, 1),
                      "work": {"node_entries": counts[0], "leaves": counts[1], "tri_tests": counts[2],
                               "rays_hit": counts[3]}},
     }
@@ -347,6 +378,9 @@ def main():
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         res["cpu_baseline"] = cpu_baseline(tree, ro, rd, n_primary, threads)
+        one = cpu_baseline(tree, ro, rd, n_primary, 1, budget_cpu_s=8.0)
+        res["cpu_baseline"]["one_core"] = {"value": one["value"], "sample": one["sample"]}
+        res["cpu_baseline"]["cpu_model"] = _cpu_model()
     print(json.dumps(res), flush=True)
 
 
