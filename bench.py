@@ -139,6 +139,19 @@ def terrain_section(re, W, torch, ctx):
                              "peak_tflops": FP32_PEAK_TFLOPS,
                              "frac": round(256 ** 3 * NOISE_FLOP_PER_VOXEL / nsec / 1e12 / FP32_PEAK_TFLOPS, 4),
                              "write_gbs": round(256 ** 3 * 4 / nsec / 1e9, 1)}
+    # brush edit (SPHERE_ADD) over the whole 256^3 volume and its material
+    # channel: 8 bytes read and 8 written per voxel
+    ev = ctx.noise_volume()
+    ev.edit("sphere_add", center=(0.5, 0.45, 0.5), extent=(0.1, 0, 0), material=1.0)
+    ems = []
+    for k in range(5):
+        ev.edit("sphere_add", center=(0.3 + 0.1 * k, 0.45, 0.5), extent=(0.1, 0, 0), material=1.0)
+        ems.append(ctx.last_kernel_ms())
+    ev.release()
+    em = float(np.mean(ems))
+    out["edit"] = {"kernel": "volume_edit_kernel", "kernel_ms": round(em, 4), "bound": "hbm",
+                   "achieved_gbs": round(256 ** 3 * 16 / (em * 1e-3) / 1e9, 1), "peak_gbs": HBM_PEAK_GBS,
+                   "frac": round(256 ** 3 * 16 / (em * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
     dv = re.distance_volume(vol)
     net = re.Surrogate(ctx, seed=1)
     t0 = time.perf_counter()

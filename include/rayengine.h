@@ -260,6 +260,50 @@ RE_API re_status re_volume_info(re_ctx* ctx, int32_t vol_id, int32_t* dim, float
                                 int64_t* bytes);
 RE_API re_status re_volume_release(re_ctx* ctx, int32_t vol_id);
 
+/* ---- .rsdf files and SDF edits ---------------------------------------- */
+
+/* Writes a volume as a .rsdf file: int32 dim[3], float spacing[3], then
+ * dim.x*dim.y*dim.z pairs (distance, material) of float32, x fastest
+ * (save_map, RayEngine/src/cuda-src/renderer.cu:838-871; the reference
+ * indexes its array x + dim.y*(y + dim.x*z), which equals x + dim.x*(y +
+ * dim.y*z) for the cubic grids it writes).  Material is 0 for volumes that
+ * never had one (the noise volumes). */
+RE_API re_status re_volume_save_rsdf(re_ctx* ctx, int32_t vol_id, const char* path);
+/* Reads a .rsdf file into a new volume with its material channel (Grid::
+ * load_volume_floam_file, RayEngine/src/World/Grid.cpp:23-64, and load_map,
+ * renderer.cu:873-947).  As in load_map the volume box is [0, (dim - 1) *
+ * spacing], the point of the last voxel.  RE_ERR_IO if the file is missing,
+ * short or longer than its header says. */
+RE_API re_status re_volume_load_rsdf(re_ctx* ctx, const char* path, int32_t* vol_id);
+/* Reads the material channel back as a linear x-fastest grid (zeros if the
+ * volume has none). */
+RE_API re_status re_volume_read_material(re_ctx* ctx, int32_t vol_id, float* linear);
+
+typedef enum re_brush_type {
+  RE_BRUSH_SPHERE_ADD = 0, /* TerrainBrushType, RayEngine/src/RayEngine/RayEngine.h:77-83 */
+  RE_BRUSH_SPHERE_SUBTRACT = 1,
+  RE_BRUSH_CUBE_ADD = 2,
+  RE_BRUSH_CUBE_SUBTRACT = 3
+} re_brush_type;
+
+typedef struct re_brush {
+  int32_t type;         /* re_brush_type */
+  float center[3];      /* brush position in volume coordinates (the brush ray's hit point) */
+  float extent[3];      /* sphere radius = extent[0]; cube half extents (the reference's int3, as floats) */
+  float material;       /* material written where the brush wins (material_index) */
+  int32_t snap_to_grid; /* snap x and z of the voxel-to-brush offset to multiples of extent[0] */
+  float smooth_k;       /* smooth-minimum width of SPHERE_ADD; the reference's 0.1 */
+} re_brush;
+
+RE_API void re_brush_default(re_brush* b);
+/* Applies a brush to every voxel of a volume, distance and material
+ * (insert_sphere_to_texture, renderer.cu:30-92, with sdf_smin / sdf_fmaxf /
+ * fminf, sdf_functions.cuh:270-289, and sphere_distance / aabb_distance,
+ * :422-437).  The reference traces the brush ray in every thread to find the
+ * hit point; here the caller passes that point (re_sdf_trace gives it).
+ * Adds a material channel (zeros) to a volume that has none. */
+RE_API re_status re_volume_edit(re_ctx* ctx, int32_t vol_id, const re_brush* brush);
+
 typedef struct re_sdf_params {
   int32_t max_steps; /* default 256 */
   float eps;         /* hit when d <= eps * t; default 1e-3 (renderer.cu:210-211) */

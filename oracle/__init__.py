@@ -35,6 +35,7 @@ _SIGS = {
     "orc_noise_fill": (None, [_P, _P, _P, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, _P, C.c_int]),
     "orc_noise_fill_f64": (None, [_P, _P, _P, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, _P, C.c_int]),
     "orc_trilinear": (C.c_float, [_P, _P, _P, C.c_float, C.c_float, C.c_float]),
+    "orc_volume_edit": (None, [_P, _P, _P, _P, C.c_int, _P, _P, C.c_float, C.c_int, C.c_float]),
     "orc_sdf_trace": (None, [_P, _P, _P, _P, _P, C.c_int64, C.c_int, C.c_float, C.c_float, C.c_int, _P, _P, _P, _P,
                              C.c_int]),
     "orc_sdf_trace_f64": (None, [_P, _P, _P, _P, _P, C.c_int64, C.c_int, C.c_float, C.c_float, C.c_int, _P, _P,
@@ -299,3 +300,20 @@ def truth_trace(grid64, spacing, o, d, h_voxels=0.125, threads=0):
     hit = np.empty(n, np.uint8)
     f(_p(g), _p(dim), _p(sp), _p(o), _p(d), n, h_voxels, _p(t), _p(hit), threads)
     return t, hit
+
+
+BRUSH_TYPES = {"sphere_add": 0, "sphere_subtract": 1, "cube_add": 2, "cube_subtract": 3}
+
+
+def volume_edit(dist, mat, spacing, kind, center, extent, material=0.0, snap=False, smooth_k=0.1):
+    """insert_sphere_to_texture on (dz, dy, dx) float32 distance and material
+    grids; returns edited copies."""
+    dist = np.ascontiguousarray(dist, np.float32).copy()
+    mat = np.ascontiguousarray(mat, np.float32).copy()
+    dim = np.array(dist.shape[::-1], np.int32)
+    sp = np.asarray(spacing, np.float32)
+    c = np.asarray(center, np.float32)
+    e = np.asarray(extent, np.float32)
+    lib().orc_volume_edit(_p(dist), _p(mat), _p(dim), _p(sp), BRUSH_TYPES[kind], _p(c), _p(e), float(material),
+                          int(bool(snap)), float(smooth_k))
+    return dist, mat

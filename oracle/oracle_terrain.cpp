@@ -305,3 +305,68 @@ ORC_API void orc_truth_trace(const double* grid, const int32_t* dim, const float
     }
   }
 }
+
+// ---------------------------------------------------------------- SDF edit
+
+namespace {
+
+// A (distance, material) pair as the reference's float2.
+struct Pair {
+  float d, m;
+};
+
+// sdf_fminf / sdf_fmaxf select a whole pair by distance; fminf on float2 is
+// componentwise (RayEngine/src/cuda-src/sdf_functions.cuh:270-280).
+Pair pick_smaller(Pair a, Pair b) { return a.d < b.d ? a : b; }
+Pair pick_larger(Pair a, Pair b) { return a.d > b.d ? a : b; }
+
+// sdf_smin (sdf_functions.cuh:283-289): the smaller pair, its distance
+// lowered by h^2 k / 4 with h = max(k - |a - b|, 0) / k.
+Pair smooth_min(Pair a, Pair b, float k) {
+  const float h = std::fmax(k - std::fabs(a.d - b.d), 0.0f) / k;
+  const Pair s = pick_smaller(a, b);
+  return Pair{s.d - h * h * k * 0.25f, s.m};
+}
+
+// sphere_distance and aabb_distance (sdf_functions.cuh:422-437).
+float norm3(float x, float y, float z) { return std::sqrt(x * x + y * y + z * z); }
+float sphere_dist(float x, float y, float z, float r) { return norm3(x, y, z) - r; }
+float aabb_dist(float x, float y, float z, const float* s) {
+  const float dx = std::fabs(x) - s[0], dy = std::fabs(y) - s[1], dz = std::fabs(z) - s[2];
+  return norm3(std::fmax(dx, 0.0f), std::fmax(dy, 0.0f), std::fmax(dz, 0.0f)) +
+         std::fmin(std::fmax(dx, std::fmax(dy, dz)), 0.0f);
+}
+
+}  // namespace
+
+// insert_sphere_to_texture (RayEngine/src/cuda-src/renderer.cu:30-92) once the
+// brush ray has found its hit point `center`: for every voxel, p = voxel
+// point - center (x and z snapped to multiples of extent[0] when snap), then
+// SPHERE_ADD smooth-min, SPHERE_SUBTRACT / CUBE_SUBTRACT pair-max against the
+// negated brush distance, CUBE_ADD componentwise min.  dist and mat are
+// linear x-fastest grids, edited in place.
+ORC_API void orc_volume_edit(float* dist, float* mat, const int32_t* dim, const float* spacing, int type,
+                             const float* center, const float* extent, float material, int snap, float k) {
+  for (int z = 0; z < dim[2]; ++z)
+    for (int y = 0; y < dim[1]; ++y)
+      for (int x = 0; x < dim[0]; ++x) {
+        const int64_t i = ((int64_t)z * dim[1] + y) * dim[0] + x;
+        float px = (float)x * spacing[0] - center[0];
+        const float py = (float)y * spacing[1] - center[1];
+        float pz = (float)z * spacing[2] - center[2];
+        if (snap) {
+          px = std::floor(px - std::fabs(std::fmod(px, extent[0])));
+          pz = std::floor(pz - std::fabs(std::fmod(pz, extent[0])));
+        }
+        const Pair a{dist[i], mat[i]};
+        Pair r;
+        switch (type) {
+          case 0: r = smooth_min(a, Pair{sphere_dist(px, py, pz, extent[0]), material}, k); break;
+          case 1: r = pick_larger(a, Pair{-sphere_dist(px, py, pz, extent[0]), material}); break;
+          case 2: r = Pair{std::fmin(a.d, aabb_dist(px, py, pz, extent)), std::fmin(a.m, material)}; break;
+          default: r = pick_larger(a, Pair{-aabb_dist(px, py, pz, extent), material}); break;
+        }
+        dist[i] = r.d;
+        mat[i] = r.m;
+      }
+}

@@ -210,6 +210,12 @@ class Context:
         check(lib().re_noise_generate(self._h, C.byref(p), C.byref(vid)), self._h)
         return Volume(self, vid.value)
 
+    def load_rsdf(self, path) -> "Volume":
+        """Reads a .rsdf file into a new volume (re_volume_load_rsdf)."""
+        vid = C.c_int32()
+        check(lib().re_volume_load_rsdf(self._h, str(path).encode(), C.byref(vid)), self._h)
+        return Volume(self, vid.value)
+
     def upload_volume(self, grid: np.ndarray, spacing) -> "Volume":
         """Uploads a (dz, dy, dx) float32 grid (x fastest) with the given spacing."""
         g = np.ascontiguousarray(grid, np.float32)
@@ -273,6 +279,9 @@ def sdf_params(**kw) -> _lib.re_sdf_params:
     return p
 
 
+BRUSH_TYPES = {"sphere_add": 0, "sphere_subtract": 1, "cube_add": 2, "cube_subtract": 3}
+
+
 class Volume:
     """A device-resident terrain volume (bricked FP32)."""
 
@@ -289,6 +298,28 @@ class Volume:
         out = np.empty((self.dim[2], self.dim[1], self.dim[0]), np.float32)
         check(lib().re_volume_read(self.ctx.handle, self.id, ptr(out)), self.ctx.handle)
         return out
+
+    def read_material(self) -> np.ndarray:
+        """The material channel as a (dz, dy, dx) float32 array (zeros if none)."""
+        out = np.empty((self.dim[2], self.dim[1], self.dim[0]), np.float32)
+        check(lib().re_volume_read_material(self.ctx.handle, self.id, ptr(out)), self.ctx.handle)
+        return out
+
+    def save_rsdf(self, path):
+        """Writes the volume as a .rsdf file (re_volume_save_rsdf)."""
+        check(lib().re_volume_save_rsdf(self.ctx.handle, self.id, str(path).encode()), self.ctx.handle)
+
+    def edit(self, kind="sphere_add", center=(0.0, 0.0, 0.0), extent=(1.0, 1.0, 1.0), material=0.0, snap=False,
+             smooth_k=0.1):
+        """Applies one brush (re_volume_edit).  kind: sphere_add,
+        sphere_subtract, cube_add, cube_subtract."""
+        b = _lib.re_brush()
+        lib().re_brush_default(C.byref(b))
+        b.type = BRUSH_TYPES[kind] if isinstance(kind, str) else int(kind)
+        b.center[:] = [float(x) for x in center]
+        b.extent[:] = [float(x) for x in extent]
+        b.material, b.snap_to_grid, b.smooth_k = float(material), int(bool(snap)), float(smooth_k)
+        check(lib().re_volume_edit(self.ctx.handle, self.id, C.byref(b)), self.ctx.handle)
 
     def trace(self, o, d, normals=False, **params):
         """Sphere trace: (t, hit, steps[, normal])."""

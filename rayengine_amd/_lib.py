@@ -85,6 +85,11 @@ class re_hybrid_params(C.Structure):
                 ("max_iters", C.c_int32), ("block", C.c_int32)]
 
 
+class re_brush(C.Structure):
+    _fields_ = [("type", C.c_int32), ("center", C.c_float * 3), ("extent", C.c_float * 3), ("material", C.c_float),
+                ("snap_to_grid", C.c_int32), ("smooth_k", C.c_float)]
+
+
 class re_batch_scene(C.Structure):
     _fields_ = [("tree", C.c_void_p), ("ray_begin", C.c_int64), ("ray_count", C.c_int64)]
 
@@ -140,6 +145,11 @@ SIGNATURES = {
     "re_volume_read": (C.c_int, [_P, C.c_int32, _P]),
     "re_volume_info": (C.c_int, [_P, C.c_int32, _P, _P, _I64]),
     "re_volume_release": (C.c_int, [_P, C.c_int32]),
+    "re_volume_save_rsdf": (C.c_int, [_P, C.c_int32, C.c_char_p]),
+    "re_volume_load_rsdf": (C.c_int, [_P, C.c_char_p, _I32]),
+    "re_volume_read_material": (C.c_int, [_P, C.c_int32, _P]),
+    "re_brush_default": (None, [_P]),
+    "re_volume_edit": (C.c_int, [_P, C.c_int32, _P]),
     "re_sdf_default_params": (None, [C.POINTER(re_sdf_params)]),
     "re_sdf_trace": (C.c_int, [_P, C.c_int32, C.POINTER(re_sdf_params), _P, _P, C.c_int64, _P, _P, _P, _P,
                                C.c_uint32]),

@@ -58,7 +58,10 @@ RE_API void re_destroy(re_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   for (auto& kv : ctx->scenes) free_scene(kv.second);
-  for (auto& kv : ctx->volumes) (void)hipFree(kv.second.data);
+  for (auto& kv : ctx->volumes) {
+    (void)hipFree(kv.second.data);
+    if (kv.second.mat) (void)hipFree(kv.second.mat);
+  }
   for (auto& kv : ctx->batches) {
     DevBatch& b = kv.second;
     if (b.exec) (void)hipGraphExecDestroy(b.exec);

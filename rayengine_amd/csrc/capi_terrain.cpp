@@ -1,8 +1,11 @@
 // C-ABI entry points of the terrain path: noise-volume generation, volume
-// upload/readback, and sphere tracing.
+// upload/readback, .rsdf files, brush edits, and sphere tracing.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstring>
 #include <vector>
 
 #include "capi_util.hpp"
@@ -30,6 +33,44 @@ re_status alloc_volume(re_ctx* ctx, const int32_t* dim, const float* spacing, De
   }
   v.bytes = (int64_t)v.bdim[0] * v.bdim[1] * v.bdim[2] * kBrickVox * (int64_t)sizeof(float);
   HIP_TRY(ctx, hipMalloc((void**)&v.data, (size_t)v.bytes), "volume hipMalloc");
+  return RE_OK;
+}
+
+// Linear x-fastest <-> bricked (Morton inside 8^3 bricks) copies.
+void to_bricked(const DevVolume& v, const float* linear, std::vector<float>& bricked) {
+  bricked.assign((size_t)(v.bytes / 4), 0.0f);
+  for (int z = 0; z < v.dim[2]; ++z)
+    for (int y = 0; y < v.dim[1]; ++y)
+      for (int x = 0; x < v.dim[0]; ++x)
+        bricked[vox_index(v.bdim, x, y, z)] = linear[((int64_t)z * v.dim[1] + y) * v.dim[0] + x];
+}
+
+void to_linear(const DevVolume& v, const std::vector<float>& bricked, float* linear) {
+  for (int z = 0; z < v.dim[2]; ++z)
+    for (int y = 0; y < v.dim[1]; ++y)
+      for (int x = 0; x < v.dim[0]; ++x)
+        linear[((int64_t)z * v.dim[1] + y) * v.dim[0] + x] = bricked[vox_index(v.bdim, x, y, z)];
+}
+
+re_status read_channel(re_ctx* ctx, const DevVolume& v, const float* src, float* linear) {
+  const int64_t n = (int64_t)v.dim[0] * v.dim[1] * v.dim[2];
+  if (!src) {
+    std::fill(linear, linear + n, 0.0f);
+    return RE_OK;
+  }
+  std::vector<float> bricked((size_t)(v.bytes / 4));
+  HIP_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+  HIP_TRY(ctx, hipMemcpyAsync(bricked.data(), src, (size_t)v.bytes, hipMemcpyDeviceToHost, ctx->stream), "volume read");
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream), "volume read sync");
+  to_linear(v, bricked, linear);
+  return RE_OK;
+}
+
+// Gives a volume a zero material channel if it has none.
+re_status ensure_material(re_ctx* ctx, DevVolume& v) {
+  if (v.mat) return RE_OK;
+  HIP_TRY(ctx, hipMalloc((void**)&v.mat, (size_t)v.bytes), "material hipMalloc");
+  HIP_TRY(ctx, hipMemsetAsync(v.mat, 0, (size_t)v.bytes, ctx->stream), "material clear");
   return RE_OK;
 }
 
@@ -152,7 +193,137 @@ RE_API re_status re_volume_release(re_ctx* ctx, int32_t vol_id) {
   if (it == ctx->volumes.end()) return set_err(ctx, RE_ERR_NOT_FOUND, "re_volume_release: unknown volume");
   (void)hipStreamSynchronize(ctx->stream);
   (void)hipFree(it->second.data);
+  if (it->second.mat) (void)hipFree(it->second.mat);
   ctx->volumes.erase(it);
+  return RE_OK;
+}
+
+RE_API re_status re_volume_save_rsdf(re_ctx* ctx, int32_t vol_id, const char* path) {
+  if (!ctx || !path) return fail(RE_ERR_INVALID, "re_volume_save_rsdf: null argument");
+  auto it = ctx->volumes.find(vol_id);
+  if (it == ctx->volumes.end()) return set_err(ctx, RE_ERR_NOT_FOUND, "re_volume_save_rsdf: unknown volume");
+  const DevVolume& v = it->second;
+  const int64_t n = (int64_t)v.dim[0] * v.dim[1] * v.dim[2];
+  std::vector<float> dist((size_t)n), mat((size_t)n);
+  re_status st = read_channel(ctx, v, v.data, dist.data());
+  if (!st) st = read_channel(ctx, v, v.mat, mat.data());
+  if (st) return st;
+  std::vector<float> pairs((size_t)(2 * n));
+  for (int64_t i = 0; i < n; ++i) {
+    pairs[(size_t)(2 * i)] = dist[(size_t)i];
+    pairs[(size_t)(2 * i + 1)] = mat[(size_t)i];
+  }
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return set_err(ctx, RE_ERR_IO, std::string("re_volume_save_rsdf: cannot open ") + path);
+  bool ok = std::fwrite(v.dim, sizeof(int32_t), 3, f) == 3 && std::fwrite(v.spacing, sizeof(float), 3, f) == 3 &&
+            std::fwrite(pairs.data(), sizeof(float), pairs.size(), f) == pairs.size();
+  ok = (std::fclose(f) == 0) && ok;
+  if (!ok) return set_err(ctx, RE_ERR_IO, std::string("re_volume_save_rsdf: write failed: ") + path);
+  return RE_OK;
+}
+
+RE_API re_status re_volume_load_rsdf(re_ctx* ctx, const char* path, int32_t* vol_id) {
+  if (!ctx || !path || !vol_id) return fail(RE_ERR_INVALID, "re_volume_load_rsdf: null argument");
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return set_err(ctx, RE_ERR_IO, std::string("re_volume_load_rsdf: cannot open ") + path);
+  int32_t dim[3];
+  float spacing[3];
+  bool ok = std::fread(dim, sizeof(int32_t), 3, f) == 3 && std::fread(spacing, sizeof(float), 3, f) == 3;
+  if (!ok) {
+    std::fclose(f);
+    return set_err(ctx, RE_ERR_IO, "re_volume_load_rsdf: short header");
+  }
+  if (!valid_dims(dim, spacing)) {
+    std::fclose(f);
+    return set_err(ctx, RE_ERR_IO, "re_volume_load_rsdf: header dims outside [2,4096] or spacing not > 0");
+  }
+  const int64_t n = (int64_t)dim[0] * dim[1] * dim[2];
+  std::vector<float> pairs((size_t)(2 * n));
+  ok = std::fread(pairs.data(), sizeof(float), pairs.size(), f) == pairs.size();
+  char extra;
+  const bool longer = ok && std::fread(&extra, 1, 1, f) == 1;
+  std::fclose(f);
+  if (!ok) return set_err(ctx, RE_ERR_IO, "re_volume_load_rsdf: file shorter than its header says");
+  if (longer) return set_err(ctx, RE_ERR_IO, "re_volume_load_rsdf: file longer than its header says");
+  std::vector<float> dist((size_t)n), mat((size_t)n);
+  for (int64_t i = 0; i < n; ++i) {
+    dist[(size_t)i] = pairs[(size_t)(2 * i)];
+    mat[(size_t)i] = pairs[(size_t)(2 * i + 1)];
+  }
+  HIP_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+  DevVolume v;
+  re_status st = alloc_volume(ctx, dim, spacing, v);
+  if (st) return st;
+  // load_map bounds the volume by the point of its last voxel (Grid.cpp:61).
+  for (int a = 0; a < 3; ++a) v.box_max[a] = (float)(dim[a] - 1) * spacing[a];
+  std::vector<float> bd, bm;
+  to_bricked(v, dist.data(), bd);
+  to_bricked(v, mat.data(), bm);
+  hipError_t e = hipMalloc((void**)&v.mat, (size_t)v.bytes);
+  if (e == hipSuccess) e = hipMemcpyAsync(v.data, bd.data(), (size_t)v.bytes, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(v.mat, bm.data(), (size_t)v.bytes, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) {
+    (void)hipFree(v.data);
+    if (v.mat) (void)hipFree(v.mat);
+    return hip_err(ctx, e, "re_volume_load_rsdf");
+  }
+  int32_t id = ctx->next_id++;
+  ctx->volumes[id] = v;
+  *vol_id = id;
+  return RE_OK;
+}
+
+RE_API re_status re_volume_read_material(re_ctx* ctx, int32_t vol_id, float* linear) {
+  if (!ctx || !linear) return fail(RE_ERR_INVALID, "re_volume_read_material: null argument");
+  auto it = ctx->volumes.find(vol_id);
+  if (it == ctx->volumes.end()) return set_err(ctx, RE_ERR_NOT_FOUND, "re_volume_read_material: unknown volume");
+  return read_channel(ctx, it->second, it->second.mat, linear);
+}
+
+RE_API void re_brush_default(re_brush* b) {
+  if (!b) return;
+  std::memset(b, 0, sizeof(*b));
+  b->type = RE_BRUSH_SPHERE_ADD;
+  b->extent[0] = b->extent[1] = b->extent[2] = 1.0f;
+  b->smooth_k = 0.1f;
+}
+
+RE_API re_status re_volume_edit(re_ctx* ctx, int32_t vol_id, const re_brush* brush) {
+  if (!ctx || !brush) return fail(RE_ERR_INVALID, "re_volume_edit: null argument");
+  auto it = ctx->volumes.find(vol_id);
+  if (it == ctx->volumes.end()) return set_err(ctx, RE_ERR_NOT_FOUND, "re_volume_edit: unknown volume");
+  const re_brush& b = *brush;
+  bool ok = b.type >= RE_BRUSH_SPHERE_ADD && b.type <= RE_BRUSH_CUBE_SUBTRACT && std::isfinite(b.material);
+  for (int a = 0; a < 3; ++a) ok = ok && std::isfinite(b.center[a]) && std::isfinite(b.extent[a]);
+  if (b.type == RE_BRUSH_SPHERE_ADD) ok = ok && b.smooth_k > 0.0f && std::isfinite(b.smooth_k);
+  if (b.snap_to_grid) ok = ok && b.extent[0] != 0.0f;
+  if (!ok)
+    return set_err(ctx, RE_ERR_INVALID,
+                   "re_volume_edit: brush type in [0,3], finite center/extent/material, smooth_k > 0 for "
+                   "SPHERE_ADD, extent[0] != 0 when snapping");
+  HIP_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+  DevVolume& v = it->second;
+  re_status st = ensure_material(ctx, v);
+  if (st) return st;
+  EditLaunch L;
+  L.data = v.data;
+  L.mat = v.mat;
+  for (int a = 0; a < 3; ++a) {
+    L.dim[a] = v.dim[a];
+    L.bdim[a] = v.bdim[a];
+    L.spacing[a] = v.spacing[a];
+    L.center[a] = b.center[a];
+    L.extent[a] = b.extent[a];
+  }
+  L.type = b.type;
+  L.material = b.material;
+  L.snap = b.snap_to_grid ? 1 : 0;
+  L.k = b.smooth_k;
+  HIP_TRY(ctx, ctx->launch_begin(), "event");
+  HIP_TRY(ctx, launch_volume_edit(L, ctx->stream), "volume edit launch");
+  HIP_TRY(ctx, ctx->launch_end(), "event");
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream), "volume edit sync");
   return RE_OK;
 }
 

@@ -17,6 +17,8 @@ inline re_status set_err(re_ctx* ctx, re_status st, const char* msg) {
   return st;
 }
 
+inline re_status set_err(re_ctx* ctx, re_status st, const std::string& msg) { return set_err(ctx, st, msg.c_str()); }
+
 inline re_status hip_err(re_ctx* ctx, hipError_t e, const char* where) {
   char buf[400];
   snprintf(buf, sizeof(buf), "%s: %s", where, hipGetErrorString(e));

@@ -69,6 +69,7 @@ hipError_t launch_trace_occluded(const TraceLaunch& L, hipStream_t s);
 // A volume resident in HBM: 8x8x8 Morton-ordered bricks.
 struct DevVolume {
   float* data = nullptr;
+  float* mat = nullptr;  // material channel, same layout as data; null until loaded or edited
   int32_t dim[3] = {0, 0, 0};
   int32_t bdim[3] = {0, 0, 0};
   float spacing[3] = {0, 0, 0};
@@ -113,6 +114,20 @@ struct SdfLaunch {
 };
 
 hipError_t launch_noise_fill(const NoiseLaunch& L, hipStream_t s);
+
+// One brush edit over a whole volume (distance and material channels).
+struct EditLaunch {
+  float* data;
+  float* mat;
+  int32_t dim[3], bdim[3];
+  float spacing[3];
+  int32_t type;
+  float center[3], extent[3];
+  float material;
+  int32_t snap;
+  float k;
+};
+hipError_t launch_volume_edit(const EditLaunch& L, hipStream_t s);
 hipError_t launch_sdf_trace(const SdfLaunch& L, hipStream_t s);
 
 // Doubled Perlin permutation table of RPerlinNoise(seed).

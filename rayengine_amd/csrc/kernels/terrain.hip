@@ -81,7 +81,82 @@ __global__ void __launch_bounds__(256) sdf_trace_kernel(VolView V, float bx0, fl
   }
 }
 
+// Brush distances (sphere_distance / aabb_distance,
+// RayEngine/src/cuda-src/sdf_functions.cuh:422-437): |p| - r, and the box
+// distance |max(|p| - s, 0)| + min(max(d.x, d.y, d.z), 0).
+__device__ __forceinline__ float vlen(float x, float y, float z) { return sqrtf(x * x + y * y + z * z); }
+
+__device__ __forceinline__ float box_distance(float px, float py, float pz, const float* s) {
+  const float dx = fabsf(px) - s[0], dy = fabsf(py) - s[1], dz = fabsf(pz) - s[2];
+  return vlen(fmaxf(dx, 0.0f), fmaxf(dy, 0.0f), fmaxf(dz, 0.0f)) + fminf(fmaxf(dx, fmaxf(dy, dz)), 0.0f);
+}
+
+// One brush applied to every voxel (insert_sphere_to_texture,
+// RayEngine/src/cuda-src/renderer.cu:30-92): offset p = voxel point - brush
+// point, optionally snapped in x and z; then, on (distance, material) pairs,
+// SPHERE_ADD: sdf_smin (h = max(k - |a - b|, 0) / k, the pair of the smaller
+// distance with h^2 k / 4 taken off its distance), SPHERE_SUBTRACT and
+// CUBE_SUBTRACT: the pair of the larger distance against the negated brush
+// distance (sdf_fmaxf), CUBE_ADD: the componentwise minimum (fminf on
+// float2) -- sdf_functions.cuh:270-289.  One work-group per brick, as in the
+// noise fill; the kernel streams the volume once (8 bytes in and out per
+// voxel).
+__global__ void __launch_bounds__(256) volume_edit_kernel(EditLaunch L) {
+  const int64_t brick = blockIdx.x;
+  const int gx = (int)(brick % L.bdim[0]), gy = (int)((brick / L.bdim[0]) % L.bdim[1]);
+  const int gz = (int)(brick / ((int64_t)L.bdim[0] * L.bdim[1]));
+  for (int m = threadIdx.x; m < kBrickVox; m += blockDim.x) {
+    const int x = gx * kBrick + (int)compact3((uint32_t)m);
+    const int y = gy * kBrick + (int)compact3((uint32_t)m >> 1);
+    const int z = gz * kBrick + (int)compact3((uint32_t)m >> 2);
+    if (x >= L.dim[0] || y >= L.dim[1] || z >= L.dim[2]) continue;
+    const int64_t i = brick * kBrickVox + m;
+    float px = (float)x * L.spacing[0] - L.center[0];
+    const float py = (float)y * L.spacing[1] - L.center[1];
+    float pz = (float)z * L.spacing[2] - L.center[2];
+    if (L.snap) {
+      const float snap = L.extent[0];
+      px = floorf(px - fabsf(fmodf(px, snap)));
+      pz = floorf(pz - fabsf(fmodf(pz, snap)));
+    }
+    const float ad = L.data[i], am = L.mat[i];
+    float rd = ad, rm = am;
+    if (L.type == 0) {
+      const float bd = vlen(px, py, pz) - L.extent[0];
+      const float h = fmaxf(L.k - fabsf(ad - bd), 0.0f) / L.k;
+      const bool a_min = ad < bd ? true : false;
+      rd = (a_min ? ad : bd) - h * h * L.k * 0.25f;
+      rm = a_min ? am : L.material;
+    } else if (L.type == 1) {
+      const float bd = -(vlen(px, py, pz) - L.extent[0]);
+      if (!(ad > bd)) {
+        rd = bd;
+        rm = L.material;
+      }
+    } else if (L.type == 2) {
+      const float bd = box_distance(px, py, pz, L.extent);
+      rd = fminf(ad, bd);
+      rm = fminf(am, L.material);
+    } else {
+      const float bd = -box_distance(px, py, pz, L.extent);
+      if (!(ad > bd)) {
+        rd = bd;
+        rm = L.material;
+      }
+    }
+    L.data[i] = rd;
+    L.mat[i] = rm;
+  }
+}
+
 }  // namespace
+
+hipError_t launch_volume_edit(const EditLaunch& L, hipStream_t s) {
+  const int64_t bricks = (int64_t)L.bdim[0] * L.bdim[1] * L.bdim[2];
+  if (bricks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(volume_edit_kernel, dim3((unsigned)bricks), dim3(256), 0, s, L);
+  return hipGetLastError();
+}
 
 hipError_t launch_noise_fill(const NoiseLaunch& L, hipStream_t s) {
   const int64_t bricks = (int64_t)L.bdim[0] * L.bdim[1] * L.bdim[2];
