@@ -221,6 +221,23 @@ def terrain_section(re, W, torch, ctx):
     return out
 
 
+def kd_build_section(re, ctx, tri_v, host_tree, max_leaf, host_build_s, reps=3):
+    """Device kd-tree build of the cluster soup (re_kd_build_gpu) beside the
+    host build, and whether the two trees are identical."""
+    dev = None
+    ms, walls = [], []
+    for _ in range(reps + 1):
+        t0 = time.perf_counter()
+        dev = re.KDTree(tri_v, max_leaf_tris=max_leaf, ctx=ctx)
+        walls.append(time.perf_counter() - t0)
+        ms.append(dev.build_ms)
+    same = all(np.array_equal(a.view(np.uint8), b.view(np.uint8)) for a, b in zip(host_tree.export(), dev.export()))
+    return {"triangles": int(len(tri_v)), "device_ms": round(float(np.mean(ms[1:])), 2),
+            "call_ms": round(float(np.mean(walls[1:])) * 1e3, 1), "host_build_s": round(host_build_s, 2),
+            "speedup_vs_host_call": round(host_build_s / float(np.mean(walls[1:])), 1), "identical_to_host": bool(same),
+            "note": "call_ms includes the host input check, the upload of the soup and the download of the arrays"}
+
+
 def shadow_section(W, torch, ctx, scene, tri_v, po, pd, pp, pt, order, steps=10):
     """Any-hit (shadow) queries: one ray from each primary hit toward a point
     light (pixels in the tiled order of the closest-hit line), timed on the
@@ -305,6 +322,8 @@ def main():
     tree = re.KDTree(tri_v, max_leaf_tris=tune["max_leaf_tris"])
     log(f"scene: {tri_v.shape[0]} tris, tree {tree.info['n_leaves']} leaves, "
         f"{tree.info['n_trirecs']} tri records, build {time.time() - t0:.1f} s")
+    host_build_s = time.time() - t0
+    kd_gpu = kd_build_section(re, ctx, tri_v, tree, tune["max_leaf_tris"], host_build_s)
     scene = ctx.upload_scene([re.SceneObject(tree)])
     po, pd = re.camera_rays(W.CLUSTER_CAMERA, 1920, 1080)
     pp, pt = scene.trace_closest(po, pd)
@@ -382,6 +401,7 @@ def main():
                      "work": {"node_entries": counts[0], "leaves": counts[1], "tri_tests": counts[2],
                               "rays_hit": counts[3]}},
     }
+    res["kd_build"] = kd_gpu
     res["shadow"] = shadow_section(W, torch, ctx, scene, tri_v, po, pd, pp, pt, order)
     if not args.no_terrain:
         try:

@@ -140,6 +140,19 @@ RE_API re_status re_kd_get_info(const re_kdtree* tree, re_kd_info* info);
  * n_nodes*8 bytes, leaves n_leaves*64 bytes, tris n_trirecs*40 bytes.  Any
  * pointer may be NULL to skip it. */
 RE_API re_status re_kd_export(const re_kdtree* tree, void* nodes, void* leaves, void* tris);
+/* Builds the same tree as re_kd_build on the context's GPU: breadth first,
+ * one wave per node choosing its split plane, references partitioned (and
+ * straddlers clipped) in parallel, then preorder numbering, ropes and the
+ * flat arrays on the device.  Replaces the host build the reference runs
+ * before every upload (RKDTreeCPU::build / buildRopeStructure /
+ * optimizeRopes, RayEngine/src/Primitives/KDTree.cpp:335-834, timed at
+ * RTScene.cpp:186-198) and the flattening (RKDThreeGPU::buildTree,
+ * KDThreeGPU.cpp:155-194).  The result is a tree object identical to
+ * re_kd_build's for the same input and parameters (n_bins at most 64);
+ * tri_v is host memory.  device_ms (nullable) gets the device time of the
+ * build, upload of the soup and download of the arrays excluded. */
+RE_API re_status re_kd_build_gpu(re_ctx* ctx, const float* tri_v, int64_t n_tris, const re_kd_build_params* params,
+                                 re_kdtree** out, float* device_ms);
 
 /* ------------------------------------------------------ scene upload */
 

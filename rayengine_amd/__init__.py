@@ -119,12 +119,22 @@ class KDTree:
     """Rope kd-tree over a triangle soup ((n, 9) float32: v0, v1, v2)."""
 
     def __init__(self, tri_v, max_leaf_tris: int = 8, max_depth: int = 24, cost_traversal: float = 1.0,
-                 cost_intersect: float = 1.5, empty_bonus: float = 0.8, n_bins: int = 32, clip: bool = True):
+                 cost_intersect: float = 1.5, empty_bonus: float = 0.8, n_bins: int = 32, clip: bool = True,
+                 ctx: "Context | None" = None):
+        """Host build (re_kd_build), or with ctx the device build
+        (re_kd_build_gpu, the same tree; its device time in build_ms)."""
         self.tri_v = _f32c(tri_v, 9).reshape(-1, 9)
         p = _lib.re_kd_build_params(max_leaf_tris, max_depth, cost_traversal, cost_intersect, empty_bonus, n_bins,
                                     int(clip))
         h = C.c_void_p()
-        check(lib().re_kd_build(ptr(self.tri_v), self.tri_v.shape[0], C.byref(p), C.byref(h)))
+        self.build_ms = None
+        if ctx is None:
+            check(lib().re_kd_build(ptr(self.tri_v), self.tri_v.shape[0], C.byref(p), C.byref(h)))
+        else:
+            ms = C.c_float()
+            check(lib().re_kd_build_gpu(ctx.handle, ptr(self.tri_v), self.tri_v.shape[0], C.byref(p), C.byref(h),
+                                        C.byref(ms)), ctx.handle)
+            self.build_ms = ms.value
         self._h = h
         info = _lib.re_kd_info()
         check(lib().re_kd_get_info(h, C.byref(info)))

@@ -123,6 +123,7 @@ SIGNATURES = {
     "re_gen_bounce_rays": (C.c_int, [_P, C.c_int64, _P, _P, _P, _P, C.c_int64, C.c_uint64, _P, _P, _I64]),
     "re_kd_default_params": (None, [C.POINTER(re_kd_build_params)]),
     "re_kd_build": (C.c_int, [_P, C.c_int64, C.POINTER(re_kd_build_params), C.POINTER(C.c_void_p)]),
+    "re_kd_build_gpu": (C.c_int, [_P, _P, C.c_int64, _P, _P, _P]),
     "re_kd_free": (None, [_P]),
     "re_kd_get_info": (C.c_int, [_P, C.POINTER(re_kd_info)]),
     "re_kd_export": (C.c_int, [_P, _P, _P, _P]),

@@ -457,6 +457,16 @@ RE_API re_status re_kd_export(const re_kdtree* tree, void* nodes, void* leaves, 
 
 // Internal accessors for the device upload (capi.cpp).
 namespace re {
+re_kdtree* kd_from_arrays(std::vector<NodeEntry>&& nodes, std::vector<LeafRec>&& leaves, std::vector<TriRec>&& tris,
+                          const re_kd_info& info) {
+  re_kdtree* t = new (std::nothrow) re_kdtree();
+  if (!t) return nullptr;
+  t->nodes = std::move(nodes);
+  t->leaves = std::move(leaves);
+  t->tris = std::move(tris);
+  t->info = info;
+  return t;
+}
 const std::vector<NodeEntry>& kd_nodes(const re_kdtree* t) { return t->nodes; }
 const std::vector<LeafRec>& kd_leaves(const re_kdtree* t) { return t->leaves; }
 const std::vector<TriRec>& kd_tris(const re_kdtree* t) { return t->tris; }
