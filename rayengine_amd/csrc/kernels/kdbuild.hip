@@ -334,47 +334,119 @@ __device__ __forceinline__ bool will_try(const LNode& nd) {
   return nd.mode != kModeLeaf && nd.axis >= 0 && !(nd.nl == nd.n && nd.nr == nd.n);
 }
 
-// Which side(s) each reference goes to, with clipped boxes for straddlers
-// (host: build, partition loop); per-node totals by atomics.
-__global__ void __launch_bounds__(256) part_flags_kernel(const float* __restrict__ tv, const Ref* __restrict__ refs,
+// Adds, per node, the below/above counts vl, vr (0 or 1) of the wave's
+// lanes; key is the lane's node (-1 for none).  The lanes of a wave usually
+// share one node (a node's references are contiguous), so one atomic per
+// wave replaces up to 64 on the same address.
+__device__ __forceinline__ void wave_count(LNode* __restrict__ nodes, int key, int vl, int vr) {
+  const int lane = threadIdx.x & 63;
+  const unsigned long long act = __ballot(1);
+  const int leader = __ffsll((unsigned long long)act) - 1;
+  const int lk = __shfl(key, leader);
+  const bool same = key == lk;
+  const unsigned long long ml = __ballot(same && vl), mr = __ballot(same && vr);
+  if (lane == leader && lk >= 0) {
+    if (ml) atomicAdd(&nodes[lk].cl, __popcll(ml));
+    if (mr) atomicAdd(&nodes[lk].cr, __popcll(mr));
+  }
+  if (!same && key >= 0) {
+    if (vl) atomicAdd(&nodes[key].cl, 1);
+    if (vr) atomicAdd(&nodes[key].cr, 1);
+  }
+}
+
+// Which side(s) each reference goes to (host: build, partition loop).
+// References that straddle the plane while clipping is on are listed for
+// part_clip_kernel, which clips them with full waves.
+__global__ void __launch_bounds__(256) part_flags_kernel(const Ref* __restrict__ refs,
                                                          const int32_t* __restrict__ ref_node, int64_t nref,
                                                          LNode* __restrict__ nodes, Params P,
                                                          int32_t* __restrict__ fl, int32_t* __restrict__ fr,
-                                                         Ref* __restrict__ box_l, Ref* __restrict__ box_r) {
+                                                         Ref* __restrict__ box_l, Ref* __restrict__ box_r,
+                                                         int32_t* __restrict__ strad, int32_t* __restrict__ n_strad) {
+  const int lane = threadIdx.x & 63;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nref) return;
-  LNode& nd = nodes[ref_node[i]];
+  const bool in = i < nref;
+  const int key = in ? ref_node[i] : -1;
   int l = 0, rr = 0;
-  if (will_try(nd)) {
+  bool st = false;
+  if (in && will_try(nodes[key])) {
+    const LNode& nd = nodes[key];
     const Ref r = refs[i];
-    const int ax = nd.axis;
-    const float pos = nd.pos;
-    const bool to_l = get3(r.lo, ax) < pos, to_r = get3(r.hi, ax) >= pos;
+    const bool to_l = get3(r.lo, nd.axis) < nd.pos, to_r = get3(r.hi, nd.axis) >= nd.pos;
     if (to_l && to_r && P.clip) {
-      float lhi[3] = {nd.hi[0], nd.hi[1], nd.hi[2]}, rlo[3] = {nd.lo[0], nd.lo[1], nd.lo[2]};
-      if (ax == 0) { lhi[0] = pos; rlo[0] = pos; }
-      else if (ax == 1) { lhi[1] = pos; rlo[1] = pos; }
-      else { lhi[2] = pos; rlo[2] = pos; }
-      Ref c;
-      if (clip_box(tv, r.prim, nd.lo, lhi, c)) {
-        l = 1;
-        box_l[i] = c;
-      }
-      if (clip_box(tv, r.prim, rlo, nd.hi, c)) {
-        rr = 1;
-        box_r[i] = c;
-      }
+      st = true;
     } else {
       l = to_l ? 1 : 0;
       rr = to_r ? 1 : 0;
       box_l[i] = r;
       box_r[i] = r;
     }
-    if (l) atomicAdd(&nd.cl, 1);
-    if (rr) atomicAdd(&nd.cr, 1);
   }
-  fl[i] = l;
-  fr[i] = rr;
+  if (in) {
+    fl[i] = l;
+    fr[i] = rr;
+  }
+  const unsigned long long sm = __ballot(st);
+  if (sm) {
+    const int leader = __ffsll(sm) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(n_strad, __popcll(sm));
+    base = __shfl(base, leader);
+    if (st) strad[base + __popcll(sm & ((1ull << lane) - 1ull))] = (int32_t)i;
+  }
+  wave_count(nodes, key, l, rr);
+}
+
+// Thread 2s + side clips straddler s against the below (side 0) or above
+// (side 1) child box (host: Builder::clip_box on both children).
+__global__ void __launch_bounds__(256) part_clip_kernel(const float* __restrict__ tv, const Ref* __restrict__ refs,
+                                                        const int32_t* __restrict__ ref_node, LNode* __restrict__ nodes,
+                                                        const int32_t* __restrict__ strad,
+                                                        const int32_t* __restrict__ n_strad, int32_t* __restrict__ fl,
+                                                        int32_t* __restrict__ fr, Ref* __restrict__ box_l,
+                                                        Ref* __restrict__ box_r) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t ns = *n_strad;
+  if ((t - (threadIdx.x & 63)) >= 2 * ns) return;  // whole waves past the list leave together
+  const bool in = t < 2 * ns;
+  int key = -1, ok = 0;
+  const int side = (int)(t & 1);
+  if (in) {
+    const int64_t i = strad[t >> 1];
+    key = ref_node[i];
+    const LNode& nd = nodes[key];
+    float lo[3] = {nd.lo[0], nd.lo[1], nd.lo[2]}, hi[3] = {nd.hi[0], nd.hi[1], nd.hi[2]};
+    if (side == 0) {
+      if (nd.axis == 0) hi[0] = nd.pos;
+      else if (nd.axis == 1) hi[1] = nd.pos;
+      else hi[2] = nd.pos;
+    } else {
+      if (nd.axis == 0) lo[0] = nd.pos;
+      else if (nd.axis == 1) lo[1] = nd.pos;
+      else lo[2] = nd.pos;
+    }
+    Ref c;
+    ok = clip_box(tv, refs[i].prim, lo, hi, c) ? 1 : 0;
+    if (side == 0) {
+      fl[i] = ok;
+      if (ok) box_l[i] = c;
+    } else {
+      fr[i] = ok;
+      if (ok) box_r[i] = c;
+    }
+  }
+  wave_count(nodes, key, side == 0 ? ok : 0, side == 1 ? ok : 0);
+}
+
+// Totals of the three exclusive scans over a level's nodes.
+__global__ void totals_kernel(const int32_t* kids, const int32_t* kx, const int32_t* crefs, const int32_t* cx,
+                              const int32_t* leafn, const int32_t* lx, int64_t nn, int64_t* out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    out[0] = (int64_t)kx[nn - 1] + kids[nn - 1];
+    out[1] = (int64_t)cx[nn - 1] + crefs[nn - 1];
+    out[2] = (int64_t)lx[nn - 1] + leafn[nn - 1];
+  }
 }
 
 __global__ void decide_kernel(LNode* __restrict__ nodes, int32_t nn, int32_t* __restrict__ kids,
@@ -614,7 +686,7 @@ struct Bufs {
     return e;
   }
   void release(void* p) {
-    if (std::find(ptrs.begin(), ptrs.end(), p) != ptrs.end()) dead.push_back(p);
+    if (p && std::find(ptrs.begin(), ptrs.end(), p) != ptrs.end()) dead.push_back(p);
   }
   void flush() {
     for (void* p : dead) {
@@ -628,46 +700,38 @@ struct Bufs {
   }
 };
 
+// A device array that only grows; `keep` elements are copied on growth.
 template <typename T>
-hipError_t exclusive_scan(Bufs& B, const T* in, T* out, int64_t n, hipStream_t s) {
-  if (n <= 0) return hipSuccess;
-  size_t bytes = 0;
-  hipError_t e = hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, (int)n, s);
-  if (e != hipSuccess) return e;
-  char* tmp = nullptr;
-  e = B.alloc(&tmp, bytes);
-  if (e != hipSuccess) return e;
-  e = hipcub::DeviceScan::ExclusiveSum(tmp, bytes, in, out, (int)n, s);
-  B.release(tmp);
-  return e;
-}
+struct Arr {
+  T* p = nullptr;
+  int64_t cap = 0;
+  hipError_t ensure(Bufs& B, int64_t need, hipStream_t s, int64_t keep = 0) {
+    if (need <= cap) return hipSuccess;
+    const int64_t nc = std::max<int64_t>(need + need / 2, 64);
+    T* q = nullptr;
+    hipError_t e = B.alloc(&q, (size_t)nc);
+    if (e != hipSuccess) return e;
+    if (keep > 0) e = hipMemcpyAsync(q, p, (size_t)keep * sizeof(T), hipMemcpyDeviceToDevice, s);
+    B.release(p);
+    p = q;
+    cap = nc;
+    return e;
+  }
+};
 
-// total of an exclusively scanned array: last scanned value + last input
-template <typename T>
-hipError_t scan_total(const T* in, const T* out, int64_t n, hipStream_t s, int64_t& total) {
-  total = 0;
-  if (n <= 0) return hipSuccess;
-  T a = 0, b = 0;
-  hipError_t e = hipMemcpyAsync(&a, out + n - 1, sizeof(T), hipMemcpyDeviceToHost, s);
-  if (e == hipSuccess) e = hipMemcpyAsync(&b, in + n - 1, sizeof(T), hipMemcpyDeviceToHost, s);
-  if (e == hipSuccess) e = hipStreamSynchronize(s);
-  total = (int64_t)a + (int64_t)b;
-  return e;
-}
-
-template <typename T>
-hipError_t grow(Bufs& B, T** buf, int64_t& cap, int64_t need, int64_t used, hipStream_t s) {
-  if (need <= cap) return hipSuccess;
-  int64_t nc = std::max<int64_t>(need, cap * 2);
-  T* nb = nullptr;
-  hipError_t e = B.alloc(&nb, (size_t)nc);
-  if (e != hipSuccess) return e;
-  if (used > 0) e = hipMemcpyAsync(nb, *buf, (size_t)used * sizeof(T), hipMemcpyDeviceToDevice, s);
-  if (*buf) B.release(*buf);
-  *buf = nb;
-  cap = nc;
-  return e;
-}
+// Exclusive prefix sum with a reusable temporary buffer.
+struct Scanner {
+  Arr<char> tmp;
+  template <typename T>
+  hipError_t run(Bufs& B, const T* in, T* out, int64_t n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    size_t bytes = 0;
+    hipError_t e = hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, (int)n, s);
+    if (e == hipSuccess) e = tmp.ensure(B, (int64_t)bytes, s);
+    if (e == hipSuccess) e = hipcub::DeviceScan::ExclusiveSum(tmp.p, bytes, in, out, (int)n, s);
+    return e;
+  }
+};
 
 }  // namespace
 
@@ -700,15 +764,23 @@ hipError_t build_device(const float* tri_v, int64_t n, const float* root_min, co
   KDB_TRY(hipMemcpyAsync(tv, tri_v, (size_t)(9 * n) * sizeof(float), hipMemcpyHostToDevice, s));
   KDB_TRY(hipEventRecord(ev0, s));
 
-  Ref* refs = nullptr;
-  int32_t* ref_node = nullptr;
-  KDB_TRY(B.alloc(&refs, (size_t)n));
-  KDB_TRY(B.alloc(&ref_node, (size_t)n));
-  hipLaunchKernelGGL(init_refs_kernel, dim3(blocks(n, 256)), dim3(256), 0, s, tv, n, refs, ref_node);
+  // Level arrays: current and next references / nodes swap every level.
+  Arr<Ref> refs, nrefs, box_l, box_r;
+  Arr<int32_t> ref_node, nnode, fl, fr, flx, frx, strad;
+  Arr<LNode> lv, next;
+  Arr<int32_t> kids, crefs, leafn, kx, cx, lx, hist;
+  Arr<GNode> gn;
+  Arr<int32_t> pool;
+  Scanner scan;
+  int64_t* tot = nullptr;
+  int32_t* n_strad = nullptr;
+  KDB_TRY(B.alloc(&tot, 3));
+  KDB_TRY(B.alloc(&n_strad, 1));
+  KDB_TRY(refs.ensure(B, n, s));
+  KDB_TRY(ref_node.ensure(B, n, s));
+  hipLaunchKernelGGL(init_refs_kernel, dim3(blocks(n, 256)), dim3(256), 0, s, tv, n, refs.p, ref_node.p);
   KDB_TRY(hipGetLastError());
-
-  LNode* lv = nullptr;
-  KDB_TRY(B.alloc(&lv, 1));
+  KDB_TRY(lv.ensure(B, 1, s));
   {
     LNode root{};
     for (int a = 0; a < 3; ++a) {
@@ -719,90 +791,77 @@ hipError_t build_device(const float* tri_v, int64_t n, const float* root_min, co
     root.n = (int32_t)n;
     root.depth = 1;
     root.gid = 0;
-    KDB_TRY(hipMemcpyAsync(lv, &root, sizeof(root), hipMemcpyHostToDevice, s));
+    KDB_TRY(hipMemcpyAsync(lv.p, &root, sizeof(root), hipMemcpyHostToDevice, s));
   }
-  int64_t nn = 1, nref = n;
-  int64_t gcap = 1024, gused = 1, pcap = n + 1024, pused = 0;
-  GNode* gn = nullptr;
-  int32_t* pool = nullptr;
-  KDB_TRY(B.alloc(&gn, (size_t)gcap));
-  KDB_TRY(B.alloc(&pool, (size_t)pcap));
+  int64_t nn = 1, nref = n, gused = 1, pused = 0;
+  KDB_TRY(gn.ensure(B, 1024, s));
+  KDB_TRY(pool.ensure(B, n + 1024, s));
   std::vector<std::pair<int32_t, int32_t>> levels;  // (first gid, count)
   int32_t gbase = 0;
 
   while (nn > 0) {
     levels.emplace_back(gbase, (int32_t)nn);
-    hipLaunchKernelGGL(classify_kernel, dim3(blocks(nn, 256)), dim3(256), 0, s, lv, (int32_t)nn, P);
-    hipLaunchKernelGGL(exact_split_kernel, dim3(blocks(nn * 64, 256)), dim3(256), 0, s, lv, (int32_t)nn, refs, P);
-    int32_t* hist = nullptr;
-    KDB_TRY(B.alloc(&hist, (size_t)(nn * 6 * nb)));
-    KDB_TRY(hipMemsetAsync(hist, 0, (size_t)(nn * 6 * nb) * sizeof(int32_t), s));
-    hipLaunchKernelGGL(bin_count_kernel, dim3(blocks(nref, kBinChunk)), dim3(256), 0, s, refs, ref_node, nref, lv,
-                       hist, P);
-    hipLaunchKernelGGL(binned_split_kernel, dim3(blocks(nn * 64, 256)), dim3(256), 0, s, lv, (int32_t)nn, hist, P);
+    hipLaunchKernelGGL(classify_kernel, dim3(blocks(nn, 256)), dim3(256), 0, s, lv.p, (int32_t)nn, P);
+    hipLaunchKernelGGL(exact_split_kernel, dim3(blocks(nn * 64, 256)), dim3(256), 0, s, lv.p, (int32_t)nn, refs.p,
+                       P);
+    KDB_TRY(hist.ensure(B, nn * 6 * nb, s));
+    KDB_TRY(hipMemsetAsync(hist.p, 0, (size_t)(nn * 6 * nb) * sizeof(int32_t), s));
+    hipLaunchKernelGGL(bin_count_kernel, dim3(blocks(nref, kBinChunk)), dim3(256), 0, s, refs.p, ref_node.p, nref,
+                       lv.p, hist.p, P);
+    hipLaunchKernelGGL(binned_split_kernel, dim3(blocks(nn * 64, 256)), dim3(256), 0, s, lv.p, (int32_t)nn, hist.p,
+                       P);
     KDB_TRY(hipGetLastError());
-    B.release(hist);
 
-    int32_t *fl = nullptr, *fr = nullptr, *flx = nullptr, *frx = nullptr;
-    Ref *bl = nullptr, *br = nullptr;
-    KDB_TRY(B.alloc(&fl, (size_t)nref));
-    KDB_TRY(B.alloc(&fr, (size_t)nref));
-    KDB_TRY(B.alloc(&flx, (size_t)nref));
-    KDB_TRY(B.alloc(&frx, (size_t)nref));
-    KDB_TRY(B.alloc(&bl, (size_t)nref));
-    KDB_TRY(B.alloc(&br, (size_t)nref));
-    hipLaunchKernelGGL(part_flags_kernel, dim3(blocks(nref, 256)), dim3(256), 0, s, tv, refs, ref_node, nref, lv, P,
-                       fl, fr, bl, br);
+    for (Arr<int32_t>* a : {&fl, &fr, &flx, &frx, &strad}) KDB_TRY(a->ensure(B, nref, s));
+    KDB_TRY(box_l.ensure(B, nref, s));
+    KDB_TRY(box_r.ensure(B, nref, s));
+    KDB_TRY(hipMemsetAsync(n_strad, 0, sizeof(int32_t), s));
+    hipLaunchKernelGGL(part_flags_kernel, dim3(blocks(nref, 256)), dim3(256), 0, s, refs.p, ref_node.p, nref, lv.p, P,
+                       fl.p, fr.p, box_l.p, box_r.p, strad.p, n_strad);
+    if (P.clip)
+      hipLaunchKernelGGL(part_clip_kernel, dim3(blocks(2 * nref, 256)), dim3(256), 0, s, tv, refs.p, ref_node.p,
+                         lv.p, strad.p, n_strad, fl.p, fr.p, box_l.p, box_r.p);
     KDB_TRY(hipGetLastError());
-    KDB_TRY(exclusive_scan(B, fl, flx, nref, s));
-    KDB_TRY(exclusive_scan(B, fr, frx, nref, s));
+    KDB_TRY(scan.run(B, fl.p, flx.p, nref, s));
+    KDB_TRY(scan.run(B, fr.p, frx.p, nref, s));
 
-    int32_t *kids = nullptr, *crefs = nullptr, *leafn = nullptr, *kx = nullptr, *cx = nullptr, *lx = nullptr;
-    KDB_TRY(B.alloc(&kids, (size_t)nn));
-    KDB_TRY(B.alloc(&crefs, (size_t)nn));
-    KDB_TRY(B.alloc(&leafn, (size_t)nn));
-    KDB_TRY(B.alloc(&kx, (size_t)nn));
-    KDB_TRY(B.alloc(&cx, (size_t)nn));
-    KDB_TRY(B.alloc(&lx, (size_t)nn));
-    hipLaunchKernelGGL(decide_kernel, dim3(blocks(nn, 256)), dim3(256), 0, s, lv, (int32_t)nn, kids, crefs, leafn);
+    for (Arr<int32_t>* a : {&kids, &crefs, &leafn, &kx, &cx, &lx}) KDB_TRY(a->ensure(B, nn, s));
+    hipLaunchKernelGGL(decide_kernel, dim3(blocks(nn, 256)), dim3(256), 0, s, lv.p, (int32_t)nn, kids.p, crefs.p,
+                       leafn.p);
     KDB_TRY(hipGetLastError());
-    KDB_TRY(exclusive_scan(B, kids, kx, nn, s));
-    KDB_TRY(exclusive_scan(B, crefs, cx, nn, s));
-    KDB_TRY(exclusive_scan(B, leafn, lx, nn, s));
-    int64_t n_next = 0, r_next = 0, l_now = 0;
-    KDB_TRY(scan_total(kids, kx, nn, s, n_next));
-    KDB_TRY(scan_total(crefs, cx, nn, s, r_next));
-    KDB_TRY(scan_total(leafn, lx, nn, s, l_now));
-    if (gbase + nn + n_next > (int64_t)0x3fffffff || r_next > (int64_t)0x7fffffff)
-      return hipErrorInvalidValue;
-
-    KDB_TRY(grow(B, &gn, gcap, gused + n_next, gused, s));
-    KDB_TRY(grow(B, &pool, pcap, pused + l_now, pused, s));
-    LNode* next = nullptr;
-    Ref* nrefs = nullptr;
-    int32_t* nnode = nullptr;
-    KDB_TRY(B.alloc(&next, (size_t)n_next));
-    KDB_TRY(B.alloc(&nrefs, (size_t)r_next));
-    KDB_TRY(B.alloc(&nnode, (size_t)r_next));
-    const int32_t next_gid = gbase + (int32_t)nn;
-    hipLaunchKernelGGL(emit_kernel, dim3(blocks(nn, 256)), dim3(256), 0, s, lv, (int32_t)nn, kx, cx, lx, next,
-                       next_gid, pused, gn);
-    hipLaunchKernelGGL(scatter_kernel, dim3(blocks(nref, 256)), dim3(256), 0, s, refs, ref_node, nref, lv, fl, fr,
-                       flx, frx, bl, br, nrefs, nnode, pool);
-    KDB_TRY(hipGetLastError());
+    KDB_TRY(scan.run(B, kids.p, kx.p, nn, s));
+    KDB_TRY(scan.run(B, crefs.p, cx.p, nn, s));
+    KDB_TRY(scan.run(B, leafn.p, lx.p, nn, s));
+    hipLaunchKernelGGL(totals_kernel, dim3(1), dim3(64), 0, s, kids.p, kx.p, crefs.p, cx.p, leafn.p, lx.p, nn, tot);
+    int64_t t3[3] = {0, 0, 0};
+    KDB_TRY(hipMemcpyAsync(t3, tot, sizeof(t3), hipMemcpyDeviceToHost, s));
     KDB_TRY(hipStreamSynchronize(s));
-    for (void* p : {(void*)fl, (void*)fr, (void*)flx, (void*)frx, (void*)bl, (void*)br, (void*)kids, (void*)crefs,
-                    (void*)leafn, (void*)kx, (void*)cx, (void*)lx, (void*)lv, (void*)refs, (void*)ref_node})
-      B.release(p);
-    B.flush();
-    lv = next;
-    refs = nrefs;
-    ref_node = nnode;
+    const int64_t n_next = t3[0], r_next = t3[1], l_now = t3[2];
+    if (gbase + nn + n_next > (int64_t)0x3fffffff || r_next > (int64_t)0x7fffffff) return hipErrorInvalidValue;
+
+    KDB_TRY(gn.ensure(B, gused + n_next, s, gused));
+    KDB_TRY(pool.ensure(B, pused + l_now, s, pused));
+    KDB_TRY(next.ensure(B, n_next, s));
+    KDB_TRY(nrefs.ensure(B, r_next, s));
+    KDB_TRY(nnode.ensure(B, r_next, s));
+    const int32_t next_gid = gbase + (int32_t)nn;
+    hipLaunchKernelGGL(emit_kernel, dim3(blocks(nn, 256)), dim3(256), 0, s, lv.p, (int32_t)nn, kx.p, cx.p, lx.p,
+                       next.p, next_gid, pused, gn.p);
+    hipLaunchKernelGGL(scatter_kernel, dim3(blocks(nref, 256)), dim3(256), 0, s, refs.p, ref_node.p, nref, lv.p,
+                       fl.p, fr.p, flx.p, frx.p, box_l.p, box_r.p, nrefs.p, nnode.p, pool.p);
+    KDB_TRY(hipGetLastError());
+    std::swap(lv, next);
+    std::swap(refs, nrefs);
+    std::swap(ref_node, nnode);
     gused += n_next;
     pused += l_now;
     gbase = next_gid;
     nn = n_next;
     nref = r_next;
+    if (!B.dead.empty()) {
+      KDB_TRY(hipStreamSynchronize(s));
+      B.flush();
+    }
   }
 
   const int32_t total = (int32_t)gused;
@@ -813,14 +872,14 @@ hipError_t build_device(const float* tri_v, int64_t n, const float* root_min, co
   KDB_TRY(B.alloc(&lbase, (size_t)total));
   KDB_TRY(B.alloc(&ropes, (size_t)(6 * total)));
   for (auto it = levels.rbegin(); it != levels.rend(); ++it)
-    hipLaunchKernelGGL(sizes_kernel, dim3(blocks(it->second, 256)), dim3(256), 0, s, gn, it->first, it->second, size,
-                       nleaf);
+    hipLaunchKernelGGL(sizes_kernel, dim3(blocks(it->second, 256)), dim3(256), 0, s, gn.p, it->first, it->second,
+                       size, nleaf);
   KDB_TRY(hipGetLastError());
   KDB_TRY(hipMemsetAsync(entry, 0, sizeof(int32_t), s));
   KDB_TRY(hipMemsetAsync(lbase, 0, sizeof(int32_t), s));
   KDB_TRY(hipMemsetAsync(ropes, 0xff, 6 * sizeof(int32_t), s));
   for (auto& lvl : levels)
-    hipLaunchKernelGGL(preorder_kernel, dim3(blocks(lvl.second, 256)), dim3(256), 0, s, gn, lvl.first, lvl.second,
+    hipLaunchKernelGGL(preorder_kernel, dim3(blocks(lvl.second, 256)), dim3(256), 0, s, gn.p, lvl.first, lvl.second,
                        size, nleaf, entry, lbase, ropes);
   KDB_TRY(hipGetLastError());
   int32_t n_leaves = 0;
@@ -834,20 +893,19 @@ hipError_t build_device(const float* tri_v, int64_t n, const float* root_min, co
   KDB_TRY(B.alloc(&entries, (size_t)total));
   KDB_TRY(B.alloc(&leaf_cnt, (size_t)n_leaves));
   KDB_TRY(B.alloc(&leaf_gid, (size_t)n_leaves));
-  KDB_TRY(B.alloc(&first, (size_t)n_leaves));
+  KDB_TRY(B.alloc(&first, (size_t)n_leaves + 1));
   KDB_TRY(B.alloc(&stats, 2));
   KDB_TRY(hipMemsetAsync(stats, 0, 2 * sizeof(int32_t), s));
-  hipLaunchKernelGGL(leaves_kernel, dim3(blocks(total, 256)), dim3(256), 0, s, gn, total, entry, lbase, ropes, leaves,
-                     leaf_cnt, leaf_gid, stats);
-  hipLaunchKernelGGL(entries_kernel, dim3(blocks(total, 256)), dim3(256), 0, s, gn, total, entry, lbase, entries);
+  hipLaunchKernelGGL(leaves_kernel, dim3(blocks(total, 256)), dim3(256), 0, s, gn.p, total, entry, lbase, ropes,
+                     leaves, leaf_cnt, leaf_gid, stats);
+  hipLaunchKernelGGL(entries_kernel, dim3(blocks(total, 256)), dim3(256), 0, s, gn.p, total, entry, lbase, entries);
   KDB_TRY(hipGetLastError());
-  KDB_TRY(exclusive_scan(B, leaf_cnt, first, n_leaves, s));
-  int64_t n_tris = 0;
-  KDB_TRY(scan_total(leaf_cnt, first, n_leaves, s, n_tris));
+  KDB_TRY(scan.run(B, leaf_cnt, first, n_leaves, s));
+  int64_t n_tris = pused;  // every leaf reference becomes one triangle record
   TriRec* tris = nullptr;
   KDB_TRY(B.alloc(&tris, (size_t)n_tris));
-  hipLaunchKernelGGL(tris_kernel, dim3(blocks((int64_t)n_leaves * 64, 256)), dim3(256), 0, s, gn, n_leaves, leaf_gid,
-                     first, pool, tv, leaves, tris);
+  hipLaunchKernelGGL(tris_kernel, dim3(blocks((int64_t)n_leaves * 64, 256)), dim3(256), 0, s, gn.p, n_leaves,
+                     leaf_gid, first, pool.p, tv, leaves, tris);
   KDB_TRY(hipGetLastError());
   KDB_TRY(hipEventRecord(ev1, s));
 
