@@ -210,6 +210,19 @@ def terrain_section(re, W, torch, ctx):
                          "frac_err_gt_half_vox": round(float(np.mean(e > 0.5)), 5),
                          "hit_agree": round(float(np.mean(h == ref_h)), 5), "params": params}
     out["methods"] = methods
+    # frame output of the reference tracer's frame (sky for misses, shaded hits, RGBA8)
+    dn = torch.empty(n * 3, dtype=torch.float32, device=dev)
+    dpx = torch.empty(n, dtype=torch.int32, device=dev)
+    torch.cuda.synchronize()
+    vol.trace_device(do.data_ptr(), dd.data_ptr(), n, dt.data_ptr(), dh.data_ptr(), nrm_ptr=dn.data_ptr(),
+                     **TERRAIN_METHODS["reference_march"][1])
+    fr = []
+    for _ in range(6):
+        re.frame_shade_device(ctx, dd.data_ptr(), dh.data_ptr(), dn.data_ptr(), 1920, 1080, dpx.data_ptr())
+        fr.append(ctx.last_kernel_ms())
+    fms = float(np.mean(fr[1:]))
+    out["frame"] = {"kernel": "frame_kernel", "kernel_ms": round(fms, 4), "mpixels_s": round(n / (fms * 1e-3) / 1e6, 1),
+                    "sky_pixels": int(n - int(dh.sum().item()))}
     hy, fine, ex = methods["surrogate_hybrid"], methods["reference_march_fine"], methods["exact_march"]
     out["surrogate_vs_reference_march"] = round(hy["mrays_s"] / fine["mrays_s"], 3)
     out["surrogate_vs_exact_march"] = round(hy["mrays_s"] / ex["mrays_s"], 3)

@@ -273,6 +273,43 @@ RE_API re_status re_volume_info(re_ctx* ctx, int32_t vol_id, int32_t* dim, float
                                 int64_t* bytes);
 RE_API re_status re_volume_release(re_ctx* ctx, int32_t vol_id);
 
+/* ---- frame output -------------------------------------------------------- */
+
+typedef struct re_frame_params {
+  float sun_dir[3];         /* Atmosphere::sunDirection, towards the sun; default (0, 1, 0), the
+                               direction Craze sets at angle 0 (renderer.cu:290-300) */
+  float earth_radius;       /* Atmosphere defaults (Atmosphere.cuh): 6360e3 */
+  float atmosphere_radius;  /* 6420e3 */
+  float hr;                 /* Rayleigh scale height, 79945 as in the reference */
+  float hm;                 /* Mie scale height, 1200 */
+  float sky_origin[3];      /* viewpoint of the sky, (0, earth_radius + 10000, 300000) (renderer.cu:262) */
+  float base_color[3];      /* colour of hit pixels before simple_shade, default 0 */
+  int32_t gamma;            /* RenderingSettings.gamma, default 0 */
+  int32_t vignetting;       /* RenderingSettings.vignetting, default 0 */
+  float vignetting_k;       /* default 0.25 */
+} re_frame_params;
+
+RE_API void re_frame_default_params(re_frame_params* p);
+/* Colours a width x height frame (pixel index y * width + x) from its rays
+ * and hits, the per-pixel tail of render_sphere_trace
+ * (RayEngine/src/cuda-src/renderer.cu:257-283): a missed pixel gets the
+ * single-scattering sky (compute_incident_light, material_functions.cuh:
+ * 295-344; 32 view and 8 light samples, the reference's expression kept)
+ * and its tone curve; a hit pixel gets the normal-visualisation material
+ * (simple_shade, cuda_helper_functions.h:137-140) on base_color; then
+ * optional gamma and vignetting, clipping to [0, 1] and packing to RGBA8
+ * (rgbaFloatToInt, cuda_helper_functions.h:60-68: R in the low byte, alpha
+ * 255).  ray_d: 3 floats per pixel; hit: 0/1 per pixel; normal: 3 floats
+ * per pixel (read for hits only).  Host pointers, or device pointers with
+ * RE_DEVICE_PTRS. */
+RE_API re_status re_frame_shade(re_ctx* ctx, const float* ray_d, const uint8_t* hit, const float* normal,
+                                int32_t width, int32_t height, const re_frame_params* params, uint32_t* rgba,
+                                uint32_t flags);
+/* Writes RGBA8 pixels (as re_frame_shade packs them) as a binary PPM (P6,
+ * RGB, rows top to bottom), the headless counterpart of the reference's
+ * window output. */
+RE_API re_status re_write_ppm(const char* path, const uint32_t* rgba, int32_t width, int32_t height);
+
 /* ---- .rsdf files and SDF edits ---------------------------------------- */
 
 /* Writes a volume as a .rsdf file: int32 dim[3], float spacing[3], then

@@ -35,6 +35,10 @@ _SIGS = {
     "orc_noise_fill": (None, [_P, _P, _P, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, _P, C.c_int]),
     "orc_noise_fill_f64": (None, [_P, _P, _P, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, _P, C.c_int]),
     "orc_trilinear": (C.c_float, [_P, _P, _P, C.c_float, C.c_float, C.c_float]),
+    "orc_sky": (None, [_P, _P, C.c_float, _P, C.c_int64, _P]),
+    "orc_sky_f64": (None, [_P, _P, C.c_float, _P, C.c_int64, _P]),
+    "orc_sky_tone": (C.c_float, [C.c_float]),
+    "orc_frame_shade": (None, [_P, _P, _P, C.c_int32, C.c_int32, _P, _P]),
     "orc_volume_edit": (None, [_P, _P, _P, _P, C.c_int, _P, _P, C.c_float, C.c_int, C.c_float]),
     "orc_sdf_trace": (None, [_P, _P, _P, _P, _P, C.c_int64, C.c_int, C.c_float, C.c_float, C.c_int, _P, _P, _P, _P,
                              C.c_int]),
@@ -317,3 +321,56 @@ def volume_edit(dist, mat, spacing, kind, center, extent, material=0.0, snap=Fal
     lib().orc_volume_edit(_p(dist), _p(mat), _p(dim), _p(sp), BRUSH_TYPES[kind], _p(c), _p(e), float(material),
                           int(bool(snap)), float(smooth_k))
     return dist, mat
+
+
+class FrameParams(C.Structure):
+    """re_frame_params layout."""
+    _fields_ = [("sun_dir", C.c_float * 3), ("earth_radius", C.c_float), ("atmosphere_radius", C.c_float),
+                ("hr", C.c_float), ("hm", C.c_float), ("sky_origin", C.c_float * 3), ("base_color", C.c_float * 3),
+                ("gamma", C.c_int32), ("vignetting", C.c_int32), ("vignetting_k", C.c_float)]
+
+
+def frame_params(**kw):
+    """The frame defaults (sun (0, 1, 0), Atmosphere radii and scale heights,
+    sky origin (0, 6370e3, 3e5), no gamma or vignetting), overridden by
+    keywords."""
+    p = FrameParams()
+    p.sun_dir[:] = [0.0, 1.0, 0.0]
+    p.earth_radius, p.atmosphere_radius, p.hr, p.hm = 6360e3, 6420e3, 79945.0, 1200.0
+    p.sky_origin[:] = [0.0, 6370e3, 300000.0]
+    p.base_color[:] = [0.0, 0.0, 0.0]
+    p.gamma, p.vignetting, p.vignetting_k = 0, 0, 0.25
+    for k, v in kw.items():
+        if isinstance(v, (tuple, list)):
+            getattr(p, k)[:] = [float(x) for x in v]
+        else:
+            setattr(p, k, v)
+    return p
+
+
+ATM_BETA_R = np.float32([3.8e-6, 13.5e-6, 33.1e-6])
+ATM_BETA_M = np.float32(21e-6)
+
+
+def sky(params, dirs, beta_r=ATM_BETA_R, beta_m=ATM_BETA_M, f64=False):
+    """compute_incident_light for each direction (before the tone curve);
+    params is an re_frame_params-layout ctypes structure."""
+    d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+    br = np.ascontiguousarray(beta_r, np.float32)
+    out = np.empty((len(d), 3), np.float64 if f64 else np.float32)
+    fn = lib().orc_sky_f64 if f64 else lib().orc_sky
+    fn(C.byref(params), _p(br), float(beta_m), _p(d), len(d), _p(out))
+    return out
+
+
+def sky_tone(c):
+    return np.float32(lib().orc_sky_tone(float(c)))
+
+
+def frame_shade(params, ray_d, hit, normal, width, height):
+    d = np.ascontiguousarray(ray_d, np.float32)
+    h = np.ascontiguousarray(hit, np.uint8)
+    nrm = np.ascontiguousarray(normal, np.float32)
+    out = np.empty((height, width), np.uint32)
+    lib().orc_frame_shade(_p(d), _p(h), _p(nrm), int(width), int(height), C.byref(params), _p(out))
+    return out

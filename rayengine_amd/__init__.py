@@ -289,6 +289,47 @@ def sdf_params(**kw) -> _lib.re_sdf_params:
     return p
 
 
+def frame_params(**kw) -> _lib.re_frame_params:
+    """re_frame_params with the defaults, overridden by keywords (sun_dir,
+    earth_radius, atmosphere_radius, hr, hm, sky_origin, base_color, gamma,
+    vignetting, vignetting_k)."""
+    p = _lib.re_frame_params()
+    lib().re_frame_default_params(C.byref(p))
+    for k, v in kw.items():
+        if k in ("sun_dir", "sky_origin", "base_color"):
+            getattr(p, k)[:] = [float(x) for x in v]
+        else:
+            setattr(p, k, type(getattr(p, k))(v))
+    return p
+
+
+def frame_shade(ctx: "Context", ray_d, hit, normal, width: int, height: int, **kw) -> np.ndarray:
+    """Colours a traced frame (re_frame_shade): (height, width) uint32 RGBA8."""
+    d = _f32c(ray_d, 3)
+    nrm = _f32c(normal, 3)
+    h = np.ascontiguousarray(hit, np.uint8)
+    out = np.empty((height, width), np.uint32)
+    p = frame_params(**kw)
+    check(lib().re_frame_shade(ctx.handle, ptr(d), ptr(h), ptr(nrm), int(width), int(height), C.byref(p), ptr(out),
+                               0), ctx.handle)
+    return out
+
+
+def frame_shade_device(ctx: "Context", d_ptr: int, hit_ptr: int, nrm_ptr: int, width: int, height: int, out_ptr: int,
+                       sync=True, **kw):
+    """re_frame_shade on device-resident arrays."""
+    p = frame_params(**kw)
+    flags = RE_DEVICE_PTRS | (0 if sync else RE_ASYNC)
+    check(lib().re_frame_shade(ctx.handle, d_ptr, hit_ptr, nrm_ptr, int(width), int(height), C.byref(p), out_ptr,
+                               flags), ctx.handle)
+
+
+def write_ppm(path, rgba: np.ndarray):
+    """Writes (height, width) RGBA8 pixels as a binary PPM (re_write_ppm)."""
+    a = np.ascontiguousarray(rgba, np.uint32)
+    check(lib().re_write_ppm(str(path).encode(), ptr(a), int(a.shape[1]), int(a.shape[0])))
+
+
 BRUSH_TYPES = {"sphere_add": 0, "sphere_subtract": 1, "cube_add": 2, "cube_subtract": 3}
 
 

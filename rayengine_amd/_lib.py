@@ -85,6 +85,12 @@ class re_hybrid_params(C.Structure):
                 ("max_iters", C.c_int32), ("block", C.c_int32)]
 
 
+class re_frame_params(C.Structure):
+    _fields_ = [("sun_dir", C.c_float * 3), ("earth_radius", C.c_float), ("atmosphere_radius", C.c_float),
+                ("hr", C.c_float), ("hm", C.c_float), ("sky_origin", C.c_float * 3), ("base_color", C.c_float * 3),
+                ("gamma", C.c_int32), ("vignetting", C.c_int32), ("vignetting_k", C.c_float)]
+
+
 class re_brush(C.Structure):
     _fields_ = [("type", C.c_int32), ("center", C.c_float * 3), ("extent", C.c_float * 3), ("material", C.c_float),
                 ("snap_to_grid", C.c_int32), ("smooth_k", C.c_float)]
@@ -146,6 +152,9 @@ SIGNATURES = {
     "re_volume_read": (C.c_int, [_P, C.c_int32, _P]),
     "re_volume_info": (C.c_int, [_P, C.c_int32, _P, _P, _I64]),
     "re_volume_release": (C.c_int, [_P, C.c_int32]),
+    "re_frame_default_params": (None, [_P]),
+    "re_frame_shade": (C.c_int, [_P, _P, _P, _P, C.c_int32, C.c_int32, _P, _P, C.c_uint32]),
+    "re_write_ppm": (C.c_int, [C.c_char_p, _P, C.c_int32, C.c_int32]),
     "re_volume_save_rsdf": (C.c_int, [_P, C.c_int32, C.c_char_p]),
     "re_volume_load_rsdf": (C.c_int, [_P, C.c_char_p, _I32]),
     "re_volume_read_material": (C.c_int, [_P, C.c_int32, _P]),

@@ -115,6 +115,19 @@ struct SdfLaunch {
 
 hipError_t launch_noise_fill(const NoiseLaunch& L, hipStream_t s);
 
+// Frame colouring (sky for misses, simple_shade for hits, tone, pack).
+struct FrameLaunch {
+  const float* ray_d;
+  const uint8_t* hit;
+  const float* normal;
+  int32_t width, height;
+  float sun[3], er, ar, hr, hm, origin[3], base[3];
+  int32_t gamma, vignetting;
+  float vignetting_k;
+  uint32_t* rgba;
+};
+hipError_t launch_frame(const FrameLaunch& L, hipStream_t s);
+
 // One brush edit over a whole volume (distance and material channels).
 struct EditLaunch {
   float* data;
