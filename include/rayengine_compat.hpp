@@ -232,6 +232,110 @@ inline void sphere_trace_primary_rays(HitResult* hit_results, const RCamera& cam
   for (size_t i = 0; i < n; ++i) hit_results[i] = HitResult{t[i], h[i] ? 0 : -1, h[i] ? 0 : -1, h[i] != 0};
 }
 
+// trace_shadow, RayEngine/src/cuda-src/ray_functions.cuh:365-374, for a
+// batch of rays against the uploaded scene: occluded[i] = 1 when some object
+// blocks ray i (traversed with is_secondary = true, as trace_shadow does).
+inline void trace_shadow_rays(uint8_t* occluded, const float3* origins, const float3* dirs, size_t n,
+                              int max_leaf_visits = 50) {
+  detail::State& s = detail::state();
+  if (s.scene < 0) throw Error(RE_ERR_INVALID, "trace_shadow_rays before copy_memory");
+  std::vector<float> o(3 * n), d(3 * n);
+  for (size_t i = 0; i < n; ++i) {
+    o[3 * i] = origins[i].x; o[3 * i + 1] = origins[i].y; o[3 * i + 2] = origins[i].z;
+    d[3 * i] = dirs[i].x; d[3 * i + 1] = dirs[i].y; d[3 * i + 2] = dirs[i].z;
+  }
+  re_trace_params tp;
+  re_trace_default_params(&tp);
+  tp.max_leaf_visits = max_leaf_visits;
+  tp.secondary = 1;
+  check(re_trace_occluded(s.ctx, s.scene, o.data(), d.data(), (int64_t)n, &tp, occluded, 0), s.ctx);
+}
+
+// TerrainBrushType / TerrainBrush, RayEngine/src/RayEngine/RayEngine.h:77-90.
+enum TerrainBrushType { SPHERE_ADD = 0, SPHERE_SUBTRACT = 1, CUBE_ADD = 2, CUBE_SUBTRACT = 3 };
+struct int3 { int x, y, z; };
+struct TerrainBrush {
+  TerrainBrushType brush_type;
+  int3 brush_extent;
+  int material_index;
+  bool snap_to_grid;
+};
+
+// insert_sphere_to_texture, RayEngine/src/cuda-src/renderer.cu:30-92: the
+// brush ray is sphere traced (the reference's clamp step rule and 1e-5
+// threshold) and, on a hit, the brush is applied around the hit point.
+// Returns whether the ray hit.
+inline bool insert_sphere_to_texture(const TerrainBrush& brush, float3 ray_o, float3 ray_dir) {
+  detail::State& s = detail::state();
+  if (s.volume < 0) throw Error(RE_ERR_INVALID, "brush before generate_noise / load_map");
+  const float o[3] = {ray_o.x, ray_o.y, ray_o.z}, d[3] = {ray_dir.x, ray_dir.y, ray_dir.z};
+  re_sdf_params sp;
+  re_sdf_default_params(&sp);
+  sp.step_rule = 1;
+  sp.eps = 10e-6f;
+  float t = 0.0f;
+  uint8_t hit = 0;
+  check(re_sdf_trace(s.ctx, s.volume, &sp, o, d, 1, &t, &hit, nullptr, nullptr, 0), s.ctx);
+  if (!hit) return false;
+  re_brush b;
+  re_brush_default(&b);
+  b.type = (int32_t)brush.brush_type;
+  for (int a = 0; a < 3; ++a) b.center[a] = o[a] + t * d[a];
+  b.extent[0] = (float)brush.brush_extent.x;
+  b.extent[1] = (float)brush.brush_extent.y;
+  b.extent[2] = (float)brush.brush_extent.z;
+  b.material = (float)brush.material_index;
+  b.snap_to_grid = brush.snap_to_grid ? 1 : 0;
+  check(re_volume_edit(s.ctx, s.volume, &b), s.ctx);
+  return true;
+}
+
+// save_map / load_map, RayEngine/src/cuda-src/renderer.cu:838-947, with the
+// file name given by the caller (the reference derives it from the chunk).
+inline void save_map(const std::string& path) {
+  detail::State& s = detail::state();
+  if (s.volume < 0) throw Error(RE_ERR_INVALID, "save_map without a volume");
+  check(re_volume_save_rsdf(s.ctx, s.volume, path.c_str()), s.ctx);
+}
+inline void load_map(const std::string& path) {
+  detail::State& s = detail::state();
+  re_ctx* c = detail::ctx();
+  int32_t v = -1;
+  check(re_volume_load_rsdf(c, path.c_str(), &v), c);
+  if (s.volume >= 0) check(re_volume_release(c, s.volume), c);
+  s.volume = v;
+}
+
+// The camera cuda_render_frame renders (the reference keeps it in device
+// state set by initialize_volume_render, sphere_tracing.cuh:617-755).
+inline void set_camera(const RCamera& cam) {
+  (void)detail::ctx();
+  detail::state().camera = cam;
+}
+
+// cuda_render_frame, RayEngine/src/cuda-src/renderer.cu:701-738, headless:
+// sphere traces the volume for the camera (with normals) and colours the
+// frame (sky for misses, the normal material for hits, RGBA8 with R in the
+// low byte) into output, width * height pixels.
+inline void cuda_render_frame(uint32_t* output, const uint32_t& w, const uint32_t& h) {
+  detail::State& s = detail::state();
+  if (s.volume < 0) throw Error(RE_ERR_INVALID, "cuda_render_frame before generate_noise / load_map");
+  const RCamera& cam = s.camera;
+  re_camera rc;
+  rc.pos[0] = cam.campos.x; rc.pos[1] = cam.campos.y; rc.pos[2] = cam.campos.z;
+  rc.view[0] = cam.view.x; rc.view[1] = cam.view.y; rc.view[2] = cam.view.z;
+  rc.camdown[0] = cam.camdown.x; rc.camdown[1] = cam.camdown.y; rc.camdown[2] = cam.camdown.z;
+  rc.fov[0] = cam.fov.x; rc.fov[1] = cam.fov.y;
+  rc.focal_distance = cam.focial_distance;
+  const size_t n = (size_t)w * h;
+  std::vector<float> o(3 * n), d(3 * n), t(n), nrm(3 * n);
+  std::vector<uint8_t> hit(n);
+  check(re_gen_camera_rays(&rc, (int)w, (int)h, o.data(), d.data()));
+  check(re_sdf_trace(s.ctx, s.volume, nullptr, o.data(), d.data(), (int64_t)n, t.data(), hit.data(), nullptr,
+                     nrm.data(), 0), s.ctx);
+  check(re_frame_shade(s.ctx, d.data(), hit.data(), nrm.data(), (int32_t)w, (int32_t)h, nullptr, output, 0), s.ctx);
+}
+
 // free_memory, RayEngine/src/cuda-src/sphere_tracing.cuh:757-767.
 inline void free_memory() {
   detail::State& s = detail::state();

@@ -47,6 +47,38 @@ int main(int argc, char** argv) {
   int n_hit = 0;
   for (const HitResult& r : hits) n_hit += r.hits;
   std::printf("hits %d of %u\n", n_hit, w * h);
+  if (argc > 3) {
+    // trace_shadow: eight rays from outside the sphere, four towards it, four away
+    std::vector<float3> so(8), sd(8);
+    for (int i = 0; i < 8; ++i) {
+      so[i] = float3{0.1f * i, 0.05f * i, 4.0f};
+      sd[i] = float3{0.0f, 0.0f, i < 4 ? -1.0f : 1.0f};
+    }
+    std::vector<uint8_t> occ(8);
+    trace_shadow_rays(occ.data(), so.data(), sd.data(), 8);
+    std::printf("shadow");
+    for (uint8_t v : occ) std::printf(" %d", v);
+    std::printf("\n");
+    // terrain: generate, brush from above, save and reload, render a frame
+    const std::string dir = argv[3];
+    generate_noise(64, 2016);
+    TerrainBrush brush{SPHERE_ADD, int3{1, 1, 1}, 3, false};
+    const bool bhit = insert_sphere_to_texture(brush, float3{0.5f, 2.0f, 0.5f}, float3{0.0f, -1.0f, 0.0f});
+    std::printf("brush %d\n", bhit ? 1 : 0);
+    save_map(dir + "/map.rsdf");
+    load_map(dir + "/map.rsdf");
+    RCamera tc;
+    tc.campos = float3{0.5f, 0.95f, -0.35f};
+    tc.view = float3{0.0f, -0.55f, 1.0f};
+    set_camera(tc);
+    const uint32_t fw = 96, fh = 64;
+    std::vector<uint32_t> px(fw * fh);
+    cuda_render_frame(px.data(), fw, fh);
+    FILE* g = std::fopen((dir + "/frame.bin").c_str(), "wb");
+    std::fwrite(px.data(), 4, px.size(), g);
+    std::fclose(g);
+    check(re_write_ppm((dir + "/frame.ppm").c_str(), px.data(), (int32_t)fw, (int32_t)fh));
+  }
   free_memory();
   return 0;
 }

@@ -12,6 +12,14 @@ if str(ROOT) not in sys.path:
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU")
     config.addinivalue_line("markers", "slow: takes more than a few seconds on the CPU")
+    # torch first: it loads its bundled HIP runtime as libamdhip64.so, and
+    # librayengine.so's libamdhip64.so.7 then resolves to that same copy by
+    # soname.  Loaded the other way round the process holds two runtimes and
+    # the library finds no device.
+    try:
+        import torch  # noqa: F401
+    except Exception:
+        pass
 
 
 def _gpu_available() -> bool:

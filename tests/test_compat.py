@@ -48,3 +48,34 @@ def test_compat_trace_matches_python_api(demo, gpu_ctx, tmp_path):
     assert np.array_equal(rec[:, 0].view(np.int32), p)
     assert np.array_equal(rec[:, 1], t.view(np.uint32))
     assert (p >= 0).sum() > 100
+
+
+@pytest.mark.gpu
+def test_compat_shadow_brush_files_and_frame(demo, gpu_ctx, tmp_path):
+    """trace_shadow_rays, insert_sphere_to_texture, save_map / load_map and
+    cuda_render_frame through the compat header agree with the C-ABI calls
+    made from Python."""
+    out = tmp_path / "hits.bin"
+    r = subprocess.run([str(demo), "gpu", str(out), str(tmp_path)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.splitlines()
+    occ = [int(x) for x in [l for l in lines if l.startswith("shadow")][0].split()[1:]]
+    tree = re.KDTree(re.icosphere(2, 1.0))
+    rot = float(np.float32(0.3))
+    scene = gpu_ctx.upload_scene([re.SceneObject(tree, location=(0.25, -0.1, 0.0), rotation_y=rot)])
+    so = np.float32([[0.1 * i, 0.05 * i, 4.0] for i in range(8)])
+    sd = np.float32([[0, 0, -1 if i < 4 else 1] for i in range(8)])
+    assert occ == list(scene.occluded(so, sd))
+    assert occ[:4] == [1, 1, 1, 1] and occ[4:] == [0, 0, 0, 0]
+    scene.release()
+    assert "brush 1" in lines
+    # the reloaded map is the edited volume; the frame is the Python pipeline's
+    vol = gpu_ctx.load_rsdf(tmp_path / "map.rsdf")
+    assert tuple(vol.dim) == (64, 64, 64) and (vol.read_material() == 3).any()
+    o, d = re.camera_rays(re.Camera(pos=(0.5, 0.95, -0.35), view=(0.0, -0.55, 1.0)), 96, 64)
+    t, h, _, nrm = vol.trace(o, d, normals=True)
+    px = re.frame_shade(gpu_ctx, d, h, nrm, 96, 64)
+    demo_px = np.fromfile(tmp_path / "frame.bin", np.uint32).reshape(64, 96)
+    assert np.array_equal(demo_px, px)
+    assert (tmp_path / "frame.ppm").read_bytes()[:12] == b"P6\n96 64\n255"
+    vol.release()
