@@ -132,8 +132,10 @@ __device__ __forceinline__ void trace_ray(const NodeEntry* __restrict__ nodes, c
     const ObjXform ob = objs[k];
     f3 ro, rd;
     xform_ray(ob, o, d, secondary, ro, rd);
+    // the slab reciprocals once per object, not per leaf (bit-identical)
+    const f3 fr{1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z};
     float t_entry, t_exit;
-    if (!ray_box(ob.root_min, ob.root_max, ro, rd, t_entry, t_exit)) continue;
+    if (!ray_box_inv(ob.root_min, ob.root_max, ro, fr, t_entry, t_exit)) continue;
     int32_t cur = ob.root;
     bool found = false;
     float hit_t = kInf;
@@ -166,7 +168,7 @@ __device__ __forceinline__ void trace_ray(const NodeEntry* __restrict__ nodes, c
       const uint32_t first = __float_as_uint(h1.z), count = __float_as_uint(h1.w);
       if (COUNT) ++rc.leaf;
       float tn, tf;
-      const bool box_ok = ray_box(bmin, bmax, ro, rd, tn, tf);
+      const bool box_ok = ray_box_inv(bmin, bmax, ro, fr, tn, tf);
       const int face = box_ok ? exit_face(bmin, bmax, add(ro, smul(tf, rd))) : -1;
       const int32_t r = face >= 0 ? lp->rope[face] : kNoRope;
       have_window = r >= 0;
