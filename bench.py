@@ -204,7 +204,19 @@ def terrain_section(re, W, torch, ctx):
             run(kind, params, sync=False)
         torch.cuda.synchronize()
         ms = float(np.mean(ctx.kernel_times(5)))
-        methods[name] = {"mrays_s": round(n / (ms * 1e-3) / 1e6, 1), "kernel_ms": round(ms, 4),
+        # field samples per ray (untimed host-pointer call): 32 bytes of the
+        # trilinear cell per sample + 32 bytes of ray in and result out
+        if kind == "plain":
+            _, _, steps = vol.trace(o, d, **params)
+        else:
+            _, _, evals, steps = re.hybrid_trace(vol, net if kind == "hybrid" else None, o, d, **params)
+        alg = float(32 * n + 32 * np.sum(steps, dtype=np.int64))
+        roof = {"samples_per_ray": round(float(np.mean(steps)), 2),
+                "algorithmic_gbs": round(alg / (ms * 1e-3) / 1e9, 1),
+                "hbm_frac": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        if kind == "hybrid":
+            roof["network_evals_per_ray"] = round(float(np.mean(evals)), 2)
+        methods[name] = {"mrays_s": round(n / (ms * 1e-3) / 1e6, 1), "kernel_ms": round(ms, 4), "roofline": roof,
                          "err_p50_vox": round(float(np.median(e)), 4), "err_p99_vox": round(float(np.percentile(e, 99)), 4),
                          "err_max_vox": round(float(e.max()), 3) if e.size else None,
                          "frac_err_gt_half_vox": round(float(np.mean(e > 0.5)), 5),
