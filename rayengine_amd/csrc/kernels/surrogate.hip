@@ -197,44 +197,87 @@ __global__ void __launch_bounds__(512) mlp_infer_kernel(const bf16x8* __restrict
 
 // Per 64-point chunk (16 per wave) every wave runs forward and backward on its
 // tile and stages, transposed as [neuron][point] in LDS, the bf16 dZ of every
-// layer and every layer's input activations.  Wave w then accumulates the
-// weight gradient of layer w over the chunk's 64 points with K = points.
+// layer and the input activations of layers 0..3.  Wave w then accumulates the
+// weight gradient of layer w over the chunk's 64 points with K = points, and
+// its bias gradient as the product with a column of ones.  The output layer's
+// gradient is summed in registers, per lane over its own points, and reduced
+// over lanes and waves once at the end.
 constexpr int kChunkPts = 64;
+// Staged rows are padded by 16 bytes (72 bf16): the 16 rows of one MFMA operand
+// read then cover all 64 LDS banks once, and the four neuron groups of a
+// staging write fall in different banks.
+constexpr int kRow = kChunkPts + 8;
+// Layer-0 inputs staged: the 99 used, padded to 7 column tiles of 16.
+constexpr int kInStage = 112;
+constexpr int kTrainBlock = 256;
 
 struct TrainLds {
   bf16x8 pack[kPack >> 3];            // 64 KB: forward and backward operands
   float small[kSmall + 3];
-  __bf16 dzT[4][kH][kChunkPts];       // 32 KB
-  __bf16 xT[kInPad][kChunkPts];       // 16 KB
-  __bf16 hT[4][kH][kChunkPts];        // 32 KB (h0..h3)
-  float dout[kChunkPts];
+  __bf16 dzT[4][kH][kRow];            // 36 KB
+  __bf16 xT[kInStage][kRow];          // 15.75 KB
+  __bf16 hT[3][kH][kRow];             // 27 KB (h0..h2, the inputs of layers 1..3)
 };
+static_assert(sizeof(TrainLds) <= 160 * 1024, "one training block must fit the LDS of a CU");
+static_assert(sizeof(__bf16) * 4 * kH * kRow >= sizeof(float) * (4 * kH + 8),
+              "the final reduction reuses the dZ staging area");
+static_assert((kPack >> 3) % kTrainBlock == 0, "the operand pack is copied in whole rounds");
 
-__global__ void __launch_bounds__(256) mlp_train_kernel(const bf16x8* __restrict__ gpack,
-                                                        const float* __restrict__ gsmall,
-                                                        const float4* __restrict__ samples, int64_t n_samples,
-                                                        int64_t batch_start, int32_t batch, float over_weight,
-                                                        float* __restrict__ partial) {
+__device__ __forceinline__ int64_t train_sample_index(int64_t batch_start, int ch, int p_local, int64_t n_samples) {
+  return (batch_start + (int64_t)ch * kChunkPts + p_local) % n_samples;
+}
+
+__global__ void __launch_bounds__(kTrainBlock) mlp_train_kernel(const bf16x8* __restrict__ gpack,
+                                                                const float* __restrict__ gsmall,
+                                                                const float4* __restrict__ samples, int64_t n_samples,
+                                                                int64_t batch_start, int32_t batch, float over_weight,
+                                                                float* __restrict__ partial) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   TrainLds& S = *reinterpret_cast<TrainLds*>(lds_raw);
-  load_weights(gpack, gsmall, S.pack, S.small, kPack >> 3);
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
   const int g = lane >> 4, j = lane & 15;
   const int p_local = 16 * w + j;
   const float inv_b = 1.0f / (float)batch;
-
-  f32x4 acc[4][8];
-#pragma unroll
-  for (int a = 0; a < 4; ++a)
-#pragma unroll
-    for (int b = 0; b < 8; ++b) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
-  float db = 0.0f, dw4 = 0.0f, db4 = 0.0f, loss = 0.0f;
-
   const int n_chunks = batch / kChunkPts;
+
+  // The first chunk's sample is requested before the operand copy, and the
+  // copy issues all of a thread's loads before its first LDS store.
+  float4 smp_next = make_float4(0.f, 0.f, 0.f, 0.f);
+  if ((int)blockIdx.x < n_chunks) smp_next = samples[train_sample_index(batch_start, blockIdx.x, p_local, n_samples)];
+  {
+    constexpr int kPer = (kPack >> 3) / kTrainBlock;
+    bf16x8 r[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) r[k] = gpack[threadIdx.x + kTrainBlock * k];
+    for (int i = threadIdx.x; i < kSmall; i += kTrainBlock) S.small[i] = gsmall[i];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) S.pack[threadIdx.x + kTrainBlock * k] = r[k];
+  }
+  __syncthreads();
+
+  f32x4 acc[4][7], accb[4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    accb[a] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int b = 0; b < 7; ++b) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+  }
+  // B operand of the bias product: column 0 all ones, the other columns zero
+  bf16x8 ones;
+#pragma unroll
+  for (int s = 0; s < 8; ++s) ones[s] = (__bf16)(j == 0 ? 1.0f : 0.0f);
+  // per lane, over its points: dout * h3 for neurons 16 m + 4 g + r (index
+  // 4 m + r), and dout (g == 0 lanes)
+  float dw4r[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) dw4r[q] = 0.0f;
+  float db4r = 0.0f, loss = 0.0f;
+
   for (int ch = blockIdx.x; ch < n_chunks; ch += gridDim.x) {
-    const int64_t idx = (batch_start + (int64_t)ch * kChunkPts + p_local) % n_samples;
-    const float4 smp = samples[idx];
+    const float4 smp = smp_next;
+    if (ch + (int)gridDim.x < n_chunks)
+      smp_next = samples[train_sample_index(batch_start, ch + gridDim.x, p_local, n_samples)];
     const f3 p{smp.x, smp.y, smp.z};
     bf16x8 xb[4];
     features(p, g, xb);
@@ -243,7 +286,14 @@ __global__ void __launch_bounds__(256) mlp_train_kernel(const bf16x8* __restrict
     const float r = o - smp.w;
     const float wgt = r > 0.0f ? over_weight : 1.0f;
     const float dout = 2.0f * wgt * r * inv_b;
-    if (g == 0) loss += wgt * r * r * inv_b;
+    if (g == 0) {
+      loss += wgt * r * r * inv_b;
+      db4r += dout;
+    }
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+      for (int r2 = 0; r2 < 4; ++r2) dw4r[4 * m + r2] += dout * T.h3[m][r2];
 
     // backward through the hidden layers, chaining dZ in registers
     float dz[4][4];
@@ -253,7 +303,7 @@ __global__ void __launch_bounds__(256) mlp_train_kernel(const bf16x8* __restrict
       for (int r2 = 0; r2 < 4; ++r2) dz[m][r2] = dout * S.small[256 + 16 * m + 4 * g + r2] * dsilu(T.z[3][m][r2]);
 #pragma unroll
     for (int L = 3; L >= 0; --L) {
-      // stage dZ_L and the layer's input activations
+      // stage dZ_L
 #pragma unroll
       for (int m = 0; m < 4; ++m)
 #pragma unroll
@@ -277,48 +327,35 @@ __global__ void __launch_bounds__(256) mlp_train_kernel(const bf16x8* __restrict
 #pragma unroll
         for (int r2 = 0; r2 < 4; ++r2) dz[m][r2] = dh[m][r2] * dsilu(T.z[L - 1][m][r2]);
     }
-    // stage activations: features, h0..h3, and dout
+    // stage the inputs of layers 0..3: features (rows 0..111), h0..h2
 #pragma unroll
-    for (int c = 0; c < 4; ++c)
+    for (int c = 0; c < 4; ++c) {
+      if (c == 3 && g >= 2) break;  // rows 112..127 are zero padding, never read
 #pragma unroll
       for (int s = 0; s < 8; ++s) S.xT[32 * c + 8 * g + s][p_local] = xb[c][s];
+    }
 #pragma unroll
-    for (int L = 0; L < 4; ++L)
+    for (int L = 0; L < 3; ++L)
 #pragma unroll
       for (int m = 0; m < 4; ++m)
 #pragma unroll
         for (int r2 = 0; r2 < 4; ++r2) S.hT[L][16 * m + 4 * g + r2][p_local] = T.hb[L][m >> 1][(m & 1) * 4 + r2];
-    if (g == 0) S.dout[p_local] = dout;
     __syncthreads();
 
     // weight gradient of layer w: dW = dZ^T (64 x pts) . H (pts x in), K = points
-    const int n_tiles = (w == 0) ? 8 : 4;
+    const int n_tiles = (w == 0) ? 7 : 4;
 #pragma unroll
     for (int kc = 0; kc < 2; ++kc) {
 #pragma unroll
       for (int mi = 0; mi < 4; ++mi) {
         const bf16x8 a = *reinterpret_cast<const bf16x8*>(&S.dzT[w][16 * mi + j][32 * kc + 8 * g]);
+        accb[mi] = mfma16(a, ones, accb[mi]);
 #pragma unroll
-        for (int nj = 0; nj < 8; ++nj) {
+        for (int nj = 0; nj < 7; ++nj) {
           if (nj >= n_tiles) break;
           const __bf16* brow = (w == 0) ? &S.xT[16 * nj + j][32 * kc + 8 * g] : &S.hT[w - 1][16 * nj + j][32 * kc + 8 * g];
           acc[mi][nj] = mfma16(a, *reinterpret_cast<const bf16x8*>(brow), acc[mi][nj]);
         }
-      }
-    }
-    // bias of layer w (lane = neuron), and the output layer on wave 3
-    {
-      float s = 0.0f;
-      for (int q = 0; q < kChunkPts; ++q) s += (float)S.dzT[w][lane][q];
-      db += s;
-      if (w == 3) {
-        float s4 = 0.0f, sb = 0.0f;
-        for (int q = 0; q < kChunkPts; ++q) {
-          s4 += S.dout[q] * (float)S.hT[3][lane][q];
-          sb += S.dout[q];
-        }
-        dw4 += s4;
-        db4 += sb;
       }
     }
     __syncthreads();
@@ -326,33 +363,58 @@ __global__ void __launch_bounds__(256) mlp_train_kernel(const bf16x8* __restrict
 
   // write this block's partial gradient (flat parameter layout) and loss
   float* P = partial + (int64_t)blockIdx.x * (kParams + 1);
-  const int nin = (w == 0) ? kIn : kH;
-  const int oW = (w == 0) ? oW0 : (w == 1 ? oW1 : (w == 2 ? oW2 : oW3));
-  const int ob = (w == 0) ? ob0 : (w == 1 ? ob1 : (w == 2 ? ob2 : ob3));
-  const int n_tiles = (w == 0) ? 8 : 4;
+  {
+    const int nin = (w == 0) ? kIn : kH;
+    const int oW = (w == 0) ? oW0 : (w == 1 ? oW1 : (w == 2 ? oW2 : oW3));
+    const int ob = (w == 0) ? ob0 : (w == 1 ? ob1 : (w == 2 ? ob2 : ob3));
+    const int n_tiles = (w == 0) ? 7 : 4;
+    if (j == 0)
 #pragma unroll
-  for (int mi = 0; mi < 4; ++mi)
+      for (int mi = 0; mi < 4; ++mi)
 #pragma unroll
-    for (int nj = 0; nj < 8; ++nj) {
-      if (nj >= n_tiles) break;
+        for (int r2 = 0; r2 < 4; ++r2) P[ob + 16 * mi + 4 * g + r2] = accb[mi][r2];
 #pragma unroll
-      for (int r2 = 0; r2 < 4; ++r2) {
-        const int i = 16 * mi + 4 * g + r2, k = 16 * nj + j;
-        if (k < nin) P[oW + i * nin + k] = acc[mi][nj][r2];
+    for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+      for (int nj = 0; nj < 7; ++nj) {
+        if (nj >= n_tiles) break;
+#pragma unroll
+        for (int r2 = 0; r2 < 4; ++r2) {
+          const int i = 16 * mi + 4 * g + r2, k = 16 * nj + j;
+          if (k < nin) P[oW + i * nin + k] = acc[mi][nj][r2];
+        }
       }
-    }
-  P[ob + lane] = db;
-  if (w == 3) {
-    P[oW4 + lane] = dw4;
-    if (lane == 0) P[ob4] = db4;
   }
-  // loss: lanes with g == 0 of every wave hold per-point terms
-  float l = loss;
-  for (int off = 32; off > 0; off >>= 1) l += __shfl_xor(l, off);
-  __shared__ float wl[4];
-  if (lane == 0) wl[w] = l;
+  // the output layer: over the 16 points of a lane group, then over the four
+  // waves (the loop ended on a barrier: the staging area is free)
+  float* red = reinterpret_cast<float*>(&S.dzT[0][0][0]);  // [wave][64], then [wave][2]
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    float v = dw4r[q];
+    v += __shfl_xor(v, 1);
+    v += __shfl_xor(v, 2);
+    v += __shfl_xor(v, 4);
+    v += __shfl_xor(v, 8);
+    if (j == 0) red[w * 64 + 16 * (q >> 2) + 4 * g + (q & 3)] = v;
+  }
+  float d4 = db4r, l = loss;
+  for (int off = 32; off > 0; off >>= 1) {
+    d4 += __shfl_xor(d4, off);
+    l += __shfl_xor(l, off);
+  }
+  if (lane == 0) {
+    red[4 * 64 + 2 * w] = d4;
+    red[4 * 64 + 2 * w + 1] = l;
+  }
   __syncthreads();
-  if (threadIdx.x == 0) P[kParams] = wl[0] + wl[1] + wl[2] + wl[3];
+  if (threadIdx.x < 64) {
+    const int n = threadIdx.x;
+    P[oW4 + n] = red[n] + red[64 + n] + red[128 + n] + red[192 + n];
+    if (n == 0) {
+      P[ob4] = red[256] + red[258] + red[260] + red[262];
+      P[kParams] = red[257] + red[259] + red[261] + red[263];
+    }
+  }
 }
 
 // Sum of the blocks' partial gradients; optionally an Adam step
@@ -697,7 +759,7 @@ hipError_t launch_mlp_infer(const void* pack, const float* small, const float* p
 hipError_t launch_mlp_train(const void* pack, const float* small, const float4* samples, int64_t n_samples,
                             int64_t batch_start, int32_t batch, float over_weight, float* partial, int32_t blocks,
                             hipStream_t s) {
-  hipLaunchKernelGGL(mlp_train_kernel, dim3(blocks), dim3(256), sizeof(TrainLds), s, (const bf16x8*)pack, small,
+  hipLaunchKernelGGL(mlp_train_kernel, dim3(blocks), dim3(kTrainBlock), sizeof(TrainLds), s, (const bf16x8*)pack, small,
                      samples, n_samples, batch_start, batch, over_weight, partial);
   return hipGetLastError();
 }
