@@ -618,6 +618,8 @@ __device__ __forceinline__ float wave_surrogate(const bf16x8* spack, const float
 // Surrogate phase: step by (s - margin) while the surrogate distance s
 // exceeds tau, then hand over to the exact phase; after rejoin of exact
 // travel without a crossing, return to the surrogate.
+constexpr int kDeferTile = 16;
+
 template <bool USE_MLP>
 __global__ void __launch_bounds__(512) hybrid_trace_kernel(
     VolView V, float bx1, float by1, float bz1, const bf16x8* __restrict__ gpack, const float* __restrict__ gsmall,
@@ -662,15 +664,33 @@ __global__ void __launch_bounds__(512) hybrid_trace_kernel(
       if (f_prev <= 0.0f) { hit = true; phase = DONE; }
     }
   }
+  // A lane is advanced at most max_iters times (one network evaluation and/or
+  // one exact step per advance).  While some lane can take an exact step, the
+  // network only runs full 16-point tiles of the waiting lanes (which ones is
+  // rotated from iteration to iteration); when none can, it runs for all of
+  // them.  A waiting lane keeps its state, so every ray takes the same steps
+  // as when the network runs for every waiting lane at once, in fewer
+  // part-filled tiles.  Each wave iteration advances at least one lane, which
+  // bounds the loop.
+  int32_t advances = 0;
+  const int64_t wave_iters = 64ll * hp.max_iters;
 #pragma unroll 1
-  for (int it = 0; it < hp.max_iters; ++it) {
+  for (int64_t it = 0; it < wave_iters; ++it) {
+    if (phase != DONE && advances >= hp.max_iters) phase = DONE;
     if (!__any(phase != DONE)) break;
+    bool advanced = phase == EXACT;
     if (USE_MLP) {
-      const bool need = phase == SURR;
-      if (__any(need)) {
+      const bool wait = phase == SURR;
+      const unsigned long long wmask = __ballot(wait);
+      const int n_wait = __popcll(wmask);
+      const int n_run = __any(phase == EXACT) ? (n_wait & ~(kDeferTile - 1)) : n_wait;
+      if (n_run > 0) {
+        const int rank = __popcll(wmask & ((1ull << (threadIdx.x & 63)) - 1ull));
+        const bool need = wait && (rank + (int)((uint32_t)it % (uint32_t)n_wait)) % n_wait < n_run;
         float* ws = slots[threadIdx.x >> 6];
         const float s = wave_surrogate(spack, ssmall, add(o, smul(t, d)), need, ws, ws + 192);
         if (need) {
+          advanced = true;
           ++evals;
           if (s > hp.tau) {
             t += s - hp.margin;
@@ -712,6 +732,7 @@ __global__ void __launch_bounds__(512) hybrid_trace_kernel(
         else if (USE_MLP && travel > hp.rejoin) phase = SURR;
       }
     }
+    if (advanced) ++advances;
   }
   if (i < n) {
     t_out[i] = hit ? t : kInf;
