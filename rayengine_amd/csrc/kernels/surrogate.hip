@@ -489,6 +489,58 @@ __global__ void mlp_update_kernel(float* __restrict__ params, float* __restrict_
   pack_param(p, wv, pack, small);
 }
 
+// The two stages above in one launch: a block covers 64 parameters with four
+// quarters of threads; quarter q sums the kFold-row groups q, q + 4, ... in
+// row order, then one thread per parameter adds the group sums in group order
+// and takes the Adam step.  The additions are the ones the two stages make,
+// in the same order, so the result is the same to the bit.
+constexpr int kReduceParams = 64;
+constexpr int kMaxFoldGroups = 64;
+
+__global__ void __launch_bounds__(256) grad_reduce_update_kernel(
+    float* __restrict__ params, float* __restrict__ m1, float* __restrict__ m2, const float* __restrict__ partial,
+    int32_t n_blocks, int32_t step, float lr, float b1, float b2, float eps, int32_t do_adam,
+    float* __restrict__ grad_out, __bf16* __restrict__ pack, float* __restrict__ small, float* __restrict__ loss_out) {
+  __shared__ float gs[kMaxFoldGroups][kReduceParams];
+  const int pl = threadIdx.x & (kReduceParams - 1), q = threadIdx.x / kReduceParams;
+  const int p = blockIdx.x * kReduceParams + pl;
+  const int n_groups = (n_blocks + kFold - 1) / kFold;
+  if (p <= kParams) {
+    for (int sg = q; sg < n_groups; sg += 256 / kReduceParams) {
+      const int r0 = sg * kFold;
+      const int r1 = r0 + kFold < n_blocks ? r0 + kFold : n_blocks;
+      float v[kFold];
+#pragma unroll
+      for (int k = 0; k < kFold; ++k) v[k] = r0 + k < r1 ? partial[(int64_t)(r0 + k) * (kParams + 1) + p] : 0.0f;
+      float acc = 0.0f;
+#pragma unroll
+      for (int k = 0; k < kFold; ++k)
+        if (r0 + k < r1) acc += v[k];
+      gs[sg][pl] = acc;
+    }
+  }
+  __syncthreads();
+  if (q != 0 || p > kParams) return;
+  float gsum = 0.0f;
+  for (int sg = 0; sg < n_groups; ++sg) gsum += gs[sg][pl];
+  if (p == kParams) {
+    if (loss_out) *loss_out = gsum;
+    return;
+  }
+  if (grad_out) grad_out[p] = gsum;
+  float wv = params[p];
+  if (do_adam) {
+    const float c1 = 1.0f - powf(b1, (float)step), c2 = 1.0f - powf(b2, (float)step);
+    const float mm = b1 * m1[p] + (1.0f - b1) * gsum;
+    const float vv = b2 * m2[p] + (1.0f - b2) * gsum * gsum;
+    m1[p] = mm;
+    m2[p] = vv;
+    wv = wv - lr * (mm / c1) / (sqrtf(vv / c2) + eps);
+    params[p] = wv;
+  }
+  pack_param(p, wv, pack, small);
+}
+
 // ----------------------------------------------------------------- training data
 
 // Squared distance (voxel units) from each voxel to the zero set, seeded at
@@ -790,6 +842,12 @@ hipError_t launch_mlp_update(float* params, float* m1, float* m2, float* partial
                              void* pack, float* small, float* loss_out, hipStream_t s) {
   const int threads = 256;
   const int blocks = (kParams + 1 + threads - 1) / threads;
+  if (partial && n_blocks > 0 && (n_blocks + kFold - 1) / kFold <= kMaxFoldGroups) {
+    hipLaunchKernelGGL(grad_reduce_update_kernel, dim3((kParams + kReduceParams) / kReduceParams), dim3(256), 0, s,
+                       params, m1, m2, partial, n_blocks, step, lr, b1, b2, eps, do_adam, grad_out, (__bf16*)pack,
+                       small, loss_out);
+    return hipGetLastError();
+  }
   if (partial && n_blocks > 0)
     hipLaunchKernelGGL(partial_fold_kernel, dim3(blocks, (n_blocks + kFold - 1) / kFold), dim3(threads), 0, s,
                        partial, n_blocks);
