@@ -247,22 +247,22 @@ def terrain_section(re, W, torch, ctx):
     return out
 
 
-def mixed_workload_section(re, W, torch, ctx, tune, reps=3, n_streams=4):
+def mixed_workload_section(re, W, torch, ctx, reps=3, n_streams=4):
     """The batched mixed workload (workloads.batch_workload, seed 7: 64
     triangle scenes of 1k-500k triangles with 16k-2M incoherent rays each, 16
-    noise-terrain tiles of 64^3-256^3 with their own rays), run unbatched: one
-    closest-hit launch per scene and one exact-field march per tile, in
-    ascending ray count, every input resident in HBM.  Run on one stream, and
-    dealt round-robin over n_streams contexts (one HIP stream each) so that
-    small launches overlap.  Aggregate Mrays/s over the wall time of the whole
+    noise-terrain tiles of 64^3-256^3 with their own rays), run unbatched by
+    rayengine_amd.mixed: one closest-hit launch per scene and one exact-field
+    march per tile, in ascending ray count, every input resident in HBM, on
+    one stream and dealt round-robin over n_streams contexts (one HIP stream
+    each).  Leaf size and block are picked first by autotune.tune_mixed on the
+    n_streams issue.  Aggregate Mrays/s over the wall time of the whole
     workload; an item's completion time is the device time from the start of
-    its stream's first launch to the end of its own (HIP-event durations
-    summed in launch order on that stream)."""
+    its stream's first launch to the end of its own."""
+    from rayengine_amd.autotune import tune_mixed
+    from rayengine_amd.mixed import MixedRun
     t0 = time.perf_counter()
     soups, dims, seeds, ro, rd, sr, tr = W.batch_workload()
-    trees = [re.KDTree(s, max_leaf_tris=tune["max_leaf_tris"], ctx=ctx) for s in soups]
-    items = [("scene", k, b, c) for k, (b, c) in enumerate(sr)] + [("tile", k, b, c) for k, (b, c) in enumerate(tr)]
-    items.sort(key=lambda it: it[3])
+    tiles = list(zip(dims, seeds))
     ctxs = [ctx] + [re.Context(0) for _ in range(n_streams - 1)]
     dev = torch.device("cuda:0")
     n = len(ro)
@@ -270,8 +270,9 @@ def mixed_workload_section(re, W, torch, ctx, tune, reps=3, n_streams=4):
     d_p = torch.empty(n, dtype=torch.int32, device=dev)
     d_t = torch.empty(n, dtype=torch.float32, device=dev)
     d_h = torch.empty(n, dtype=torch.uint8, device=dev)
-    exact = TERRAIN_METHODS["exact_march"][1]
-    tp = {"block": tune["block"], "variant": tune["variant"]}
+    ptrs = (d_o.data_ptr(), d_d.data_ptr(), d_p.data_ptr(), d_t.data_ptr(), d_h.data_ptr())
+    cfg, meas = tune_mixed(ctxs, soups, tiles, sr, tr, ptrs, torch.cuda.synchronize, log=log)
+    trees = [re.KDTree(s, max_leaf_tris=cfg["max_leaf_tris"], ctx=ctx) for s in soups]
     n_scene_rays = int(sum(c for _, c in sr))
     out = {"workload": "64 cluster-soup scenes (1k-500k triangles, log-uniform) with 16k-2M incoherent rays each "
                        "(log-uniform), 16 noise-terrain tiles 64^3-256^3 (seeds 2016+k) with 16k-2M rays each; "
@@ -280,66 +281,42 @@ def mixed_workload_section(re, W, torch, ctx, tune, reps=3, n_streams=4):
                    "is switched off)",
            "triangles": int(sum(len(s) for s in soups)), "rays": int(n), "scene_rays": n_scene_rays,
            "tile_rays": int(n - n_scene_rays), "tiles_traced_with": "exact-field march (k 0.05, h_min 1 voxel)",
-           "launches": len(items)}
+           "launches": len(sr) + len(tr),
+           "autotune": {"chosen": {"max_leaf_tris": cfg["max_leaf_tris"], "block": cfg["block"]},
+                        "issue": f"{n_streams} streams",
+                        "wall_ms": {f"leaf{m['max_leaf_tris']}_block{m['block']}": m["wall_ms"] for m in meas}}}
+    first = None
     for streams in (1, n_streams):
-        owner = [i % streams for i in range(len(items))]
-        res_objs = []
-        for i, (kind, k, b, c) in enumerate(items):
-            cx = ctxs[owner[i]]
-            if kind == "scene":
-                res_objs.append(cx.upload_scene([re.SceneObject(trees[k])]))
-            else:
-                dm = dims[k]
-                res_objs.append(cx.noise_volume(dim=(dm, dm, dm), spacing=(1.0 / dm,) * 3, seed=seeds[k]))
-
-        def run():
-            for i, (kind, k, b, c) in enumerate(items):
-                o_ptr, d_ptr = d_o.data_ptr() + 12 * b, d_d.data_ptr() + 12 * b
-                if kind == "scene":
-                    res_objs[i].trace_closest_device(o_ptr, d_ptr, c, d_p.data_ptr() + 4 * b,
-                                                     d_t.data_ptr() + 4 * b, sync=False, **tp)
-                else:
-                    re.hybrid_trace_device(res_objs[i], None, o_ptr, d_ptr, c, d_t.data_ptr() + 4 * b,
-                                           d_h.data_ptr() + b, sync=False, **exact)
-
-        run()
+        run = MixedRun(ctxs[:streams], trees, tiles, sr, tr, *ptrs)
+        run.run(block=cfg["block"], variant=cfg["variant"])
         torch.cuda.synchronize()
         walls, comps, ksums = [], [], []
         for _ in range(reps):
             torch.cuda.synchronize()
             ts = time.perf_counter()
-            run()
+            run.run(block=cfg["block"], variant=cfg["variant"])
             torch.cuda.synchronize()
             walls.append(time.perf_counter() - ts)
-            done = np.zeros(len(items))
-            ksum = 0.0
-            for sidx in range(streams):
-                mine = [i for i in range(len(items)) if owner[i] == sidx]
-                kt = ctxs[sidx].kernel_times(len(mine))
-                done[mine] = np.cumsum(kt)
-                ksum += float(np.sum(kt))
-            comps.append(done)
-            ksums.append(ksum)
+            comps.append(run.completion_ms())
+            ksums.append(run.kernel_ms_sum())
         wall = float(np.median(walls))
         done = np.mean(comps, axis=0)
-        sd = np.array([done[i] for i, it in enumerate(items) if it[0] == "scene"])
-        td = np.array([done[i] for i, it in enumerate(items) if it[0] == "tile"])
-        out[f"streams_{streams}"] = {
-            "wall_ms": round(wall * 1e3, 3), "mrays_s": round(n / wall / 1e6, 1),
-            "kernel_ms_sum": round(float(np.mean(ksums)), 3),
-            "scene_completion_ms": {"p50": round(float(np.percentile(sd, 50)), 3),
-                                    "p99": round(float(np.percentile(sd, 99)), 3)},
-            "tile_completion_ms": {"p50": round(float(np.percentile(td, 50)), 3),
-                                   "p99": round(float(np.percentile(td, 99)), 3)}}
+        sd = np.array([done[i] for i, it in enumerate(run.items) if it[0] == "scene"])
+        td = np.array([done[i] for i, it in enumerate(run.items) if it[0] == "tile"])
+        rec = {"wall_ms": round(wall * 1e3, 3), "mrays_s": round(n / wall / 1e6, 1),
+               "kernel_ms_sum": round(float(np.mean(ksums)), 3),
+               "scene_completion_ms": {"p50": round(float(np.percentile(sd, 50)), 3),
+                                       "p99": round(float(np.percentile(sd, 99)), 3)},
+               "tile_completion_ms": {"p50": round(float(np.percentile(td, 50)), 3),
+                                      "p99": round(float(np.percentile(td, 99)), 3)}}
         got = (d_p.cpu().numpy(), d_t.cpu().numpy().view(np.uint32), d_h.cpu().numpy())
-        if streams == 1:
+        if first is None:
             first = got
         else:
-            out[f"streams_{streams}"]["identical_to_one_stream"] = all(
-                np.array_equal(x, y) for x, y in zip(first, got))
-        for r in res_objs:
-            r.release()
-    out["prep_s"] = round(time.perf_counter() - t0, 1)
+            rec["identical_to_one_stream"] = all(np.array_equal(x, y) for x, y in zip(first, got))
+        out[f"streams_{streams}"] = rec
+        run.release()
+    out["prep_and_tune_s"] = round(time.perf_counter() - t0, 1)
     return out
 
 
@@ -528,7 +505,7 @@ def main():
     res["shadow"] = shadow_section(W, torch, ctx, scene, tri_v, po, pd, pp, pt, order)
     if not args.no_mixed:
         try:
-            res["mixed_workload"] = mixed_workload_section(re, W, torch, ctx, tune)
+            res["mixed_workload"] = mixed_workload_section(re, W, torch, ctx)
         except Exception as e:  # the closest-hit line is still reported
             res["mixed_workload"] = {"error": f"{type(e).__name__}: {e}"}
     if not args.no_terrain:

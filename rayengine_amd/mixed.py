@@ -1,0 +1,83 @@
+"""Many triangle scenes and noise-terrain tiles, each with its own slice of
+one ray array, traced one launch per item: a closest-hit launch per scene
+(re_trace_closest) and an exact-field march per tile (re_hybrid_trace without
+a network).  Items are issued in ascending ray count and dealt round-robin over
+the given contexts, one HIP stream each, so that small launches overlap.
+Inputs and outputs are device buffers given as raw pointers.
+
+This is the unbatched way to run the many-scene workload whose single
+persistent work-queue launch (re_batch_*) is switched off in this build.  No
+counterpart in the reference, which traced one scene per frame
+(RayEngine/src/cuda-src/renderer.cu:701-738).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import Context, KDTree, SceneObject, hybrid_trace_device
+
+# exact-field march of the tiles: step max(0.05 f, 1 voxel), one bisection
+TILE_MARCH = dict(k_exact=0.05, h_min_voxels=1.0, n_bisect=1)
+
+
+class MixedRun:
+    """trees: one KDTree per scene; tiles: (dim, seed) per noise tile (a dim^3
+    volume over the unit cube); scene_ranges / tile_ranges: (first ray, ray
+    count) of each item in the ray arrays.  Results: prim id and t per
+    triangle ray, t and a 0/1 hit byte per terrain ray, written at the ray's
+    index into prim_ptr (int32), t_ptr (float32) and hit_ptr (uint8)."""
+
+    def __init__(self, ctxs: list[Context], trees: list[KDTree], tiles, scene_ranges, tile_ranges, o_ptr: int,
+                 d_ptr: int, prim_ptr: int, t_ptr: int, hit_ptr: int, tile_march: dict | None = None):
+        self.ctxs = list(ctxs)
+        items = [("scene", k, int(b), int(c)) for k, (b, c) in enumerate(scene_ranges)]
+        items += [("tile", k, int(b), int(c)) for k, (b, c) in enumerate(tile_ranges)]
+        items.sort(key=lambda it: it[3])
+        self.items = items
+        self.owner = [i % len(self.ctxs) for i in range(len(items))]
+        self.ptrs = (o_ptr, d_ptr, prim_ptr, t_ptr, hit_ptr)
+        self.tile_march = dict(TILE_MARCH if tile_march is None else tile_march)
+        self.objs = []
+        try:
+            for i, (kind, k, _, _) in enumerate(items):
+                cx = self.ctxs[self.owner[i]]
+                if kind == "scene":
+                    self.objs.append(cx.upload_scene([SceneObject(trees[k])]))
+                else:
+                    dm, seed = tiles[k]
+                    self.objs.append(cx.noise_volume(dim=(dm, dm, dm), spacing=(1.0 / dm,) * 3, seed=seed))
+        except Exception:
+            self.release()
+            raise
+
+    def run(self, block: int = 64, variant: int = 3):
+        """Issues every item's launch (asynchronous on its context's stream)."""
+        o_ptr, d_ptr, p_ptr, t_ptr, h_ptr = self.ptrs
+        for (kind, _, b, c), obj in zip(self.items, self.objs):
+            if kind == "scene":
+                obj.trace_closest_device(o_ptr + 12 * b, d_ptr + 12 * b, c, p_ptr + 4 * b, t_ptr + 4 * b, sync=False,
+                                         block=block, variant=variant)
+            else:
+                hybrid_trace_device(obj, None, o_ptr + 12 * b, d_ptr + 12 * b, c, t_ptr + 4 * b, h_ptr + b,
+                                    sync=False, **self.tile_march)
+
+    def completion_ms(self) -> np.ndarray:
+        """After a run has finished: per item (in self.items order) the device
+        time from the start of its stream's first launch to the end of its own
+        launch, from the HIP-event durations of each stream's launches."""
+        done = np.zeros(len(self.items))
+        for s, cx in enumerate(self.ctxs):
+            mine = [i for i in range(len(self.items)) if self.owner[i] == s]
+            if mine:
+                done[mine] = np.cumsum(cx.kernel_times(len(mine)))
+        return done
+
+    def kernel_ms_sum(self) -> float:
+        """After a run has finished: the summed kernel durations of all items."""
+        return float(sum(np.sum(cx.kernel_times(sum(1 for o in self.owner if o == s)))
+                         for s, cx in enumerate(self.ctxs)))
+
+    def release(self):
+        for o in self.objs:
+            o.release()
+        self.objs = []

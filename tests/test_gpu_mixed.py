@@ -1,0 +1,81 @@
+"""Many scenes and terrain tiles traced one launch per item over two streams
+(rayengine_amd.mixed): every triangle ray's prim id and t equal the CPU
+oracle's walk bit for bit, every terrain ray equals the single-call
+exact-field march, and the completion times are per-stream running sums."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+import rayengine_amd as re
+from rayengine_amd import workloads as W
+from rayengine_amd.mixed import MixedRun, TILE_MARCH
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def test_mixed_run_matches_oracle_and_single_calls(gpu_ctx):
+    import torch
+    soups, dims, seeds, ro, rd, sr, tr = W.batch_workload(n_scenes=5, n_tiles=2, tri_lo=500, tri_hi=5000,
+                                                           ray_lo=500, ray_hi=5000, tile_lo=32, tile_hi=48, seed=11)
+    tiles = list(zip(dims, seeds))
+    say(f"mixed: {len(soups)} scenes, tiles {dims}, {len(ro)} rays")
+    trees = [re.KDTree(s) for s in soups]
+    ctxs = [gpu_ctx, re.Context(0)]
+    dev = torch.device("cuda:0")
+    n = len(ro)
+    d_o, d_d = torch.from_numpy(ro).to(dev), torch.from_numpy(rd).to(dev)
+    d_p = torch.full((n,), -7, dtype=torch.int32, device=dev)
+    d_t = torch.zeros(n, dtype=torch.float32, device=dev)
+    d_h = torch.full((n,), 7, dtype=torch.uint8, device=dev)
+    run = MixedRun(ctxs, trees, tiles, sr, tr, d_o.data_ptr(), d_d.data_ptr(), d_p.data_ptr(), d_t.data_ptr(),
+                   d_h.data_ptr())
+    try:
+        for rep in range(2):
+            run.run(block=64, variant=3)
+            torch.cuda.synchronize()
+            p, t, h = d_p.cpu().numpy(), d_t.cpu().numpy(), d_h.cpu().numpy()
+            for tree, (b, c) in zip(trees, sr):
+                po, to = oracle.trace(*tree.export(), tree.identity_object(), ro[b:b + c], rd[b:b + c])
+                assert np.array_equal(p[b:b + c], po), f"run {rep}: prim ids of rays [{b}, {b + c})"
+                assert np.array_equal(t[b:b + c].view(np.uint32), to.view(np.uint32)), f"run {rep}: t"
+            for (dm, sd), (b, c) in zip(tiles, tr):
+                v = gpu_ctx.noise_volume(dim=(dm, dm, dm), spacing=(1.0 / dm,) * 3, seed=sd)
+                ts, hs, _, _ = re.hybrid_trace(v, None, ro[b:b + c], rd[b:b + c], **TILE_MARCH)
+                v.release()
+                assert np.array_equal(h[b:b + c], hs), f"run {rep}: tile hits of rays [{b}, {b + c})"
+                assert np.array_equal(t[b:b + c].view(np.uint32), ts.view(np.uint32)), f"run {rep}: tile t"
+            done = run.completion_ms()
+            assert done.shape == (len(run.items),) and (done > 0).all()
+            for s in range(len(ctxs)):
+                mine = done[[i for i in range(len(run.items)) if run.owner[i] == s]]
+                assert (np.diff(mine) > 0).all(), "completion times grow along a stream"
+            # the summed kernel time is the sum of the per-stream totals
+            per_stream = [done[[i for i in range(len(run.items)) if run.owner[i] == s]][-1] for s in range(2)]
+            assert run.kernel_ms_sum() == pytest.approx(sum(per_stream), rel=1e-4)
+            say(f"run {rep}: identical; completion {done.max():.3f} ms")
+            d_p.fill_(-7)
+            d_h.fill_(7)
+    finally:
+        run.release()
+
+
+def test_mixed_run_items_in_ascending_ray_count(gpu_ctx):
+    soups, dims, seeds, ro, rd, sr, tr = W.batch_workload(n_scenes=3, n_tiles=2, tri_lo=200, tri_hi=400,
+                                                           ray_lo=100, ray_hi=2000, tile_lo=16, tile_hi=16, seed=5)
+    trees = [re.KDTree(s) for s in soups]
+    run = MixedRun([gpu_ctx], trees, list(zip(dims, seeds)), sr, tr, 0, 0, 0, 0, 0)
+    try:
+        counts = [it[3] for it in run.items]
+        assert counts == sorted(counts)
+        assert sorted((it[0], it[1]) for it in run.items) == sorted(
+            [("scene", k) for k in range(3)] + [("tile", k) for k in range(2)])
+        assert run.owner == [0] * 5
+    finally:
+        run.release()
