@@ -6,8 +6,10 @@
 // material_functions.cuh:295-344), after scratchapixel's "simulating the
 // colors of the sky", with its constants and its final expression as
 // written there (sumR betaR phaseR + sumM + 100 betaM phaseM, times 20).
-// The reference evaluates exponentials with the fast __expf; this kernel
-// uses expf.
+// Like the reference, built with fast math, the sky's inner loops use the
+// hardware exponential, reciprocal and square root (a few ulp each) instead
+// of the correctly rounded library sequences; hit pixels use exact
+// arithmetic.
 #include "../internal.hpp"
 
 namespace re {
@@ -19,8 +21,13 @@ struct V3 {
 __device__ __forceinline__ V3 operator+(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
 __device__ __forceinline__ V3 operator*(float s, V3 a) { return {s * a.x, s * a.y, s * a.z}; }
 __device__ __forceinline__ float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-__device__ __forceinline__ float len(V3 a) { return sqrtf(dot(a, a)); }
 __device__ __forceinline__ float saturate(float x) { return x > 0.0f ? (x < 1.0f ? x : 1.0f) : 0.0f; }
+
+// Hardware forms: e^x as 2^(x log2 e), 1/x, sqrt.
+__device__ __forceinline__ float hexp(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504088896341f); }
+__device__ __forceinline__ float hrcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float hsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float hlen(V3 a) { return hsqrt(dot(a, a)); }
 
 // solve_quadratic / gpu_ray_sphere_intersect (ray_functions.cuh:95-126).
 __device__ bool ray_sphere(V3 o, V3 d, float radius, float& t0, float& t1) {
@@ -30,11 +37,11 @@ __device__ bool ray_sphere(V3 o, V3 d, float radius, float& t0, float& t1) {
   const float discr = b * b - 4.f * a * c;
   if (discr < 0.0f) return false;
   if (discr == 0.0f) {
-    t0 = t1 = -0.5f * b / a;
+    t0 = t1 = -0.5f * b * hrcp(a);
   } else {
-    const float q = (b > 0.0f) ? -0.5f * (b + sqrtf(discr)) : -0.5f * (b - sqrtf(discr));
-    t0 = q / a;
-    t1 = c / q;
+    const float q = (b > 0.0f) ? -0.5f * (b + hsqrt(discr)) : -0.5f * (b - hsqrt(discr));
+    t0 = q * hrcp(a);
+    t1 = c * hrcp(q);
   }
   if (t0 > t1) {
     const float t = t0;
@@ -54,6 +61,7 @@ __device__ V3 incident_light(const FrameLaunch& L, V3 orig, V3 dir) {
   if (t0 > tmin && t0 > 0) tmin = t0;
   if (t1 < tmax) tmax = t1;
   const int num_samples = 32, num_samples_light = 8;
+  const float inv_hr = 1.0f / L.hr, inv_hm = 1.0f / L.hm;
   const float seg = (tmax - tmin) / (float)num_samples;
   float t_cur = tmin;
   V3 sum_r{0, 0, 0}, sum_m{0, 0, 0};
@@ -66,9 +74,9 @@ __device__ V3 incident_light(const FrameLaunch& L, V3 orig, V3 dir) {
                                 (double)((2.f + g * g) * powf(1.f + g * g - 2.f * g * mu, 1.5f)));
   for (int i = 0; i < num_samples; ++i) {
     const V3 sp = orig + (t_cur + seg * 0.5f) * dir;
-    const float height = len(sp) - L.er;
-    const float hr = expf(-height / L.hr) * seg;
-    const float hm = expf(-height / L.hm) * seg;
+    const float height = hlen(sp) - L.er;
+    const float hr = hexp(-height * inv_hr) * seg;
+    const float hm = hexp(-height * inv_hm) * seg;
     od_r += hr;
     od_m += hm;
     float t0l = 0.0f, t1l = 0.0f;
@@ -79,16 +87,16 @@ __device__ V3 incident_light(const FrameLaunch& L, V3 orig, V3 dir) {
     int j;
     for (j = 0; j < num_samples_light; ++j) {
       const V3 spl = sp + (t_cur_l + seg_l * 0.5f) * sun;
-      const float hl = len(spl) - L.er;
+      const float hl = hlen(spl) - L.er;
       if (hl < 0) break;
-      od_lr += expf(-hl / L.hr) * seg_l;
-      od_lm += expf(-hl / L.hm) * seg_l;
+      od_lr += hexp(-hl * inv_hr) * seg_l;
+      od_lm += hexp(-hl * inv_hm) * seg_l;
       t_cur_l += seg_l;
     }
     if (j == num_samples_light) {
       const float m = betaM * 1.1f * (od_m + od_lm);
       const V3 tau{betaR.x * (od_r + od_lr) + m, betaR.y * (od_r + od_lr) + m, betaR.z * (od_r + od_lr) + m};
-      const V3 att{expf(-tau.x), expf(-tau.y), expf(-tau.z)};
+      const V3 att{hexp(-tau.x), hexp(-tau.y), hexp(-tau.z)};
       sum_r = sum_r + V3{att.x * hr, att.y * hr, att.z * hr};
       sum_m = sum_m + V3{att.x * hm, att.y * hm, att.z * hm};
     }

@@ -32,7 +32,8 @@ def test_terrain_frame_matches_oracle(gpu_ctx, opts):
     hits = hit.astype(bool)
     assert np.array_equal(a[hits], b[hits])                       # hit pixels: exact arithmetic, identical
     diff = np.abs(a - b).max(1)
-    assert diff.max() <= 1 and (diff > 0).mean() < 0.01           # sky: libm vs device exp/pow, one step at most
+    say(f"sky: largest difference {diff.max()} step(s), {(diff > 0).mean():.5f} of the pixels differ")
+    assert diff.max() <= 1 and (diff > 0).mean() < 0.01           # sky: libm vs hardware exp/rcp/sqrt, one step
     assert (a[:, 3] == 255).all() and 0 < hits.mean() < 1
     vol.release()
 
