@@ -396,7 +396,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              f"(rays shard across GPUs with no exchange, see DESIGN.md)", file=sys.stderr)
+              f"(rays would shard across GPUs with no exchange between them)", file=sys.stderr)
         sys.exit(2)
 
     import torch
