@@ -223,7 +223,7 @@ RE_API re_status re_batch_create(re_ctx* ctx, const re_batch_scene* scenes, int3
     hp.margin = cfg.hybrid.margin_voxels * tv;
     hp.h_min = cfg.hybrid.h_min_voxels * tv;
     hp.rejoin = cfg.hybrid.rejoin_voxels * tv;
-    e = launch_hybrid_trace(v, m.pack, m.small, b.ray_o + 3 * st.begin, b.ray_d + 3 * st.begin, st.count, hp, true,
+    e = launch_hybrid_trace(v, infer_pack(m), infer_small(m), b.ray_o + 3 * st.begin, b.ray_d + 3 * st.begin, st.count, hp, true,
                             b.t + st.begin, b.hit_scratch + st.begin, nullptr, nullptr, ctx->stream);
     if (e == hipSuccess) e = launch_hit_to_prim(b.hit_scratch + st.begin, b.prim + st.begin, st.count, ctx->stream);
     if (e == hipSuccess) e = launch_batch_stamp(b.done_ts, st.slot, b.remaining, ctx->stream);

@@ -62,8 +62,8 @@ RE_API re_status re_mlp_create(re_ctx* ctx, uint64_t seed, int32_t* mlp_id) {
   if (e == hipSuccess) e = hipMalloc((void**)&m.params, pb);
   if (e == hipSuccess) e = hipMalloc((void**)&m.m1, pb);
   if (e == hipSuccess) e = hipMalloc((void**)&m.m2, pb);
-  if (e == hipSuccess) e = hipMalloc(&m.pack, (size_t)kPack * 2);
-  if (e == hipSuccess) e = hipMalloc((void**)&m.small, (size_t)kSmall * 4);
+  if (e == hipSuccess) e = hipMalloc(&m.pack, (size_t)kPackAlloc * 2);
+  if (e == hipSuccess) e = hipMalloc((void**)&m.small, (size_t)kSmallAlloc * 4);
   if (e == hipSuccess) e = hipMalloc((void**)&m.partial, (size_t)m.partial_blocks * (kParams + 1) * 4);
   if (e == hipSuccess) e = hipMalloc((void**)&m.grad, pb);
   if (e == hipSuccess) e = hipMalloc((void**)&m.loss, 4 * 1024);
@@ -85,11 +85,11 @@ RE_API re_status re_mlp_create(re_ctx* ctx, uint64_t seed, int32_t* mlp_id) {
   fill(oW3, kH, kH, 1.0f);
   fill(oW4, 1, kH, 0.1f);
   std::vector<float> zeros(kParams, 0.0f);
-  std::vector<uint16_t> zpack(kPack, 0);
+  std::vector<uint16_t> zpack(kPackAlloc, 0);
   e = hipMemcpyAsync(m.params, P.data(), pb, hipMemcpyHostToDevice, ctx->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(m.m1, zeros.data(), pb, hipMemcpyHostToDevice, ctx->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(m.m2, zeros.data(), pb, hipMemcpyHostToDevice, ctx->stream);
-  if (e == hipSuccess) e = hipMemcpyAsync(m.pack, zpack.data(), (size_t)kPack * 2, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(m.pack, zpack.data(), (size_t)kPackAlloc * 2, hipMemcpyHostToDevice, ctx->stream);
   if (e != hipSuccess) {
     free_mlp(m);
     return hip_err(ctx, e, "re_mlp_create upload");
@@ -168,7 +168,7 @@ RE_API re_status re_mlp_forward(re_ctx* ctx, int32_t mlp_id, const float* pts, i
     d_pts = sp;
   }
   HIP_TRY(ctx, ctx->launch_begin(), "event");
-  HIP_TRY(ctx, launch_mlp_infer(m->pack, m->small, d_pts, n, d_out, ctx->stream), "mlp forward");
+  HIP_TRY(ctx, launch_mlp_infer(infer_pack(*m), infer_small(*m), d_pts, n, d_out, ctx->stream), "mlp forward");
   HIP_TRY(ctx, ctx->launch_end(), "event");
   if (!dev) HIP_TRY(ctx, hipMemcpyAsync(out, d_out, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream), "copy out");
   if (!(flags & RE_ASYNC) || !dev) HIP_TRY(ctx, hipStreamSynchronize(ctx->stream), "mlp forward sync");
@@ -311,7 +311,7 @@ RE_API re_status re_hybrid_trace(re_ctx* ctx, int32_t vol_id, int32_t mlp_id, co
     d_d = sd;
   }
   HIP_TRY(ctx, ctx->launch_begin(), "event");
-  HIP_TRY(ctx, launch_hybrid_trace(v, m ? m->pack : nullptr, m ? m->small : nullptr, d_o, d_d, n, hp, m != nullptr,
+  HIP_TRY(ctx, launch_hybrid_trace(v, m ? infer_pack(*m) : nullptr, m ? infer_small(*m) : nullptr, d_o, d_d, n, hp, m != nullptr,
                                    d_t, d_h, d_e, d_s, ctx->stream), "hybrid trace");
   HIP_TRY(ctx, ctx->launch_end(), "event");
   if (!dev) {

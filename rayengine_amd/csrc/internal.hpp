@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "../../include/rayengine.h"
+#include "mlp_layout.hpp"
 #include "re_math.hpp"
 #include "terrain_math.hpp"
 
@@ -151,8 +152,8 @@ struct DevMlp {
   float* params = nullptr;   // FP32 master parameters (flat layout, mlp_layout.hpp)
   float* m1 = nullptr;       // Adam first moment
   float* m2 = nullptr;       // Adam second moment
-  void* pack = nullptr;      // bf16 MFMA operands (64 KB)
-  float* small = nullptr;    // FP32 b0..b3, W4, b4
+  void* pack = nullptr;      // bf16 MFMA operands: training (64 KB), then inference (40 KB)
+  float* small = nullptr;    // FP32 b0..b3, W4, b4: training, then inference
   float* partial = nullptr;  // per-block gradient partials (+ loss)
   float* grad = nullptr;     // summed gradient of the last batch
   float* loss = nullptr;     // per-step loss history
@@ -162,6 +163,11 @@ struct DevMlp {
 };
 
 size_t mlp_train_lds_bytes();
+
+// The inference operands (mlp_layout.hpp: pre-activations carried as
+// -log2(e) z) stored after the training ones in the same buffers.
+inline const void* infer_pack(const DevMlp& m) { return (const uint16_t*)m.pack + mlp::oPackInfer; }
+inline const float* infer_small(const DevMlp& m) { return m.small + mlp::oSmallInfer; }
 
 struct HybridParams {
   float tau;      // surrogate distance below which the exact field takes over

@@ -37,6 +37,9 @@ __device__ __forceinline__ float sigmoid_fast(float z) {
   return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(z * -1.44269504089f));
 }
 __device__ __forceinline__ float silu(float z) { return z * sigmoid_fast(z); }
+// The same unit on the scaled pre-activation u = -log2(e) z of the inference
+// operands: u / (1 + 2^u) = -log2(e) SiLU(z).
+__device__ __forceinline__ float silu_scaled(float u) { return u * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(u)); }
 
 // Its derivative: s (1 + z (1 - s)) with s = sigmoid(z).
 __device__ __forceinline__ float dsilu(float z) {
@@ -100,7 +103,8 @@ struct TileAct {
   float h3[4][4];   // FP32 h3 for the output layer
 };
 
-template <bool KEEP>
+// SCALED: the inference operands (pre-activations carried as -log2(e) z).
+template <bool KEEP, bool SCALED = false>
 __device__ __forceinline__ float forward_tile(const bf16x8* __restrict__ packF, const float* __restrict__ small,
                                               const bf16x8 xb[4], int lane, TileAct& T) {
   const int g = lane >> 4;
@@ -120,7 +124,7 @@ __device__ __forceinline__ float forward_tile(const bf16x8* __restrict__ packF, 
       for (int r = 0; r < 4; ++r) {
         const float z = acc[m][r];
         if (KEEP) T.z[0][m][r] = z;
-        T.hb[0][m >> 1][(m & 1) * 4 + r] = (__bf16)silu(z);
+        T.hb[0][m >> 1][(m & 1) * 4 + r] = (__bf16)(SCALED ? silu_scaled(z) : silu(z));
       }
   }
   // layers 1..3: 2 K-steps of 32 neurons, chained in registers
@@ -142,7 +146,7 @@ __device__ __forceinline__ float forward_tile(const bf16x8* __restrict__ packF, 
       for (int r = 0; r < 4; ++r) {
         const float z = acc[m][r];
         if (KEEP) T.z[L][m][r] = z;
-        const float h = silu(z);
+        const float h = SCALED ? silu_scaled(z) : silu(z);
         T.hb[L][m >> 1][(m & 1) * 4 + r] = (__bf16)h;
         if (L == 3) T.h3[m][r] = h;
       }
@@ -188,7 +192,7 @@ __global__ void __launch_bounds__(512) mlp_infer_kernel(const bf16x8* __restrict
     bf16x8 xb[4];
     features(p, lane >> 4, xb);
     TileAct T;
-    const float o = forward_tile<false>(spack, ssmall, xb, lane, T);
+    const float o = forward_tile<false, true>(spack, ssmall, xb, lane, T);
     if (lane < 16 && i < n) out[i] = o;
   }
 }
@@ -425,12 +429,14 @@ __device__ __forceinline__ void pack_param(int p, float wv, __bf16* pack, float*
     const int i = p / kIn, k = p % kIn;
     const int m = i >> 4, c = k >> 5, gg = (k & 31) >> 3, s = k & 7;
     pack[oPackF0 + pack_index(m, c, 4, 16 * gg + (i & 15), s)] = (__bf16)wv;
+    pack[oPackInfer + oPackF0 + pack_index(m, c, 4, 16 * gg + (i & 15), s)] = (__bf16)(wv * kNegLog2e);
     return;
   }
   const int oWs[3] = {oW1, oW2, oW3}, obs[4] = {ob0, ob1, ob2, ob3};
   for (int L = 0; L < 4; ++L)
     if (p >= obs[L] && p < obs[L] + kH) {
       small[64 * L + (p - obs[L])] = wv;
+      small[oSmallInfer + 64 * L + (p - obs[L])] = wv * kNegLog2e;
       return;
     }
   for (int L = 1; L < 4; ++L)
@@ -439,12 +445,18 @@ __device__ __forceinline__ void pack_param(int p, float wv, __bf16* pack, float*
       int gg, s;
       kperm_inv(k & 31, gg, s);
       pack[oPackF1 + (L - 1) * kPackL + pack_index(i >> 4, k >> 5, 2, 16 * gg + (i & 15), s)] = (__bf16)wv;
+      pack[oPackInfer + oPackF1 + (L - 1) * kPackL + pack_index(i >> 4, k >> 5, 2, 16 * gg + (i & 15), s)] = (__bf16)wv;
       kperm_inv(i & 31, gg, s);
       pack[oPackB1 + (L - 1) * kPackL + pack_index(k >> 4, i >> 5, 2, 16 * gg + (k & 15), s)] = (__bf16)wv;
       return;
     }
-  if (p >= oW4 && p < oW4 + kH) small[256 + (p - oW4)] = wv;
-  else if (p == ob4) small[320] = wv;
+  if (p >= oW4 && p < oW4 + kH) {
+    small[256 + (p - oW4)] = wv;
+    small[oSmallInfer + 256 + (p - oW4)] = wv * kNegLn2;
+  } else if (p == ob4) {
+    small[320] = wv;
+    small[oSmallInfer + 320] = wv;
+  }
 }
 
 // First stage of the sum over blocks: row kFold * s of the partial-gradient
@@ -655,7 +667,7 @@ __device__ __forceinline__ float wave_surrogate(const bf16x8* spack, const float
     bf16x8 xb[4];
     features(pq, lane >> 4, xb);
     TileAct T;
-    const float o = forward_tile<false>(spack, ssmall, xb, lane, T);
+    const float o = forward_tile<false, true>(spack, ssmall, xb, lane, T);
     if (lane < 16 && idx < count) wout[idx] = o;
   }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");

@@ -63,6 +63,20 @@ constexpr int kPack = oPackB3 + kPackL;  // 32768 elements, 64 KB
 // FP32 small parameters kept beside the pack: b0..b3, W4, b4.
 constexpr int kSmall = 4 * kH + kH + 1;  // 321
 
+// Inference copy of the forward operands, after the training pack (and of
+// the small parameters, after the training ones), with the pre-activations
+// carried as u = -log2(e) z: layer-0 weights and every bias are stored times
+// -log2(e), the weights of layers 1..3 as they are, and W4 times -ln 2.  A
+// hidden unit is then u / (1 + 2^u) = -log2(e) SiLU(z), one multiply fewer
+// than SiLU(z) itself, and the next layer's pre-activation comes out as u
+// again because log2(e) ln 2 = 1.
+constexpr int oPackInfer = kPack;                // elements
+constexpr int kPackAlloc = kPack + (kPackL0 + 3 * kPackL);
+constexpr int oSmallInfer = kSmall;
+constexpr int kSmallAlloc = 2 * kSmall;
+constexpr float kNegLog2e = -1.44269504088896341f;
+constexpr float kNegLn2 = -0.693147180559945309f;
+
 // Neuron (within a 32-wide K-step) held in hardware K slot 8 g + s.
 RE_HD int kperm(int g, int s) { return s < 4 ? 4 * g + s : 16 + 4 * g + (s - 4); }
 // Inverse: neuron q (0..31) of a K-step -> (g, s).
