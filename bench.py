@@ -251,8 +251,9 @@ def mixed_workload_section(re, W, torch, ctx, reps=3, n_streams=4):
     """The batched mixed workload (workloads.batch_workload, seed 7: 64
     triangle scenes of 1k-500k triangles with 16k-2M incoherent rays each, 16
     noise-terrain tiles of 64^3-256^3 with their own rays), run unbatched by
-    rayengine_amd.mixed: one closest-hit launch per scene and one exact-field
-    march per tile, in ascending ray count, every input resident in HBM, on
+    rayengine_amd.mixed: one closest-hit launch per scene and, per tile, the
+    exact-field march (even tiles) or the surrogate hybrid with a network
+    trained on that tile (odd tiles), in ascending ray count, every input resident in HBM, on
     one stream and dealt round-robin over n_streams contexts (one HIP stream
     each).  Leaf size and block are picked first by autotune.tune_mixed on the
     n_streams issue.  Aggregate Mrays/s over the wall time of the whole
@@ -262,7 +263,23 @@ def mixed_workload_section(re, W, torch, ctx, reps=3, n_streams=4):
     from rayengine_amd.mixed import MixedRun
     t0 = time.perf_counter()
     soups, dims, seeds, ro, rd, sr, tr = W.batch_workload()
-    tiles = list(zip(dims, seeds))
+    # every other tile is traced by the surrogate hybrid with a network of its
+    # own (2,000 Adam steps on 2^20 samples of that tile's distance field)
+    t_net = time.perf_counter()
+    tiles = []
+    for k, (dm, sd) in enumerate(zip(dims, seeds)):
+        if k % 2 == 1:
+            v = ctx.noise_volume(dim=(dm, dm, dm), spacing=(1.0 / dm,) * 3, seed=sd)
+            dv = re.distance_volume(v)
+            net = re.Surrogate(ctx, seed=1)
+            net.train(dv, n_samples=1 << 20, batch=65536, steps=2000, lr=1e-3, seed=1, over_weight=4.0)
+            tiles.append((dm, sd, net.params()[0]))
+            net.release()
+            dv.release()
+            v.release()
+        else:
+            tiles.append((dm, sd))
+    net_s = time.perf_counter() - t_net
     ctxs = [ctx] + [re.Context(0) for _ in range(n_streams - 1)]
     dev = torch.device("cuda:0")
     n = len(ro)
@@ -280,7 +297,11 @@ def mixed_workload_section(re, W, torch, ctx, reps=3, n_streams=4):
            "mode": "unbatched: one launch per scene or tile, ascending ray count (the batched persistent launch "
                    "is switched off)",
            "triangles": int(sum(len(s) for s in soups)), "rays": int(n), "scene_rays": n_scene_rays,
-           "tile_rays": int(n - n_scene_rays), "tiles_traced_with": "exact-field march (k 0.05, h_min 1 voxel)",
+           "tile_rays": int(n - n_scene_rays),
+           "tiles_traced_with": "even tiles: exact-field march (k 0.05, h_min 1 voxel); odd tiles: surrogate hybrid "
+                                "(tau 4 voxels, margin 1, k 0.1, 4 bisections) with a network per tile, 2,000 Adam "
+                                "steps on 2^20 samples of its field",
+           "tile_networks_train_s": round(net_s, 2),
            "launches": len(sr) + len(tr),
            "autotune": {"chosen": {"max_leaf_tris": cfg["max_leaf_tris"], "block": cfg["block"]},
                         "issue": f"{n_streams} streams",

@@ -1,7 +1,8 @@
 """Many scenes and terrain tiles traced one launch per item over two streams
 (rayengine_amd.mixed): every triangle ray's prim id and t equal the CPU
 oracle's walk bit for bit, every terrain ray equals the single-call
-exact-field march, and the completion times are per-stream running sums."""
+exact-field march or, on a tile with a network, the single-call surrogate
+hybrid, and the completion times are per-stream running sums."""
 import sys
 
 import numpy as np
@@ -10,7 +11,7 @@ import pytest
 import oracle
 import rayengine_amd as re
 from rayengine_amd import workloads as W
-from rayengine_amd.mixed import MixedRun, TILE_MARCH
+from rayengine_amd.mixed import MixedRun, TILE_HYBRID, TILE_MARCH
 
 pytestmark = pytest.mark.gpu
 
@@ -24,8 +25,17 @@ def test_mixed_run_matches_oracle_and_single_calls(gpu_ctx):
     import torch
     soups, dims, seeds, ro, rd, sr, tr = W.batch_workload(n_scenes=5, n_tiles=2, tri_lo=500, tri_hi=5000,
                                                            ray_lo=500, ray_hi=5000, tile_lo=32, tile_hi=48, seed=11)
-    tiles = list(zip(dims, seeds))
-    say(f"mixed: {len(soups)} scenes, tiles {dims}, {len(ro)} rays")
+    # the second tile is traced by the surrogate hybrid with a network trained on it
+    dm1, sd1 = dims[1], seeds[1]
+    v1 = gpu_ctx.noise_volume(dim=(dm1, dm1, dm1), spacing=(1.0 / dm1,) * 3, seed=sd1)
+    dv1 = re.distance_volume(v1)
+    net = re.Surrogate(gpu_ctx, seed=2)
+    net.train(dv1, n_samples=1 << 16, batch=4096, steps=200, lr=1e-3, seed=3)
+    params = net.params()[0]
+    for x in (net, dv1, v1):
+        x.release()
+    tiles = [(dims[0], seeds[0]), (dm1, sd1, params)]
+    say(f"mixed: {len(soups)} scenes, tiles {dims} (the second with a network), {len(ro)} rays")
     trees = [re.KDTree(s) for s in soups]
     ctxs = [gpu_ctx, re.Context(0)]
     dev = torch.device("cuda:0")
@@ -45,9 +55,17 @@ def test_mixed_run_matches_oracle_and_single_calls(gpu_ctx):
                 po, to = oracle.trace(*tree.export(), tree.identity_object(), ro[b:b + c], rd[b:b + c])
                 assert np.array_equal(p[b:b + c], po), f"run {rep}: prim ids of rays [{b}, {b + c})"
                 assert np.array_equal(t[b:b + c].view(np.uint32), to.view(np.uint32)), f"run {rep}: t"
-            for (dm, sd), (b, c) in zip(tiles, tr):
+            for tile, (b, c) in zip(tiles, tr):
+                dm, sd = tile[0], tile[1]
                 v = gpu_ctx.noise_volume(dim=(dm, dm, dm), spacing=(1.0 / dm,) * 3, seed=sd)
-                ts, hs, _, _ = re.hybrid_trace(v, None, ro[b:b + c], rd[b:b + c], **TILE_MARCH)
+                if len(tile) > 2:
+                    ref_net = re.Surrogate(gpu_ctx)
+                    ref_net.set_params(tile[2])
+                    ts, hs, ev, _ = re.hybrid_trace(v, ref_net, ro[b:b + c], rd[b:b + c], **TILE_HYBRID)
+                    ref_net.release()
+                    assert ev.sum() > 0, "the surrogate tile evaluates its network"
+                else:
+                    ts, hs, _, _ = re.hybrid_trace(v, None, ro[b:b + c], rd[b:b + c], **TILE_MARCH)
                 v.release()
                 assert np.array_equal(h[b:b + c], hs), f"run {rep}: tile hits of rays [{b}, {b + c})"
                 assert np.array_equal(t[b:b + c].view(np.uint32), ts.view(np.uint32)), f"run {rep}: tile t"
