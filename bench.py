@@ -522,6 +522,18 @@ def main():
                      "work": {"node_entries": counts[0], "leaves": counts[1], "tri_tests": counts[2],
                               "rays_hit": counts[3]}},
     }
+    # the same call with rays and results in pageable host memory (copied
+    # through the context's staging buffer): the PCIe-inclusive rate of the
+    # host-pointer boundary, reported beside `value`, never as it
+    scene.trace_closest(ro, rd, **tp)
+    hw = []
+    for _ in range(3):
+        th = time.perf_counter()
+        scene.trace_closest(ro, rd, **tp)
+        hw.append(time.perf_counter() - th)
+    res["host_pointer_call"] = {"ms": round(float(np.median(hw)) * 1e3, 3),
+                                "mrays_s": round(n / float(np.median(hw)) / 1e6, 1),
+                                "bytes_over_pcie": int(32 * n)}
     res["kd_build"] = kd_gpu
     res["shadow"] = shadow_section(W, torch, ctx, scene, tri_v, po, pd, pp, pt, order)
     if not args.no_mixed:
