@@ -97,3 +97,24 @@ def test_mixed_run_items_in_ascending_ray_count(gpu_ctx):
         assert run.owner == [0] * 5
     finally:
         run.release()
+
+
+def test_tune_mixed_picks_a_measured_configuration(gpu_ctx):
+    import torch
+    from rayengine_amd.autotune import tune_mixed
+    soups, dims, seeds, ro, rd, sr, tr = W.batch_workload(n_scenes=3, n_tiles=1, tri_lo=500, tri_hi=2000,
+                                                           ray_lo=500, ray_hi=3000, tile_lo=32, tile_hi=32, seed=4)
+    dev = torch.device("cuda:0")
+    n = len(ro)
+    d_o, d_d = torch.from_numpy(ro).to(dev), torch.from_numpy(rd).to(dev)
+    d_p = torch.empty(n, dtype=torch.int32, device=dev)
+    d_t = torch.empty(n, dtype=torch.float32, device=dev)
+    d_h = torch.empty(n, dtype=torch.uint8, device=dev)
+    ptrs = (d_o.data_ptr(), d_d.data_ptr(), d_p.data_ptr(), d_t.data_ptr(), d_h.data_ptr())
+    cfg, meas = tune_mixed([gpu_ctx], soups, list(zip(dims, seeds)), sr, tr, ptrs, torch.cuda.synchronize,
+                           leaf_sizes=(4, 8), blocks=(64, 128), reps=2, log=say)
+    assert len(meas) == 4
+    assert {(m["max_leaf_tris"], m["block"]) for m in meas} == {(4, 64), (4, 128), (8, 64), (8, 128)}
+    best = min(meas, key=lambda m: m["wall_ms"])
+    assert (cfg["max_leaf_tris"], cfg["block"]) == (best["max_leaf_tris"], best["block"])
+    assert cfg["rays"] == n and all(m["wall_ms"] > 0 for m in meas)
