@@ -474,6 +474,8 @@ typedef struct re_batch_tile {
   int32_t mode; /* 0 exact-field march, 1 reference sphere trace, 2 surrogate hybrid */
   int64_t ray_begin;
   int64_t ray_count;
+  int32_t mlp_id; /* network of a surrogate tile; -1: re_batch_config.mlp_id */
+  int32_t reserved;
 } re_batch_tile;
 
 typedef struct re_batch_config {
@@ -486,9 +488,15 @@ typedef struct re_batch_config {
 } re_batch_config;
 
 RE_API void re_batch_default_config(re_batch_config* c);
-/* Uploads the scenes, tiles and rays (host arrays) and captures the graph.
- * Results of a triangle ray are (prim id, t); of a terrain ray (0 hit / -1
- * miss, t). */
+/* Uploads the scenes, tiles and rays (host arrays) and captures the graph:
+ * a reset kernel, one persistent work-queue kernel over every scene and
+ * every exact / reference-march tile (items of item_rays rays, taken by
+ * whole waves from a device counter), then one hybrid-trace launch per
+ * surrogate tile.  Results of a triangle ray are (prim id, t); of a terrain
+ * ray (0 hit / -1 miss, t); each equals the single-workload entry point's
+ * result for that ray bit for bit.  The per-ray loops are bounded by the
+ * config's trace, sdf and hybrid parameters, validated as the single-workload
+ * entry points validate them. */
 RE_API re_status re_batch_create(re_ctx* ctx, const re_batch_scene* scenes, int32_t n_scenes,
                                  const re_batch_tile* tiles, int32_t n_tiles, const float* ray_o,
                                  const float* ray_d, int64_t n_rays, const re_batch_config* config,

@@ -506,8 +506,9 @@ class Batch:
     """A batched mixed workload captured in a hipGraph (re_batch_*).
 
     scenes: list of (KDTree, ray_begin, ray_count); tiles: list of
-    (Volume, mode, ray_begin, ray_count) with mode 0 exact march, 1 reference
-    sphere trace, 2 surrogate hybrid."""
+    (Volume, mode, ray_begin, ray_count[, Surrogate]) with mode 0 exact march,
+    1 reference sphere trace, 2 surrogate hybrid (with the tile's own network,
+    or `surrogate` when the tile names none)."""
 
     def __init__(self, ctx: Context, scenes, tiles, ray_o, ray_d, item_rays=256, block=256, surrogate=None,
                  trace=None, sdf=None, hybrid=None):
@@ -518,8 +519,11 @@ class Batch:
         for i, (tree, b, c) in enumerate(scenes):
             sc[i].tree, sc[i].ray_begin, sc[i].ray_count = tree.handle, int(b), int(c)
         tl = (_lib.re_batch_tile * max(len(tiles), 1))()
-        for i, (vol, mode, b, c) in enumerate(tiles):
+        for i, tile in enumerate(tiles):
+            vol, mode, b, c = tile[:4]
+            net = tile[4] if len(tile) > 4 else None
             tl[i].vol_id, tl[i].mode, tl[i].ray_begin, tl[i].ray_count = vol.id, int(mode), int(b), int(c)
+            tl[i].mlp_id = net.id if net is not None else -1
         cfg = _lib.re_batch_config()
         lib().re_batch_default_config(C.byref(cfg))
         cfg.item_rays, cfg.block = int(item_rays), int(block)

@@ -101,7 +101,8 @@ class re_batch_scene(C.Structure):
 
 
 class re_batch_tile(C.Structure):
-    _fields_ = [("vol_id", C.c_int32), ("mode", C.c_int32), ("ray_begin", C.c_int64), ("ray_count", C.c_int64)]
+    _fields_ = [("vol_id", C.c_int32), ("mode", C.c_int32), ("ray_begin", C.c_int64), ("ray_count", C.c_int64),
+                ("mlp_id", C.c_int32), ("reserved", C.c_int32)]
 
 
 class re_batch_config(C.Structure):

@@ -13,10 +13,13 @@ using namespace re;
 
 namespace {
 
-// Switched off: the first run of a batch on an MI355X (creation, capture and
-// replay) did not complete, and the cause has not been identified.  Arguments
-// are still validated; a valid batch is refused until the cause is fixed.
-constexpr bool kBatchEnabled = false;
+// Hybrid / exact-march distances are given in voxels of each tile and scaled
+// by that tile's own voxel size, as re_hybrid_trace does for one volume.
+HybridParams tile_params(const re_hybrid_params& p, const DevVolume& v) {
+  const float vox = v.spacing[0];
+  return HybridParams{p.tau_voxels * vox, p.margin_voxels * vox, p.k_exact, p.h_min_voxels * vox,
+                      p.rejoin_voxels * vox, p.n_bisect, p.max_iters};
+}
 
 void free_batch(DevBatch& b) {
   if (b.exec) (void)hipGraphExecDestroy(b.exec);
@@ -64,13 +67,24 @@ RE_API re_status re_batch_create(re_ctx* ctx, const re_batch_scene* scenes, int3
       return set_err(ctx, RE_ERR_INVALID, "re_batch_create: tile ray range or mode invalid");
     if (ctx->volumes.find(tl.vol_id) == ctx->volumes.end())
       return set_err(ctx, RE_ERR_NOT_FOUND, "re_batch_create: unknown tile volume");
-    any_surrogate |= tl.mode == 2;
+    if (tl.mode == 2) {
+      any_surrogate = true;
+      const int32_t mid = tl.mlp_id >= 0 ? tl.mlp_id : cfg.mlp_id;
+      if (ctx->mlps.find(mid) == ctx->mlps.end())
+        return set_err(ctx, RE_ERR_NOT_FOUND, "re_batch_create: a surrogate tile needs a valid mlp_id");
+    }
   }
-  if (any_surrogate && ctx->mlps.find(cfg.mlp_id) == ctx->mlps.end())
-    return set_err(ctx, RE_ERR_NOT_FOUND, "re_batch_create: surrogate tiles need a valid mlp_id");
-  if (!kBatchEnabled)
-    return set_err(ctx, RE_ERR_UNSUPPORTED,
-                   "re_batch_create: batched launches are disabled (a batch run did not complete; cause not identified)");
+  // The per-ray loops are bounded by these parameters: the same limits as the
+  // single-workload entry points (re_trace_closest, re_sdf_trace, re_hybrid_trace).
+  const re_sdf_params& sp = cfg.sdf;
+  const re_hybrid_params& hy = cfg.hybrid;
+  if (sp.max_steps < 1 || sp.max_steps > (1 << 20) || !(sp.step_rule == 0 || sp.step_rule == 1))
+    return set_err(ctx, RE_ERR_INVALID, "re_batch_create: sdf max_steps in [1,2^20], step_rule 0/1");
+  if (hy.max_iters < 1 || hy.max_iters > (1 << 20) || hy.n_bisect < 0 || hy.n_bisect > 32 || !(hy.h_min_voxels > 0.0f) ||
+      !(hy.k_exact >= 0.0f))
+    return set_err(ctx, RE_ERR_INVALID, "re_batch_create: hybrid max_iters in [1,2^20], n_bisect in [0,32], h_min > 0");
+  if (any_surrogate && !(hy.tau_voxels > hy.margin_voxels))
+    return set_err(ctx, RE_ERR_INVALID, "re_batch_create: tau_voxels must exceed margin_voxels");
   HIP_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
 
   DevBatch b;
@@ -96,12 +110,14 @@ RE_API re_status re_batch_create(re_ctx* ctx, const re_batch_scene* scenes, int3
   }
   const size_t rb = (size_t)n_rays * 12;
   std::vector<BatchTile> htiles(n_tiles);
-  const DevVolume* first_vol = nullptr;
   for (int32_t k = 0; k < n_tiles; ++k) {
     const DevVolume& v = ctx->volumes[tiles[k].vol_id];
-    if (!first_vol) first_vol = &v;
     htiles[k].vol = v.view();
-    for (int a = 0; a < 3; ++a) htiles[k].box_max[a] = v.box_max[a];
+    for (int a = 0; a < 3; ++a) {
+      htiles[k].box_min[a] = v.box_min[a];
+      htiles[k].box_max[a] = v.box_max[a];
+    }
+    htiles[k].hp = tile_params(hy, v);
     b.tile_vol.push_back(tiles[k].vol_id);
     b.tile_mode.push_back(tiles[k].mode);
   }
@@ -127,7 +143,7 @@ RE_API re_status re_batch_create(re_ctx* ctx, const re_batch_scene* scenes, int3
       begin = tl.ray_begin;
       count = tl.ray_count;
       if (tl.mode == 2) {
-        b.surrogate.push_back(SurrogateTile{target, s, begin, count, nullptr});
+        b.surrogate.push_back(SurrogateTile{target, s, begin, count, tl.mlp_id >= 0 ? tl.mlp_id : cfg.mlp_id});
         rem[s] = 1;
         continue;
       }
@@ -140,6 +156,10 @@ RE_API re_status re_batch_create(re_ctx* ctx, const re_batch_scene* scenes, int3
     }
   }
   b.n_items = (int64_t)items.size();
+  if (b.n_items > kMaxBatchItems) {
+    free_batch(b);
+    return set_err(ctx, RE_ERR_INVALID, "re_batch_create: more than 2^30 work items (raise item_rays)");
+  }
   hipError_t e = hipSuccess;
   auto alloc = [&](void** p, size_t bytes) {
     if (e == hipSuccess) e = hipMalloc(p, bytes > 0 ? bytes : 64);
@@ -199,13 +219,6 @@ RE_API re_status re_batch_create(re_ctx* ctx, const re_batch_scene* scenes, int3
   L.one_sided = cfg.trace.one_sided;
   L.secondary = cfg.trace.secondary;
   L.sdf = BatchSdf{cfg.sdf.max_steps, cfg.sdf.eps, cfg.sdf.step_scale, cfg.sdf.step_rule};
-  const float vox = first_vol ? first_vol->spacing[0] : 1.0f;
-  // Exact-march distances are given in voxels of each tile; tiles of one
-  // batch share the voxel size of the first tile.
-  b.hp = HybridParams{cfg.hybrid.tau_voxels * vox, cfg.hybrid.margin_voxels * vox, cfg.hybrid.k_exact,
-                      cfg.hybrid.h_min_voxels * vox, cfg.hybrid.rejoin_voxels * vox, cfg.hybrid.n_bisect,
-                      cfg.hybrid.max_iters};
-  L.hp = b.hp;
   L.block = cfg.block;
   L.grid_blocks = batch_grid(cfg.block);
 
@@ -216,13 +229,8 @@ RE_API re_status re_batch_create(re_ctx* ctx, const re_batch_scene* scenes, int3
   for (const SurrogateTile& st : b.surrogate) {
     if (e != hipSuccess) break;
     const DevVolume& v = ctx->volumes[b.tile_vol[st.tile]];
-    const DevMlp& m = ctx->mlps[b.mlp_id];
-    HybridParams hp = b.hp;
-    const float tv = v.spacing[0];
-    hp.tau = cfg.hybrid.tau_voxels * tv;
-    hp.margin = cfg.hybrid.margin_voxels * tv;
-    hp.h_min = cfg.hybrid.h_min_voxels * tv;
-    hp.rejoin = cfg.hybrid.rejoin_voxels * tv;
+    const DevMlp& m = ctx->mlps[st.mlp_id];
+    const HybridParams hp = tile_params(hy, v);
     e = launch_hybrid_trace(v, infer_pack(m), infer_small(m), b.ray_o + 3 * st.begin, b.ray_d + 3 * st.begin, st.count, hp, true,
                             b.t + st.begin, b.hit_scratch + st.begin, nullptr, nullptr, ctx->stream);
     if (e == hipSuccess) e = launch_hit_to_prim(b.hit_scratch + st.begin, b.prim + st.begin, st.count, ctx->stream);

@@ -194,10 +194,18 @@ struct BatchItem {
   int32_t slot;    // completion slot (scenes first, then tiles)
 };
 
+// A terrain tile of a batch: its volume, box and exact-march parameters in
+// world units (scaled by that tile's own voxel size, as re_hybrid_trace
+// scales them).
 struct BatchTile {
   VolView vol;
-  float box_max[3];
+  float box_min[3], box_max[3];
+  HybridParams hp;
 };
+
+// Work items of one batch are counted by a 32-bit device counter, which each
+// wave also advances once past the end.
+constexpr int64_t kMaxBatchItems = (int64_t)1 << 30;
 
 struct BatchSdf {
   int32_t max_steps;
@@ -217,14 +225,13 @@ struct BatchLaunch {
   const float* ray_d;
   int32_t* prim;
   float* t;
-  unsigned long long* ctl;           // [0] item counter, [1] start timestamp
+  unsigned long long* ctl;           // [0] item counter (32-bit, low word), [1] start timestamp
   int32_t* remaining;                // per slot: items left
   const int32_t* remaining_init;
   unsigned long long* done_ts;       // per slot completion timestamp
   int32_t n_slots;
   int32_t cap, one_sided, secondary;
   BatchSdf sdf;
-  HybridParams hp;                   // exact-march parameters (world units)
   int32_t block;
   int64_t grid_blocks;
 };
@@ -233,7 +240,7 @@ struct BatchLaunch {
 struct SurrogateTile {
   int32_t tile, slot;
   int64_t begin, count;
-  uint8_t* hit;  // scratch hit flags, converted into prim by the stamp step
+  int32_t mlp_id;  // the tile's network
 };
 
 struct DevBatch {
@@ -257,7 +264,6 @@ struct DevBatch {
   std::vector<SurrogateTile> surrogate;
   int32_t mlp_id = -1;
   BatchLaunch launch;
-  HybridParams hp;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   double wall_clock_khz = 100000.0;

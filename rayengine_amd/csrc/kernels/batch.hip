@@ -17,7 +17,7 @@ __global__ void batch_reset_kernel(unsigned long long* __restrict__ ctl, int32_t
                                    unsigned long long* __restrict__ done_ts, int32_t n_slots) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i == 0) {
-    ctl[0] = 0ull;
+    ctl[0] = 0ull;  // item counter (low 32 bits)
     ctl[1] = wall_clock64();
   }
   if (i < n_slots) {
@@ -39,14 +39,36 @@ __global__ void hit_to_prim_kernel(const uint8_t* __restrict__ hit, int32_t* __r
   if (i < n) prim[i] = hit[i] ? 0 : -1;
 }
 
+// Wave-uniform fetch-and-add: every lane executes the atomic, lane 0 with
+// `add` and the others with 0, so the wave adds `add` once; the value before
+// the add is read back from lane 0 (the first active lane: the caller's wave
+// is converged) with readfirstlane, which puts it in a scalar register.
+// Control flow that depends on the result is therefore a scalar branch taken
+// by the whole wave.
+//
+// Why not "if (lane == 0) v = atomicAdd(...); v = __shfl(v, 0);": in a
+// loop whose exit depends on v, that form leaves the compiler free to treat
+// the exit as divergent.  The gfx950 code it gave for this kernel re-entered
+// the block holding the shuffle with lanes 1-63 only (lane 0 had left to
+// issue the next atomic), so those lanes read a stale zero, took item 0 again
+// and never left the loop: the launch did not finish.
+__device__ __forceinline__ uint32_t wave_fetch_add(uint32_t* p, uint32_t add, int lane) {
+  const uint32_t v = atomicAdd(p, lane == 0 ? add : 0u);
+  return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
+}
+
+__device__ __forceinline__ int32_t wave_fetch_sub(int32_t* p, int32_t sub, int lane) {
+  const int32_t v = atomicSub(p, lane == 0 ? sub : 0);
+  return __builtin_amdgcn_readfirstlane(v);
+}
+
 template <bool ONE_SIDED>
 __global__ void __launch_bounds__(256) batch_kernel(BatchLaunch L) {
   const int lane = threadIdx.x & 63;
+  uint32_t* const queue = reinterpret_cast<uint32_t*>(L.ctl);  // ctl[0], low word
   for (;;) {
-    unsigned long long it = 0;
-    if (lane == 0) it = atomicAdd(&L.ctl[0], 1ull);
-    it = __shfl(it, 0);
-    if ((int64_t)it >= L.n_items) break;
+    const uint32_t it = wave_fetch_add(queue, 1u, lane);
+    if ((int64_t)it >= L.n_items) break;  // scalar: `it` is wave-uniform
     const BatchItem item = L.items[it];
     for (int r = lane; r < item.count; r += 64) {
       const int64_t i = item.begin + r;
@@ -61,24 +83,24 @@ __global__ void __launch_bounds__(256) batch_kernel(BatchLaunch L) {
         L.prim[i] = bp;
         L.t[i] = bt;
       } else {
-        const BatchTile tile = L.tiles[item.target];
-        const float bmin[3] = {0.0f, 0.0f, 0.0f};
+        const BatchTile& tile = L.tiles[item.target];
         float t;
         int32_t n;
         bool h;
         if (item.kind == kItemExact)
-          h = exact_march(tile.vol, bmin, tile.box_max, o, d, L.hp.k_exact, L.hp.h_min, L.hp.n_bisect,
-                          L.hp.max_iters, t, n);
+          h = exact_march(tile.vol, tile.box_min, tile.box_max, o, d, tile.hp.k_exact, tile.hp.h_min,
+                          tile.hp.n_bisect, tile.hp.max_iters, t, n);
         else
-          h = plain_march(tile.vol, bmin, tile.box_max, o, d, L.sdf.max_steps, L.sdf.eps, L.sdf.step_scale,
+          h = plain_march(tile.vol, tile.box_min, tile.box_max, o, d, L.sdf.max_steps, L.sdf.eps, L.sdf.step_scale,
                           L.sdf.step_rule, t, n);
         L.prim[i] = h ? 0 : -1;
         L.t[i] = h ? t : kInf;
       }
     }
-    if (lane == 0) {
-      if (atomicSub(&L.remaining[item.slot], 1) == 1) L.done_ts[item.slot] = wall_clock64();
-    }
+    // The wave's one decrement of the target's item count; the wave that
+    // takes it to zero stamps the target's completion (every lane stores the
+    // same scalar value: no lane-dependent branch inside the loop).
+    if (wave_fetch_sub(&L.remaining[item.slot], 1, lane) == 1) L.done_ts[item.slot] = wall_clock64();
   }
 }
 
@@ -112,6 +134,7 @@ hipError_t launch_batch_stamp(unsigned long long* done_ts, int32_t slot, int32_t
 
 hipError_t launch_batch(const BatchLaunch& L, hipStream_t s) {
   if (L.n_items <= 0) return hipSuccess;
+  if (L.n_items > kMaxBatchItems) return hipErrorInvalidValue;
   const int block = L.block > 0 ? L.block : 256;
   int64_t grid = L.grid_blocks > 0 ? L.grid_blocks : 1024;
   const int64_t want = (L.n_items * 64 + block - 1) / block;

@@ -30,12 +30,12 @@ def _bits(x):
     return np.ascontiguousarray(x).view(np.uint32)
 
 
-@pytest.mark.skip(reason="batched launches are switched off: the first GPU run of this test did not complete and "
-                         "the cause is not identified; re_batch_create returns RE_ERR_UNSUPPORTED until it is")
+@pytest.mark.timeout(120, method="thread")
 @pytest.mark.parametrize("item_rays,block", [(256, 256), (64, 128), (1000, 64)])
 def test_batch_equals_single_calls(gpu_ctx, small_batch, item_rays, block):
     trees, vols, ro, rd, sr, tr = small_batch
     modes = [0, 1, 0]
+    say(f"test_batch_equals_single_calls: create item_rays={item_rays} block={block} rays={len(ro)}")
     b = re.Batch(gpu_ctx, [(t, s, c) for t, (s, c) in zip(trees, sr)],
                  [(v, m, s, c) for v, m, (s, c) in zip(vols, modes, tr)], ro, rd, item_rays=item_rays, block=block)
     say(f"batch items={b.items()} item_rays={item_rays} block={block}")
@@ -52,6 +52,7 @@ def test_batch_equals_single_calls(gpu_ctx, small_batch, item_rays, block):
             t, h, _ = v.trace(ro[s:s + c], rd[s:s + c])
         refs.append((s, c, np.where(h == 1, 0, -1).astype(np.int32), t))
     for rep in range(3):
+        say(f"replay {rep}")
         b.run()
         p, t = b.results()
         for s, c, pr, tr_ in refs:
@@ -70,9 +71,7 @@ def test_batch_rejects_bad_ranges(gpu_ctx, small_batch):
         re.Batch(gpu_ctx, [(trees[0], len(ro) - 5, 10)], [], ro, rd)
     with pytest.raises(re.ReError):
         re.Batch(gpu_ctx, [], [(vols[0], 2, 0, 10)], ro, rd)          # surrogate without a network
-
-
-def test_batch_is_refused_while_switched_off(gpu_ctx, small_batch):
-    trees, vols, ro, rd, sr, tr = small_batch
-    with pytest.raises(re.ReError, match="disabled"):
-        re.Batch(gpu_ctx, [(trees[0], sr[0][0], sr[0][1])], [], ro, rd)
+    with pytest.raises(re.ReError):
+        re.Batch(gpu_ctx, [], [(vols[0], 0, 0, 10)], ro, rd, hybrid=dict(max_iters=0))   # unbounded march
+    with pytest.raises(re.ReError):
+        re.Batch(gpu_ctx, [], [(vols[0], 1, 0, 10)], ro, rd, sdf=dict(max_steps=0))
