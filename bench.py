@@ -1,31 +1,41 @@
-"""Benchmark of the rope kd-tree closest-hit path on one MI355X.
+"""Benchmark of the RayEngine ray-query path on one MI355X.
 
-Workload (the cluster scene): 2^20 FP32 triangles in 64 Gaussian clusters
-(sigma 0.3, centres U[-10,10]^3, edge lengths LogNormal(-3, 0.5), seed 1), SAH
-kd-tree with ropes (leaf <= 8, depth <= 24); 1920x1080 primary rays from
-(0,0,30) towards the origin (FOV 60), issued in 8x8 pixel tiles, plus one
-cosine-weighted diffuse bounce ray per primary hit, in the order of their
-primaries.  One step = one closest-hit launch over every ray, with rays and
-results resident in HBM.
+Headline workload (the cluster scene): 2^20 FP32 triangles in 64 Gaussian
+clusters (sigma 0.3, centres U[-10,10]^3, edge lengths LogNormal(-3, 0.5),
+seed 1), SAH kd-tree with ropes (leaf <= 8, depth <= 24); 1920x1080 primary
+rays from a camera 0.8 units outside the cluster nearest the origin, looking
+at its centre (FOV 60; 95 % of the primaries hit), issued in 8x8 pixel tiles,
+plus one cosine-weighted diffuse bounce ray per primary hit, in the order of
+their primaries: 4,038,016 rays, 1,964,416 of them bounces.  One step = one
+closest-hit launch over every ray, with rays and results resident in HBM.
 
 Prints ONE JSON line.  `value` is rays traced per second (Mrays/s) over the
 timed steps.  `roofline` uses the algorithmic bytes of a launch, from the
 kernel's own work counters: per ray 24 B in + 8 B out, 8 B per node entry,
 56 B per leaf record, 40 B per triangle record; divided by the mean kernel
 duration from HIP events recorded on the launch stream during the timed
-steps.  `cpu_baseline` times the CPU oracle on a bounded sample of the same
-rays.  `terrain` reports the terrain path (noise volume, surrogate training
-and inference, and Mrays/s with hit depth error of the reference sphere
-tracer, the exact-field march and the surrogate hybrid).  `mixed_workload`
-runs the many-scene, many-tile workload one launch per scene or tile.
+steps.  `roofline.traffic` is measured by this run: two rocprofv3 counter
+passes (FETCH_SIZE, then WRITE_SIZE) over a child process that launches the
+same kernel on the same workload beside a device copy of known size, which
+calibrates the counters.  `parts` times the primary rays and the bounce rays
+(the incoherent part) as launches of their own.  `cpu_baseline` times the CPU
+oracle (the reference's traversal restated in C++) on the box's host cores.
+Further sections: the small icosphere scene (`small_scene`), any-hit queries
+(`shadow`), the device kd build (`kd_build`), the many-scene mixed workload
+unbatched and as one persistent work-queue launch in a captured hipGraph
+(`mixed_workload`), and the terrain path (`terrain`: noise volume, sphere
+tracers, surrogate training and inference, FP32-vs-FP64 depth error, CPU
+baselines).
 """
 from __future__ import annotations
 
 import argparse
+import csv
 import json
-import re as re_mod
 import os
+import subprocess
 import sys
+import tempfile
 import time
 from pathlib import Path
 
@@ -35,310 +45,215 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+BF16_PEAK_TFLOPS = 2500.0
 METRIC = "Mrays/s closest-hit; terrain Mrays/s surrogate vs exact SDF at ≤0.5-voxel err"
+TRACE_KERNEL = {0: "trace_closest_kernel", 1: "trace_persistent_kernel", 2: "trace_closest_kernel",
+                3: "trace_closest_o6_kernel"}
+DEFAULT_TUNE = {"max_leaf_tris": 8, "tile": 8, "block": 64, "variant": 3}
+COPY_PROBE_BYTES = 1 << 28
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def _cpu_model():
+# --------------------------------------------------------------------------- CPU side
+
+def cpu_info():
+    """What the CPU baselines ran on: the thread count used (the box's CPU
+    share, OMP_NUM_THREADS when set), what nproc and the affinity mask show,
+    and the lscpu model name."""
+    model = None
     try:
-        for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                return line.split(":", 1)[1].strip()
-    except OSError:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+    except Exception:
         pass
-    return None
+    if model is None:
+        try:
+            for line in open("/proc/cpuinfo"):
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+        except OSError:
+            pass
+    aff = len(os.sched_getaffinity(0))
+    omp = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    threads = min(aff, omp) if omp > 0 else aff
+    return {"threads": threads, "nproc": os.cpu_count(), "affinity_cpus": aff,
+            "omp_num_threads": omp or None, "model": model}
 
 
-def cpu_baseline(tree, ro, rd, n_primary, threads, budget_cpu_s=20.0):
-    """Oracle traversal (the reference's walk, no shortcut) timed on the host
-    cores: a stratified sample (primary and bounce rays in the workload's
-    proportion) or, when the whole workload takes less, whole passes over it,
-    sized to about budget_cpu_s of CPU time (threads x wall)."""
-    import oracle
-    nodes, leaves, tris = tree.export()
-    obj = tree.identity_object()
-    rng = np.random.default_rng(0)
-    n = len(ro)
-    frac_b = (n - n_primary) / n
-    probe = rng.choice(n, 20000, replace=False)
-    t0 = time.perf_counter()
-    oracle.trace(nodes, leaves, tris, obj, ro[probe], rd[probe], threads=threads)
-    rate = 20000 / max(time.perf_counter() - t0, 1e-6)
-    want = rate * budget_cpu_s / threads
-    whole = want >= n
-    if whole:
-        so, sd = np.ascontiguousarray(ro), np.ascontiguousarray(rd)
-        m, mb = n, n - n_primary
-    else:
-        m = int(max(20000, want))
-        mb = int(round(m * frac_b))
-        sel = np.concatenate([rng.choice(n_primary, m - mb, replace=False),
-                              n_primary + rng.choice(n - n_primary, mb, replace=False)])
-        so, sd = np.ascontiguousarray(ro[sel]), np.ascontiguousarray(rd[sel])
-    reps = 0
-    t0 = time.perf_counter()
-    while True:
-        oracle.trace(nodes, leaves, tris, obj, so, sd, threads=threads)
-        reps += 1
-        dt = time.perf_counter() - t0
-        if not whole or dt * threads >= budget_cpu_s:
-            break
-    what = (f"the whole workload ({n} rays: {n_primary} primary + {n - n_primary} bounce) x {reps} passes"
-            if whole else f"{m} rays ({m - mb} primary + {mb} bounce, uniform sample of the workload)")
-    return {"value": round(m * reps / dt / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "sample": f"{what}, CPU oracle (FP32 restatement of the reference traversal, full walks to the cap), "
-                      f"OpenMP dynamic/64 over {threads} threads, {dt:.1f} s wall"}
+def time_cpu(fn, warmup=1, reps=5):
+    """1 warm-up call, then the median wall time of `reps` calls (seconds)."""
+    for _ in range(warmup):
+        fn()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts)), [round(t, 4) for t in ts]
 
 
-# Terrain settings (hit depth error is measured against REFERENCE_DEPTH, the
-# fine exact-field march that tests/test_gpu_terrain.py pins to the FP64
-# first crossing within 0.02 voxel at the 99th percentile).
-REFERENCE_DEPTH = dict(k_exact=0.02, h_min_voxels=0.25, n_bisect=20, max_iters=1 << 16)
-TERRAIN_METHODS = {
-    # the reference sphere tracer (renderer.cu:142-235) at its own settings
-    "reference_march": ("plain", dict(step_scale=0.8, eps=1e-3, max_steps=256)),
-    # the same tracer at the smallest step and tolerance tried: its most accurate setting
-    "reference_march_fine": ("plain", dict(step_scale=0.02, eps=1e-5, max_steps=8192)),
-    # exact-field bracketing march (no network)
-    "exact_march": ("exact", dict(k_exact=0.05, h_min_voxels=1.0, n_bisect=1)),
-    # surrogate far field + exact near field
-    "surrogate_hybrid": ("hybrid", dict(tau_voxels=4.0, margin_voxels=1.0, k_exact=0.1, n_bisect=4)),
-}
-MLP_FLOP_PER_EVAL = 2 * (99 * 64 + 3 * 64 * 64 + 64)
-# forward + backward (input and weight gradients) per training sample, about
-# three forward passes
-MLP_TRAIN_FLOP_PER_SAMPLE = 3 * MLP_FLOP_PER_EVAL
-NOISE_FLOP_PER_VOXEL = 5 * 70
-FP32_PEAK_TFLOPS = 157.3
-# Surrogate training: 2^22 samples (half within 2 voxels of the surface), Adam
-# lr 1e-3, batch 65,536, 20k steps.
-TRAIN_STEPS = 20000
-BF16_PEAK_TFLOPS = 2500.0
+def cpu_rate(fn, units, unit, threads, what, kind="port", one_core_fn=None, one_core_units=None,
+             one_core_what=None):
+    """A cpu_baseline object: `units` per second of fn() (1 warm-up + median
+    of 5) on `threads` threads, and the same on one core (optionally on a
+    smaller sample)."""
+    med, runs = time_cpu(lambda: fn(threads))
+    rec = {"value": round(units / med / 1e6, 4), "unit": unit, "cores": threads, "kind": kind,
+           "sample": what, "median_s": round(med, 4), "runs_s": runs, "protocol": "1 warm-up + median of 5"}
+    f1 = one_core_fn or fn
+    u1 = one_core_units or units
+    med1, runs1 = time_cpu(lambda: f1(1))
+    rec["one_core"] = {"value": round(u1 / med1 / 1e6, 4), "median_s": round(med1, 4), "runs_s": runs1,
+                       "sample": one_core_what or what}
+    return rec
 
 
-def terrain_section(re, W, torch, ctx):
-    """Terrain path on one GPU: the 256^3 fBm volume (seed 2016) over the unit
-    cube, 1920x1080 rays from the terrain camera; noise generation, training,
-    network throughput, and hit depth error / Mrays/s of each tracer."""
-    dev = torch.device("cuda:0")
-    out = {"workload": "256^3 fBm terrain (5 octaves, seed 2016) on the unit cube, 1920x1080 rays"}
-    vol = ctx.noise_volume()
-    noise = []
-    for _ in range(3):
-        v2 = ctx.noise_volume()
-        noise.append(ctx.last_kernel_ms())
-        v2.release()
-    out["noise_ms"] = round(float(np.mean(noise)), 4)
-    out["noise_gvox_s"] = round(256 ** 3 / (np.mean(noise) * 1e-3) / 1e9, 2)
-    # VALU-bound: about 5 octaves x 70 FP32 operations per voxel, and a 4-byte write
-    nsec = float(np.mean(noise)) * 1e-3
-    out["noise_roofline"] = {"bound": "valu", "flop_per_voxel": NOISE_FLOP_PER_VOXEL,
-                             "achieved_tflops": round(256 ** 3 * NOISE_FLOP_PER_VOXEL / nsec / 1e12, 2),
-                             "peak_tflops": FP32_PEAK_TFLOPS,
-                             "frac": round(256 ** 3 * NOISE_FLOP_PER_VOXEL / nsec / 1e12 / FP32_PEAK_TFLOPS, 4),
-                             "write_gbs": round(256 ** 3 * 4 / nsec / 1e9, 1)}
-    # brush edit (SPHERE_ADD) over the whole 256^3 volume and its material
-    # channel: 8 bytes read and 8 written per voxel
-    ev = ctx.noise_volume()
-    ev.edit("sphere_add", center=(0.5, 0.45, 0.5), extent=(0.1, 0, 0), material=1.0)
-    ems = []
-    for k in range(5):
-        ev.edit("sphere_add", center=(0.3 + 0.1 * k, 0.45, 0.5), extent=(0.1, 0, 0), material=1.0)
-        ems.append(ctx.last_kernel_ms())
-    ev.release()
-    em = float(np.mean(ems))
-    out["edit"] = {"kernel": "volume_edit_kernel", "kernel_ms": round(em, 4), "bound": "hbm",
-                   "achieved_gbs": round(256 ** 3 * 16 / (em * 1e-3) / 1e9, 1), "peak_gbs": HBM_PEAK_GBS,
-                   "frac": round(256 ** 3 * 16 / (em * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
-    dv = re.distance_volume(vol)
-    net = re.Surrogate(ctx, seed=1)
-    t0 = time.perf_counter()
-    hist = net.train(dv, n_samples=1 << 22, batch=65536, steps=TRAIN_STEPS, lr=1e-3, seed=1, over_weight=4.0)
-    step_ms = (time.perf_counter() - t0) * 1e3 / TRAIN_STEPS
-    train_tf = MLP_TRAIN_FLOP_PER_SAMPLE * 65536 / (step_ms * 1e-3) / 1e12
-    out["train"] = {"steps": TRAIN_STEPS, "batch": 65536, "ms_per_step": round(step_ms, 4),
-                    "loss_first": float(hist[:10].mean()), "loss_last": float(hist[-100:].mean()),
-                    "flop_per_sample": MLP_TRAIN_FLOP_PER_SAMPLE, "tflops": round(train_tf, 2),
-                    "frac_of_bf16_peak": round(train_tf / BF16_PEAK_TFLOPS, 4)}
-    # network throughput (re_mlp_forward on 4M device points)
-    npts = 1 << 22
-    pts = torch.rand(npts, 3, device=dev)
-    val = torch.empty(npts, device=dev)
-    torch.cuda.synchronize()
-    for _ in range(2):
-        net.forward_device(pts.data_ptr(), npts, val.data_ptr())
-    for _ in range(5):
-        net.forward_device(pts.data_ptr(), npts, val.data_ptr(), sync=False)
-    torch.cuda.synchronize()
-    fms = float(np.mean(ctx.kernel_times(5)))
-    tf = npts * MLP_FLOP_PER_EVAL / (fms * 1e-3) / 1e12
-    out["mlp_forward"] = {"kernel": "mlp_infer_kernel", "gevals_s": round(npts / (fms * 1e-3) / 1e9, 3),
-                          "tflops": round(tf, 2), "bound": "mfma", "peak_tflops": BF16_PEAK_TFLOPS,
-                          "frac": round(tf / BF16_PEAK_TFLOPS, 4), "flop_per_eval": MLP_FLOP_PER_EVAL}
-    o, d = W.terrain_rays(1920, 1080)
-    n = len(o)
-    do, dd = torch.from_numpy(o).to(dev), torch.from_numpy(d).to(dev)
-    dt = torch.empty(n, dtype=torch.float32, device=dev)
-    dh = torch.empty(n, dtype=torch.uint8, device=dev)
-    torch.cuda.synchronize()
+# --------------------------------------------------------------------------- traffic (rocprofv3)
 
-    def run(kind, params, sync=True):
-        if kind == "plain":
-            vol.trace_device(do.data_ptr(), dd.data_ptr(), n, dt.data_ptr(), dh.data_ptr(), sync=sync, **params)
-        else:
-            re.hybrid_trace_device(vol, net if kind == "hybrid" else None, do.data_ptr(), dd.data_ptr(), n,
-                                   dt.data_ptr(), dh.data_ptr(), sync=sync, **params)
+def _dispatch_counters(csv_dir: Path, counter: str):
+    """[(dispatch id, kernel name, value)] of one counter from rocprofv3's
+    counter_collection CSV, in dispatch order."""
+    rows = {}
+    for f in sorted(csv_dir.rglob("*counter_collection.csv")):
+        with open(f, newline="") as fh:
+            for r in csv.DictReader(fh):
+                if r.get("Counter_Name") != counter:
+                    continue
+                key = int(r["Dispatch_Id"])
+                name = r.get("Kernel_Name", "")
+                rows.setdefault(key, [name, 0.0])
+                rows[key][1] += float(r["Counter_Value"])
+    return [(k, rows[k][0], rows[k][1]) for k in sorted(rows)]
 
-    run("exact", REFERENCE_DEPTH)
-    ref_t, ref_h = dt.cpu().numpy().astype(np.float64), dh.cpu().numpy()
-    vox = 1.0 / 256
-    methods = {}
-    for name, (kind, params) in TERRAIN_METHODS.items():
-        run(kind, params)
-        t, h = dt.cpu().numpy().astype(np.float64), dh.cpu().numpy()
-        both = (h == 1) & (ref_h == 1)
-        e = np.abs(t[both] - ref_t[both]) / vox
-        for _ in range(5):
-            run(kind, params, sync=False)
-        torch.cuda.synchronize()
-        ms = float(np.mean(ctx.kernel_times(5)))
-        # field samples per ray (untimed host-pointer call): 32 bytes of the
-        # trilinear cell per sample + 32 bytes of ray in and result out
-        if kind == "plain":
-            _, _, steps = vol.trace(o, d, **params)
-        else:
-            _, _, evals, steps = re.hybrid_trace(vol, net if kind == "hybrid" else None, o, d, **params)
-        alg = float(32 * n + 32 * np.sum(steps, dtype=np.int64))
-        roof = {"samples_per_ray": round(float(np.mean(steps)), 2),
-                "algorithmic_gbs": round(alg / (ms * 1e-3) / 1e9, 1),
-                "hbm_frac": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
-        if kind == "hybrid":
-            roof["network_evals_per_ray"] = round(float(np.mean(evals)), 2)
-        methods[name] = {"mrays_s": round(n / (ms * 1e-3) / 1e6, 1), "kernel_ms": round(ms, 4), "roofline": roof,
-                         "err_p50_vox": round(float(np.median(e)), 4), "err_p99_vox": round(float(np.percentile(e, 99)), 4),
-                         "err_max_vox": round(float(e.max()), 3) if e.size else None,
-                         "frac_err_gt_half_vox": round(float(np.mean(e > 0.5)), 5),
-                         "hit_agree": round(float(np.mean(h == ref_h)), 5), "params": params}
-    out["methods"] = methods
-    # frame output of the reference tracer's frame (sky for misses, shaded hits, RGBA8)
-    dn = torch.empty(n * 3, dtype=torch.float32, device=dev)
-    dpx = torch.empty(n, dtype=torch.int32, device=dev)
-    torch.cuda.synchronize()
-    vol.trace_device(do.data_ptr(), dd.data_ptr(), n, dt.data_ptr(), dh.data_ptr(), nrm_ptr=dn.data_ptr(),
-                     **TERRAIN_METHODS["reference_march"][1])
-    fr = []
-    for _ in range(6):
-        re.frame_shade_device(ctx, dd.data_ptr(), dh.data_ptr(), dn.data_ptr(), 1920, 1080, dpx.data_ptr())
-        fr.append(ctx.last_kernel_ms())
-    fms = float(np.mean(fr[1:]))
-    out["frame"] = {"kernel": "frame_kernel", "kernel_ms": round(fms, 4), "mpixels_s": round(n / (fms * 1e-3) / 1e6, 1),
-                    "sky_pixels": int(n - int(dh.sum().item()))}
-    hy, fine, ex = methods["surrogate_hybrid"], methods["reference_march_fine"], methods["exact_march"]
-    out["surrogate_vs_reference_march"] = round(hy["mrays_s"] / fine["mrays_s"], 3)
-    out["surrogate_vs_exact_march"] = round(hy["mrays_s"] / ex["mrays_s"], 3)
-    reach = "reaches" if fine["err_p99_vox"] <= 0.5 else "does not reach"
-    out["note"] = (f"reference_march_fine is the reference tracer's most accurate setting tried; it {reach} 0.5 voxel "
-                   f"at p99. exact_march (no network) is the ablation of the surrogate.")
-    for x in (net, dv, vol):
-        x.release()
+
+def measure_traffic(tune, timeout_s=150):
+    """HBM-side bytes per launch of the closest-hit kernel on the headline
+    workload, from two rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE) over
+    `bench.py --traffic-probe`, each calibrated by a device copy of known size
+    in the same process.  Runs before this process touches the GPU (the
+    probes are child processes).  Returns a dict, or one with "error"."""
+    exe = subprocess.run(["which", "rocprofv3"], capture_output=True, text=True).stdout.strip()
+    if not exe:
+        return {"error": "rocprofv3 not found"}
+    kname = TRACE_KERNEL[tune["variant"]]
+    out = {"kernel": kname, "counters": {}}
+    env = dict(os.environ, TMPDIR="/tmp")
+    total = 0.0
+    for counter, bytes_per_copy in (("FETCH_SIZE", COPY_PROBE_BYTES), ("WRITE_SIZE", COPY_PROBE_BYTES)):
+        d = Path(tempfile.mkdtemp(prefix=f"re_pmc_{counter.lower()}_", dir="/tmp"))
+        cmd = [exe, "--pmc", counter, "--output-format", "csv", "-d", str(d), "-o", "run", "--",
+               sys.executable, str(ROOT / "bench.py"), "--traffic-probe", "--probe-leaf", str(tune["max_leaf_tris"]),
+               "--probe-tile", str(tune["tile"]), "--probe-block", str(tune["block"]),
+               "--probe-variant", str(tune["variant"])]
+        t0 = time.time()
+        try:
+            r = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout_s, cwd="/tmp", env=env)
+        except subprocess.TimeoutExpired:
+            return {"error": f"{counter} pass timed out after {timeout_s} s"}
+        rows = _dispatch_counters(d, counter)
+        trace = [v for _, n, v in rows if kname in n]
+        copy = [v for _, n, v in rows if "copyBuffer" in n or "copy" in n.lower() and kname not in n]
+        if len(trace) < 3 or len(copy) < 3:
+            return {"error": f"{counter} pass: rc {r.returncode}, {len(trace)} trace and {len(copy)} copy dispatches "
+                             f"found; stderr tail: {r.stderr[-300:]}"}
+        trace, copy = trace[-3:], copy[-3:]            # the probe's measured launches come last
+        factor = bytes_per_copy / float(np.mean(copy))  # bytes per counter unit
+        b = float(np.mean(trace)) * factor
+        total += b
+        out["counters"][counter] = {"per_dispatch": [round(x, 1) for x in trace], "copy_per_dispatch":
+                                    [round(x, 1) for x in copy], "copy_bytes": bytes_per_copy,
+                                    "bytes_per_unit": round(factor, 4), "bytes_per_launch": int(b),
+                                    "pass_s": round(time.time() - t0, 1)}
+    out["bytes_per_launch"] = int(total)
+    out["note"] = ("counts of traffic below the L2 (L2 misses, served by the Infinity Cache or HBM), so an upper "
+                   "bound on HBM bytes; calibration: a device-to-device copy of known size in the same process")
     return out
 
 
-def mixed_workload_section(re, W, torch, ctx, reps=3, n_streams=4):
-    """The batched mixed workload (workloads.batch_workload, seed 7: 64
-    triangle scenes of 1k-500k triangles with 16k-2M incoherent rays each, 16
-    noise-terrain tiles of 64^3-256^3 with their own rays), run unbatched by
-    rayengine_amd.mixed: one closest-hit launch per scene and, per tile, the
-    exact-field march (even tiles) or the surrogate hybrid with a network
-    trained on that tile (odd tiles), in ascending ray count, every input resident in HBM, on
-    one stream and dealt round-robin over n_streams contexts (one HIP stream
-    each).  Leaf size and block are picked first by autotune.tune_mixed on the
-    n_streams issue.  Aggregate Mrays/s over the wall time of the whole
-    workload; an item's completion time is the device time from the start of
-    its stream's first launch to the end of its own."""
-    from rayengine_amd.autotune import tune_mixed
-    from rayengine_amd.mixed import MixedRun
-    t0 = time.perf_counter()
-    soups, dims, seeds, ro, rd, sr, tr = W.batch_workload()
-    # every other tile is traced by the surrogate hybrid with a network of its
-    # own (2,000 Adam steps on 2^20 samples of that tile's distance field)
-    t_net = time.perf_counter()
-    tiles = []
-    for k, (dm, sd) in enumerate(zip(dims, seeds)):
-        if k % 2 == 1:
-            v = ctx.noise_volume(dim=(dm, dm, dm), spacing=(1.0 / dm,) * 3, seed=sd)
-            dv = re.distance_volume(v)
-            net = re.Surrogate(ctx, seed=1)
-            net.train(dv, n_samples=1 << 20, batch=65536, steps=2000, lr=1e-3, seed=1, over_weight=4.0)
-            tiles.append((dm, sd, net.params()[0]))
-            net.release()
-            dv.release()
-            v.release()
-        else:
-            tiles.append((dm, sd))
-    net_s = time.perf_counter() - t_net
-    ctxs = [ctx] + [re.Context(0) for _ in range(n_streams - 1)]
+def traffic_probe(args):
+    """Child process of measure_traffic: the headline workload, the closest-hit
+    kernel launched 3 times, then 3 device copies of COPY_PROBE_BYTES."""
+    import torch
+    import rayengine_amd as re
+    from rayengine_amd import workloads as W
+    tri_v = W.cluster_soup_default()
+    ctx = re.Context(0)
+    tree = re.KDTree(tri_v, max_leaf_tris=args.probe_leaf, ctx=ctx)   # identical to the host build
+    scene = ctx.upload_scene([re.SceneObject(tree)])
+    ro, rd = W.cluster_workload_rays(scene, tri_v, args.probe_tile)
     dev = torch.device("cuda:0")
     n = len(ro)
     d_o, d_d = torch.from_numpy(ro).to(dev), torch.from_numpy(rd).to(dev)
     d_p = torch.empty(n, dtype=torch.int32, device=dev)
     d_t = torch.empty(n, dtype=torch.float32, device=dev)
-    d_h = torch.empty(n, dtype=torch.uint8, device=dev)
-    ptrs = (d_o.data_ptr(), d_d.data_ptr(), d_p.data_ptr(), d_t.data_ptr(), d_h.data_ptr())
-    cfg, meas = tune_mixed(ctxs, soups, tiles, sr, tr, ptrs, torch.cuda.synchronize, log=log)
-    trees = [re.KDTree(s, max_leaf_tris=cfg["max_leaf_tris"], ctx=ctx) for s in soups]
-    n_scene_rays = int(sum(c for _, c in sr))
-    out = {"workload": "64 cluster-soup scenes (1k-500k triangles, log-uniform) with 16k-2M incoherent rays each "
-                       "(log-uniform), 16 noise-terrain tiles 64^3-256^3 (seeds 2016+k) with 16k-2M rays each; "
-                       "seed 7",
-           "mode": "unbatched: one launch per scene or tile, ascending ray count (the batched persistent launch "
-                   "is switched off)",
-           "triangles": int(sum(len(s) for s in soups)), "rays": int(n), "scene_rays": n_scene_rays,
-           "tile_rays": int(n - n_scene_rays),
-           "tiles_traced_with": "even tiles: exact-field march (k 0.05, h_min 1 voxel); odd tiles: surrogate hybrid "
-                                "(tau 4 voxels, margin 1, k 0.1, 4 bisections) with a network per tile, 2,000 Adam "
-                                "steps on 2^20 samples of its field",
-           "tile_networks_train_s": round(net_s, 2),
-           "launches": len(sr) + len(tr),
-           "autotune": {"chosen": {"max_leaf_tris": cfg["max_leaf_tris"], "block": cfg["block"]},
-                        "issue": f"{n_streams} streams",
-                        "wall_ms": {f"leaf{m['max_leaf_tris']}_block{m['block']}": m["wall_ms"] for m in meas}}}
-    first = None
-    for streams in (1, n_streams):
-        run = MixedRun(ctxs[:streams], trees, tiles, sr, tr, *ptrs)
-        run.run(block=cfg["block"], variant=cfg["variant"])
-        torch.cuda.synchronize()
-        walls, comps, ksums = [], [], []
-        for _ in range(reps):
-            torch.cuda.synchronize()
-            ts = time.perf_counter()
-            run.run(block=cfg["block"], variant=cfg["variant"])
-            torch.cuda.synchronize()
-            walls.append(time.perf_counter() - ts)
-            comps.append(run.completion_ms())
-            ksums.append(run.kernel_ms_sum())
-        wall = float(np.median(walls))
-        done = np.mean(comps, axis=0)
-        sd = np.array([done[i] for i, it in enumerate(run.items) if it[0] == "scene"])
-        td = np.array([done[i] for i, it in enumerate(run.items) if it[0] == "tile"])
-        rec = {"wall_ms": round(wall * 1e3, 3), "mrays_s": round(n / wall / 1e6, 1),
-               "kernel_ms_sum": round(float(np.mean(ksums)), 3),
-               "scene_completion_ms": {"p50": round(float(np.percentile(sd, 50)), 3),
-                                       "p99": round(float(np.percentile(sd, 99)), 3)},
-               "tile_completion_ms": {"p50": round(float(np.percentile(td, 50)), 3),
-                                      "p99": round(float(np.percentile(td, 99)), 3)}}
-        got = (d_p.cpu().numpy(), d_t.cpu().numpy().view(np.uint32), d_h.cpu().numpy())
-        if first is None:
-            first = got
-        else:
-            rec["identical_to_one_stream"] = all(np.array_equal(x, y) for x, y in zip(first, got))
-        out[f"streams_{streams}"] = rec
-        run.release()
-    out["prep_and_tune_s"] = round(time.perf_counter() - t0, 1)
-    return out
+    torch.cuda.synchronize()
+    for _ in range(3):
+        scene.trace_closest_device(d_o.data_ptr(), d_d.data_ptr(), n, d_p.data_ptr(), d_t.data_ptr(),
+                                   block=args.probe_block, variant=args.probe_variant)
+    src = torch.ones(COPY_PROBE_BYTES // 4, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"probe": "ok", "rays": n}), flush=True)
+
+
+# --------------------------------------------------------------------------- sections
+
+def launch_stats(ctx, fn, steps):
+    """Mean HIP-event kernel time (ms) of `steps` asynchronous launches."""
+    import torch
+    fn(True)
+    for _ in range(steps):
+        fn(False)
+    torch.cuda.synchronize()
+    return float(np.mean(ctx.kernel_times(steps)))
+
+
+def part_record(n, counts, kms):
+    alg = 32 * n + 8 * counts[0] + 56 * counts[1] + 40 * counts[2]
+    gbs = alg / (kms * 1e-3) / 1e9
+    return {"rays": int(n), "kernel_ms": round(kms, 4), "mrays_s": round(n / (kms * 1e-3) / 1e6, 1),
+            "algorithmic_bytes": int(alg), "bytes_per_ray": round(alg / n, 1), "achieved_gbs": round(gbs, 1),
+            "frac_of_hbm": round(gbs / HBM_PEAK_GBS, 4),
+            "work": {"node_entries": counts[0], "leaves": counts[1], "tri_tests": counts[2], "rays_hit": counts[3]}}
+
+
+def small_scene_section(re, W, torch, ctx, cpu, steps=20):
+    """The small icosphere scene: 5,120 + 1,024 triangles, SAH leaf <= 8,
+    depth <= 24, 256x256 primary rays from (0,0,5); GPU closest hit against
+    the CPU oracle on the same rays."""
+    import oracle
+    tri_v, tree, o, d = W.small_scene()
+    scene = ctx.upload_scene([re.SceneObject(tree)])
+    n = len(o)
+    dev = torch.device("cuda:0")
+    d_o, d_d = torch.from_numpy(o).to(dev), torch.from_numpy(d).to(dev)
+    d_p = torch.empty(n, dtype=torch.int32, device=dev)
+    d_t = torch.empty(n, dtype=torch.float32, device=dev)
+    torch.cuda.synchronize()
+    counts = scene.trace_counts_device(d_o.data_ptr(), d_d.data_ptr(), n, d_p.data_ptr(), d_t.data_ptr())
+    kms = launch_stats(ctx, lambda s: scene.trace_closest_device(d_o.data_ptr(), d_d.data_ptr(), n, d_p.data_ptr(),
+                                                                 d_t.data_ptr(), sync=s), steps)
+    rec = part_record(n, counts, kms)
+    nodes, leaves, tris = tree.export()
+    obj = tree.identity_object()
+    po, to = oracle.trace(nodes, leaves, tris, obj, o, d, threads=cpu["threads"])
+    rec["bit_exact_vs_oracle"] = bool(np.array_equal(d_p.cpu().numpy(), po) and
+                                      np.array_equal(d_t.cpu().numpy().view(np.uint32), to.view(np.uint32)))
+    rec["workload"] = "icosphere (5,120 tris, 4x subdivided) + 1,024 random tris (seed 0), 256x256 rays from (0,0,5)"
+    rec["cpu_baseline"] = cpu_rate(lambda th: oracle.trace(nodes, leaves, tris, obj, o, d, threads=th), n,
+                                   "Mrays/s", cpu["threads"], f"all {n} rays, CPU oracle (full walks to the cap)")
+    scene.release()
+    return rec
 
 
 def kd_build_section(re, ctx, tri_v, host_tree, max_leaf, host_build_s, reps=3):
@@ -377,11 +292,7 @@ def shadow_section(W, torch, ctx, scene, tri_v, po, pd, pp, pt, order, steps=10)
                      ("closest_hit", lambda s: scene.trace_closest_device(d_o.data_ptr(), d_d.data_ptr(), n,
                                                                           d_p.data_ptr(), d_t.data_ptr(),
                                                                           sync=s, secondary=True))):
-        fn(True)
-        for _ in range(steps):
-            fn(False)
-        torch.cuda.synchronize()
-        kms = float(np.mean(ctx.kernel_times(steps)))
+        kms = launch_stats(ctx, fn, steps)
         out[name] = {"kernel_ms": round(kms, 4), "Mrays_per_s": round(n / (kms * 1e-3) / 1e6, 1)}
     occ = d_occ.cpu().numpy()
     out["occluded_frac"] = round(float(occ.mean()), 4)
@@ -389,50 +300,386 @@ def shadow_section(W, torch, ctx, scene, tri_v, po, pd, pp, pt, order, steps=10)
     return out
 
 
+# Terrain settings (hit depth error is measured against REFERENCE_DEPTH, the
+# fine exact-field march that tests/test_gpu_terrain.py pins to the FP64
+# first crossing within 0.02 voxel at the 99th percentile).
+REFERENCE_DEPTH = dict(k_exact=0.02, h_min_voxels=0.25, n_bisect=20, max_iters=1 << 16)
+TERRAIN_METHODS = {
+    # the reference sphere tracer (renderer.cu:142-235) at its own settings
+    "reference_march": ("plain", dict(step_scale=0.8, eps=1e-3, max_steps=256)),
+    # the same tracer at the smallest step and tolerance tried: its most accurate setting
+    "reference_march_fine": ("plain", dict(step_scale=0.02, eps=1e-5, max_steps=8192)),
+    # exact-field bracketing march (no network): the surrogate's ablation
+    "exact_march": ("exact", dict(k_exact=0.05, h_min_voxels=1.0, n_bisect=1)),
+    # surrogate far field + exact near field
+    "surrogate_hybrid": ("hybrid", dict(tau_voxels=4.0, margin_voxels=1.0, k_exact=0.1, n_bisect=4)),
+}
+MLP_FLOP_PER_EVAL = 2 * (99 * 64 + 3 * 64 * 64 + 64)
+# forward + backward (input and weight gradients) per training sample, about
+# three forward passes
+MLP_TRAIN_FLOP_PER_SAMPLE = 3 * MLP_FLOP_PER_EVAL
+NOISE_FLOP_PER_VOXEL = 5 * 70
+# Surrogate training: 2^22 samples (half within 2 voxels of the surface), Adam
+# lr 1e-3, batch 65,536, 20k steps.
+TRAIN_STEPS = 20000
+
+
+def terrain_cpu(cpu, o, d, grid32):
+    """CPU baselines of the terrain path: the host noise fill (oracle,
+    voxels/s) and the host reference sphere tracer on the same rays."""
+    import oracle
+    perm, _ = oracle.perlin_table(2016)
+    out = {}
+    slab = dict(dim=(256, 256, 16))
+    out["noise_fill"] = cpu_rate(lambda th: oracle.noise_fill(perm, threads=th), 256 ** 3, "Mvoxels/s",
+                                 cpu["threads"], "the whole 256^3 volume, CPU oracle (FP32, 5 octaves)",
+                                 one_core_fn=lambda th: oracle.noise_fill(perm, threads=th, **slab),
+                                 one_core_units=256 * 256 * 16,
+                                 one_core_what="z-slices 0-15 of the 256^3 volume (256x256x16 voxels)")
+    params = TERRAIN_METHODS["reference_march"][1]
+    n = len(o)
+    out["reference_march"] = cpu_rate(
+        lambda th: oracle.sdf_trace(grid32, (1 / 256,) * 3, o, d, threads=th, **params), n, "Mrays/s",
+        cpu["threads"], f"all {n} terrain rays, CPU oracle sphere trace (step 0.8, eps 1e-3, <= 256 steps)")
+    return out
+
+
+def terrain_fp64(o, d, grid32, t_gpu, h_gpu, cpu):
+    """The FP32 reference sphere trace (GPU) against its FP64 twin on the FP64
+    noise grid (CPU oracle), on every terrain ray."""
+    import oracle
+    perm, _ = oracle.perlin_table(2016)
+    g64 = oracle.noise_fill_f64(perm, threads=cpu["threads"])
+    params = TERRAIN_METHODS["reference_march"][1]
+    t64, h64, _ = oracle.sdf_trace_f64(g64, (1 / 256,) * 3, o, d, threads=cpu["threads"], **params)
+    both = (h64 == 1) & (h_gpu == 1)
+    e = np.abs(t_gpu[both].astype(np.float64) - t64[both])
+    gmax = float(np.abs(grid32.astype(np.float64) - g64).max())
+    return {"tracer": "reference sphere trace (step 0.8, eps 1e-3, <= 256 steps)", "rays_both_hit": int(both.sum()),
+            "hit_agreement": round(float(np.mean(h64 == h_gpu)), 6),
+            "abs_t_err_p50": float(np.median(e)), "abs_t_err_p99": float(np.percentile(e, 99)),
+            "abs_t_err_max": float(e.max()), "err_p99_voxels": round(float(np.percentile(e, 99)) * 256, 5),
+            "frac_within_1e-4": round(float(np.mean(e <= 1e-4)), 6), "grid_max_abs_err": gmax}
+
+
+def terrain_section(re, W, torch, ctx, cpu, with_cpu=True):
+    """Terrain path on one GPU: the 256^3 fBm volume (seed 2016) over the unit
+    cube, 1920x1080 rays from the terrain camera; noise generation, training,
+    network throughput, and hit depth error / Mrays/s of each tracer."""
+    dev = torch.device("cuda:0")
+    out = {"workload": "256^3 fBm terrain (5 octaves, seed 2016) on the unit cube, 1920x1080 rays"}
+    vol = ctx.noise_volume()
+    noise = []
+    for _ in range(3):
+        v2 = ctx.noise_volume()
+        noise.append(ctx.last_kernel_ms())
+        v2.release()
+    out["noise_ms"] = round(float(np.mean(noise)), 4)
+    out["noise_gvox_s"] = round(256 ** 3 / (np.mean(noise) * 1e-3) / 1e9, 2)
+    nsec = float(np.mean(noise)) * 1e-3
+    out["noise_roofline"] = {"bound": "valu", "flop_per_voxel": NOISE_FLOP_PER_VOXEL,
+                             "achieved_tflops": round(256 ** 3 * NOISE_FLOP_PER_VOXEL / nsec / 1e12, 2),
+                             "peak_tflops": FP32_PEAK_TFLOPS,
+                             "frac": round(256 ** 3 * NOISE_FLOP_PER_VOXEL / nsec / 1e12 / FP32_PEAK_TFLOPS, 4),
+                             "write_gbs": round(256 ** 3 * 4 / nsec / 1e9, 1)}
+    ev = ctx.noise_volume()
+    ev.edit("sphere_add", center=(0.5, 0.45, 0.5), extent=(0.1, 0, 0), material=1.0)
+    ems = []
+    for k in range(5):
+        ev.edit("sphere_add", center=(0.3 + 0.1 * k, 0.45, 0.5), extent=(0.1, 0, 0), material=1.0)
+        ems.append(ctx.last_kernel_ms())
+    ev.release()
+    em = float(np.mean(ems))
+    out["edit"] = {"kernel": "volume_edit_kernel", "kernel_ms": round(em, 4), "bound": "hbm",
+                   "achieved_gbs": round(256 ** 3 * 16 / (em * 1e-3) / 1e9, 1), "peak_gbs": HBM_PEAK_GBS,
+                   "frac": round(256 ** 3 * 16 / (em * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    dv = re.distance_volume(vol)
+    net = re.Surrogate(ctx, seed=1)
+    t0 = time.perf_counter()
+    hist = net.train(dv, n_samples=1 << 22, batch=65536, steps=TRAIN_STEPS, lr=1e-3, seed=1, over_weight=4.0)
+    step_ms = (time.perf_counter() - t0) * 1e3 / TRAIN_STEPS
+    train_tf = MLP_TRAIN_FLOP_PER_SAMPLE * 65536 / (step_ms * 1e-3) / 1e12
+    out["train"] = {"steps": TRAIN_STEPS, "batch": 65536, "ms_per_step": round(step_ms, 4),
+                    "loss_first": float(hist[:10].mean()), "loss_last": float(hist[-100:].mean()),
+                    "flop_per_sample": MLP_TRAIN_FLOP_PER_SAMPLE, "tflops": round(train_tf, 2),
+                    "frac_of_bf16_peak": round(train_tf / BF16_PEAK_TFLOPS, 4)}
+    npts = 1 << 22
+    pts = torch.rand(npts, 3, device=dev)
+    val = torch.empty(npts, device=dev)
+    torch.cuda.synchronize()
+    for _ in range(2):
+        net.forward_device(pts.data_ptr(), npts, val.data_ptr())
+    for _ in range(5):
+        net.forward_device(pts.data_ptr(), npts, val.data_ptr(), sync=False)
+    torch.cuda.synchronize()
+    fms = float(np.mean(ctx.kernel_times(5)))
+    tf = npts * MLP_FLOP_PER_EVAL / (fms * 1e-3) / 1e12
+    out["mlp_forward"] = {"kernel": "mlp_infer_kernel", "gevals_s": round(npts / (fms * 1e-3) / 1e9, 3),
+                          "tflops": round(tf, 2), "bound": "mfma", "peak_tflops": BF16_PEAK_TFLOPS,
+                          "frac": round(tf / BF16_PEAK_TFLOPS, 4), "flop_per_eval": MLP_FLOP_PER_EVAL}
+    o, d = W.terrain_rays(1920, 1080)
+    n = len(o)
+    do, dd = torch.from_numpy(o).to(dev), torch.from_numpy(d).to(dev)
+    dt = torch.empty(n, dtype=torch.float32, device=dev)
+    dh = torch.empty(n, dtype=torch.uint8, device=dev)
+    torch.cuda.synchronize()
+
+    def run(kind, params, sync=True):
+        if kind == "plain":
+            vol.trace_device(do.data_ptr(), dd.data_ptr(), n, dt.data_ptr(), dh.data_ptr(), sync=sync, **params)
+        else:
+            re.hybrid_trace_device(vol, net if kind == "hybrid" else None, do.data_ptr(), dd.data_ptr(), n,
+                                   dt.data_ptr(), dh.data_ptr(), sync=sync, **params)
+
+    run("exact", REFERENCE_DEPTH)
+    ref_t, ref_h = dt.cpu().numpy().astype(np.float64), dh.cpu().numpy()
+    vox = 1.0 / 256
+    methods = {}
+    ref_gpu = None
+    for name, (kind, params) in TERRAIN_METHODS.items():
+        run(kind, params)
+        t, h = dt.cpu().numpy().astype(np.float64), dh.cpu().numpy()
+        if name == "reference_march":
+            ref_gpu = (dt.cpu().numpy().copy(), h.copy())
+        both = (h == 1) & (ref_h == 1)
+        e = np.abs(t[both] - ref_t[both]) / vox
+        for _ in range(5):
+            run(kind, params, sync=False)
+        torch.cuda.synchronize()
+        ms = float(np.mean(ctx.kernel_times(5)))
+        if kind == "plain":
+            _, _, steps = vol.trace(o, d, **params)
+        else:
+            _, _, evals, steps = re.hybrid_trace(vol, net if kind == "hybrid" else None, o, d, **params)
+        alg = float(32 * n + 32 * np.sum(steps, dtype=np.int64))
+        roof = {"samples_per_ray": round(float(np.mean(steps)), 2),
+                "algorithmic_gbs": round(alg / (ms * 1e-3) / 1e9, 1),
+                "hbm_frac": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        if kind == "hybrid":
+            roof["network_evals_per_ray"] = round(float(np.mean(evals)), 2)
+            roof["mfma_tflops"] = round(float(np.sum(evals, dtype=np.int64)) * MLP_FLOP_PER_EVAL / (ms * 1e-3) / 1e12,
+                                        1)
+        methods[name] = {"mrays_s": round(n / (ms * 1e-3) / 1e6, 1), "kernel_ms": round(ms, 4), "roofline": roof,
+                         "err_p50_vox": round(float(np.median(e)), 4),
+                         "err_p99_vox": round(float(np.percentile(e, 99)), 4),
+                         "err_max_vox": round(float(e.max()), 3) if e.size else None,
+                         "frac_err_gt_half_vox": round(float(np.mean(e > 0.5)), 5),
+                         "hit_agree": round(float(np.mean(h == ref_h)), 5), "params": params}
+    out["methods"] = methods
+    hy, ex = methods["surrogate_hybrid"], methods["exact_march"]
+    # C3's ratio: the surrogate hybrid against its ablation, the exact-field
+    # march without the network (both within 0.5 voxel at p99)
+    out["surrogate_vs_exact_march"] = round(hy["mrays_s"] / ex["mrays_s"], 3)
+    out["surrogate_vs_exact_march_note"] = (
+        "the C3 ratio: surrogate hybrid Mrays/s / exact-field march Mrays/s (both meet p99 <= 0.5 voxel); the "
+        "reference sphere tracer at its own settings misses the accuracy bar (see methods.reference_march)")
+    # frame output of the reference tracer's frame (sky for misses, shaded hits, RGBA8)
+    dn = torch.empty(n * 3, dtype=torch.float32, device=dev)
+    dpx = torch.empty(n, dtype=torch.int32, device=dev)
+    torch.cuda.synchronize()
+    vol.trace_device(do.data_ptr(), dd.data_ptr(), n, dt.data_ptr(), dh.data_ptr(), nrm_ptr=dn.data_ptr(),
+                     **TERRAIN_METHODS["reference_march"][1])
+    fr = []
+    for _ in range(6):
+        re.frame_shade_device(ctx, dd.data_ptr(), dh.data_ptr(), dn.data_ptr(), 1920, 1080, dpx.data_ptr())
+        fr.append(ctx.last_kernel_ms())
+    fms = float(np.mean(fr[1:]))
+    out["frame"] = {"kernel": "frame_kernel", "kernel_ms": round(fms, 4), "mpixels_s": round(n / (fms * 1e-3) / 1e6, 1),
+                    "sky_pixels": int(n - int(dh.sum().item()))}
+    if with_cpu:
+        grid32 = vol.read()
+        try:
+            out["fp32_vs_fp64"] = terrain_fp64(o, d, grid32, ref_gpu[0], ref_gpu[1], cpu)
+        except Exception as e:
+            out["fp32_vs_fp64"] = {"error": f"{type(e).__name__}: {e}"}
+        try:
+            out["cpu_baseline"] = terrain_cpu(cpu, o, d, grid32)
+        except Exception as e:
+            out["cpu_baseline"] = {"error": f"{type(e).__name__}: {e}"}
+    for x in (net, dv, vol):
+        x.release()
+    return out
+
+
+def mixed_workload_section(re, W, torch, ctx, reps=3, n_streams=4):
+    """The many-scene mixed workload (workloads.batch_workload, seed 7: 64
+    triangle scenes of 1k-500k triangles with 16k-2M incoherent rays each, 16
+    noise-terrain tiles of 64^3-256^3 with their own rays; even tiles traced
+    by the exact-field march, odd tiles by the surrogate hybrid with a
+    network trained on that tile), every input resident in HBM, run
+    (a) as one persistent work-queue launch in a captured hipGraph
+    (re_batch_*: scenes and exact tiles in the queue, surrogate tiles as
+    hybrid launches in the same graph), leaf size, block and rays per work
+    item picked by autotune.autotune; (b) unbatched, one launch per scene or
+    tile in ascending ray count, on one stream and dealt round-robin over
+    n_streams contexts (rayengine_amd.mixed), leaf size and block picked by
+    autotune.tune_mixed.  Aggregate Mrays/s over the wall time; completion
+    times per scene and per tile."""
+    from rayengine_amd.autotune import autotune, batch_time_ms, tune_mixed
+    from rayengine_amd.mixed import MixedRun, TILE_HYBRID, TILE_MARCH
+    t0 = time.perf_counter()
+    soups, dims, seeds, ro, rd, sr, tr = W.batch_workload()
+    t_net = time.perf_counter()
+    tiles, nets, vols = [], [], []
+    for k, (dm, sd) in enumerate(zip(dims, seeds)):
+        v = ctx.noise_volume(dim=(dm, dm, dm), spacing=(1.0 / dm,) * 3, seed=sd)
+        vols.append(v)
+        if k % 2 == 1:
+            dv = re.distance_volume(v)
+            net = re.Surrogate(ctx, seed=1)
+            net.train(dv, n_samples=1 << 20, batch=65536, steps=2000, lr=1e-3, seed=1, over_weight=4.0)
+            tiles.append((dm, sd, net.params()[0]))
+            nets.append(net)
+            dv.release()
+        else:
+            tiles.append((dm, sd))
+            nets.append(None)
+    net_s = time.perf_counter() - t_net
+    n = len(ro)
+    n_scene_rays = int(sum(c for _, c in sr))
+    out = {"workload": "64 cluster-soup scenes (1k-500k triangles, log-uniform) with 16k-2M incoherent rays each "
+                       "(log-uniform), 16 noise-terrain tiles 64^3-256^3 (seeds 2016+k) with 16k-2M rays each; "
+                       "seed 7",
+           "triangles": int(sum(len(s) for s in soups)), "rays": int(n), "scene_rays": n_scene_rays,
+           "tile_rays": int(n - n_scene_rays),
+           "tiles_traced_with": "even tiles: exact-field march (k 0.05, h_min 1 voxel); odd tiles: surrogate hybrid "
+                                "(tau 4 voxels, margin 1, k 0.1, 4 bisections) with a network per tile, 2,000 Adam "
+                                "steps on 2^20 samples of its field",
+           "tile_networks_train_s": round(net_s, 2)}
+    # (a) the batched persistent launch in one captured graph
+    btiles = [(v, 0 if k % 2 == 0 else 2, b, c) + ((nets[k],) if nets[k] is not None else ())
+              for k, (v, (b, c)) in enumerate(zip(vols, tr))]
+    t_tune = time.perf_counter()
+    bcfg, bmeas = autotune(ctx, soups, btiles, ro, rd, sr, device_name="mi355x", cache=None, log=log,
+                           hybrid=TILE_HYBRID, march=TILE_MARCH)
+    trees = [re.KDTree(s, max_leaf_tris=bcfg["max_leaf_tris"], ctx=ctx) for s in soups]
+    batch = re.Batch(ctx, [(t, b, c) for t, (b, c) in zip(trees, sr)], btiles, ro, rd, item_rays=bcfg["item_rays"],
+                     block=bcfg["block"], hybrid=TILE_HYBRID, march=TILE_MARCH)
+    replay_ms = batch_time_ms(ctx, batch, 5)
+    walls = []
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        ts = time.perf_counter()
+        batch.run()
+        walls.append(time.perf_counter() - ts)
+    bwall = float(np.median(walls))
+    comp = batch.completion_ms()
+    bp, bt = batch.results()
+    info = {"items": batch.items(), "graph_replay_ms": round(replay_ms, 3), "wall_ms": round(bwall * 1e3, 3),
+            "mrays_s": round(n / (replay_ms * 1e-3) / 1e6, 1),
+            "mrays_s_wall": round(n / bwall / 1e6, 1),
+            "scene_completion_ms": {"p50": round(float(np.percentile(comp[:len(sr)], 50)), 3),
+                                    "p99": round(float(np.percentile(comp[:len(sr)], 99)), 3)},
+            "tile_completion_ms": {"p50": round(float(np.percentile(comp[len(sr):], 50)), 3),
+                                   "p99": round(float(np.percentile(comp[len(sr):], 99)), 3)},
+            "autotune": {"chosen": {k: bcfg[k] for k in ("max_leaf_tris", "block", "item_rays")},
+                         "seconds": round(time.perf_counter() - t_tune, 1),
+                         "measurements": bmeas}}
+    out["batched_graph"] = info
+    batch.release()
+    del trees
+    # (b) unbatched, one launch per item
+    ctxs = [ctx] + [re.Context(0) for _ in range(n_streams - 1)]
+    dev = torch.device("cuda:0")
+    d_o, d_d = torch.from_numpy(ro).to(dev), torch.from_numpy(rd).to(dev)
+    d_p = torch.empty(n, dtype=torch.int32, device=dev)
+    d_t = torch.empty(n, dtype=torch.float32, device=dev)
+    d_h = torch.empty(n, dtype=torch.uint8, device=dev)
+    ptrs = (d_o.data_ptr(), d_d.data_ptr(), d_p.data_ptr(), d_t.data_ptr(), d_h.data_ptr())
+    cfg, meas = tune_mixed(ctxs, soups, tiles, sr, tr, ptrs, torch.cuda.synchronize, log=log)
+    utrees = [re.KDTree(s, max_leaf_tris=cfg["max_leaf_tris"], ctx=ctx) for s in soups]
+    out["unbatched_autotune"] = {"chosen": {"max_leaf_tris": cfg["max_leaf_tris"], "block": cfg["block"]},
+                                 "issue": f"{n_streams} streams",
+                                 "wall_ms": {f"leaf{m['max_leaf_tris']}_block{m['block']}": m["wall_ms"]
+                                             for m in meas}}
+    first = None
+    for streams in (1, n_streams):
+        run = MixedRun(ctxs[:streams], utrees, tiles, sr, tr, *ptrs)
+        run.run(block=cfg["block"], variant=cfg["variant"])
+        torch.cuda.synchronize()
+        walls, comps = [], []
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            ts = time.perf_counter()
+            run.run(block=cfg["block"], variant=cfg["variant"])
+            torch.cuda.synchronize()
+            walls.append(time.perf_counter() - ts)
+            comps.append(run.completion_ms())
+        wall = float(np.median(walls))
+        done = np.mean(comps, axis=0)
+        sd_ = np.array([done[i] for i, it in enumerate(run.items) if it[0] == "scene"])
+        td = np.array([done[i] for i, it in enumerate(run.items) if it[0] == "tile"])
+        rec = {"wall_ms": round(wall * 1e3, 3), "mrays_s": round(n / wall / 1e6, 1),
+               "scene_completion_ms": {"p50": round(float(np.percentile(sd_, 50)), 3),
+                                       "p99": round(float(np.percentile(sd_, 99)), 3)},
+               "tile_completion_ms": {"p50": round(float(np.percentile(td, 50)), 3),
+                                      "p99": round(float(np.percentile(td, 99)), 3)}}
+        got = (d_p.cpu().numpy(), d_t.cpu().numpy().view(np.uint32), d_h.cpu().numpy())
+        if first is None:
+            first = got
+            # the batched results equal the unbatched ones ray for ray
+            hit_u = np.where(np.arange(n) < n_scene_rays, first[0], np.where(first[2] == 1, 0, -1))
+            out["batched_graph"]["identical_to_unbatched"] = bool(
+                np.array_equal(bp, hit_u) and np.array_equal(bt.view(np.uint32), first[1]))
+        else:
+            rec["identical_to_one_stream"] = all(np.array_equal(x, y) for x, y in zip(first, got))
+        out[f"unbatched_streams_{streams}"] = rec
+        run.release()
+    for c in ctxs[1:]:
+        c.close()
+    for x in nets + vols:
+        if x is not None:
+            x.release()
+    out["prep_and_tune_s"] = round(time.perf_counter() - t0, 1)
+    return out
+
+
+# --------------------------------------------------------------------------- main
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
     ap.add_argument("--no-terrain", action="store_true", help="skip the terrain section")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-workload section")
-    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
-                    help="only run 5 device copies of BYTES bytes (counter calibration)")
-    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 counter passes")
     ap.add_argument("--autotune", action="store_true",
                     help="pick leaf size, ray tile, block and variant by measurement before timing")
+    ap.add_argument("--traffic-probe", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--probe-leaf", type=int, default=8, help=argparse.SUPPRESS)
+    ap.add_argument("--probe-tile", type=int, default=8, help=argparse.SUPPRESS)
+    ap.add_argument("--probe-block", type=int, default=64, help=argparse.SUPPRESS)
+    ap.add_argument("--probe-variant", type=int, default=3, help=argparse.SUPPRESS)
     args = ap.parse_args()
-    if args.calibrate_copy:
-        # A device-to-device copy of known size: the reference point that turns
-        # the FETCH_SIZE / WRITE_SIZE counters into bytes.
-        import torch
-        n = args.calibrate_copy // 4
-        src = torch.ones(n, dtype=torch.float32, device="cuda:0")
-        dst = torch.empty_like(src)
-        for _ in range(5):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_bytes": n * 4, "launches": 5}), flush=True)
+    if args.traffic_probe:
+        traffic_probe(args)
         return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               f"(rays would shard across GPUs with no exchange between them)", file=sys.stderr)
         sys.exit(2)
 
+    tune = dict(DEFAULT_TUNE)
+    # counter passes first: child processes, before this process uses the GPU
+    traffic = None
+    if not args.no_traffic and not args.autotune:
+        log("traffic: two rocprofv3 counter passes over the headline launch")
+        traffic = measure_traffic(tune)
+        log(f"traffic: {traffic.get('bytes_per_launch', traffic.get('error'))}")
+
     import torch
     import rayengine_amd as re
     from rayengine_amd import workloads as W
 
+    cpu = cpu_info()
     tri_v = W.cluster_soup_default()
     ctx = re.Context(0)
     # Leaf size, ray-sort granularity (primary rays in tile x tile pixel
-    # blocks, bounces following their primaries: the order a tiled renderer
-    # issues them in), block and variant.  The defaults are within 1.3 % of
-    # the autotuner's best on this workload (leaf 4, same tile, block and
-    # variant), inside run-to-run noise; leaf 8 keeps the leaf bound of the
-    # small-scene configuration.  --autotune measures them again.
-    tune = {"max_leaf_tris": 8, "tile": 8, "block": 64, "variant": 3}
+    # blocks, bounces following their primaries), block and variant; the
+    # defaults are the autotuner's choice class on this workload, --autotune
+    # measures them again.
     tuned = None
     if args.autotune:
         from rayengine_amd.autotune import tune_trace
@@ -441,9 +688,9 @@ def main():
         tune = {k: tuned[k] for k in tune}
     t0 = time.time()
     tree = re.KDTree(tri_v, max_leaf_tris=tune["max_leaf_tris"])
-    log(f"scene: {tri_v.shape[0]} tris, tree {tree.info['n_leaves']} leaves, "
-        f"{tree.info['n_trirecs']} tri records, build {time.time() - t0:.1f} s")
     host_build_s = time.time() - t0
+    log(f"scene: {tri_v.shape[0]} tris, tree {tree.info['n_leaves']} leaves, "
+        f"{tree.info['n_trirecs']} tri records, build {host_build_s:.1f} s")
     kd_gpu = kd_build_section(re, ctx, tri_v, tree, tune["max_leaf_tris"], host_build_s)
     scene = ctx.upload_scene([re.SceneObject(tree)])
     po, pd = re.camera_rays(W.CLUSTER_CAMERA, 1920, 1080)
@@ -461,34 +708,36 @@ def main():
     d_p = torch.empty(n, dtype=torch.int32, device=dev)
     d_t = torch.empty(n, dtype=torch.float32, device=dev)
     torch.cuda.synchronize()
-    counts = scene.trace_counts_device(d_o.data_ptr(), d_d.data_ptr(), n, d_p.data_ptr(), d_t.data_ptr(), **tp)
+    o_ptr, d_ptr, p_ptr, t_ptr = d_o.data_ptr(), d_d.data_ptr(), d_p.data_ptr(), d_t.data_ptr()
+    counts = scene.trace_counts_device(o_ptr, d_ptr, n, p_ptr, t_ptr, **tp)
     alg_bytes = 32 * n + 8 * counts[0] + 56 * counts[1] + 40 * counts[2]
-    # the same walks in the reference's layout: a 128-byte node per node visit
-    # (inner or leaf), 52 bytes per triangle test (index + 3 float4 fetches)
     ref_layout_bytes = 32 * n + 128 * (counts[0] + counts[1]) + 52 * counts[2]
 
     for _ in range(args.warmup):
-        scene.trace_closest_device(d_o.data_ptr(), d_d.data_ptr(), n, d_p.data_ptr(), d_t.data_ptr(), sync=False,
-                                   **tp)
+        scene.trace_closest_device(o_ptr, d_ptr, n, p_ptr, t_ptr, sync=False, **tp)
     torch.cuda.synchronize()
     t_start = time.perf_counter()
     for _ in range(args.steps):
-        scene.trace_closest_device(d_o.data_ptr(), d_d.data_ptr(), n, d_p.data_ptr(), d_t.data_ptr(), sync=False,
-                                   **tp)
+        scene.trace_closest_device(o_ptr, d_ptr, n, p_ptr, t_ptr, sync=False, **tp)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t_start
     ktimes = ctx.kernel_times(args.steps)
     kms = float(np.mean(ktimes))
     achieved = alg_bytes / (kms * 1e-3) / 1e9
+    res_p, res_t = d_p.cpu().numpy(), d_t.cpu().numpy()
 
-    # HBM bytes per launch of the traversal kernel, from the FETCH_SIZE and
-    # WRITE_SIZE counter runs recorded in the profile summary (calibrated
-    # against a copy kernel of known size).
-    traffic = None
-    tfile = ROOT / "profiles" / "trace_closest_cluster_traffic.md"
-    if tfile.exists():
-        m = re_mod.search(r"hbm_bytes_per_launch:\s*([0-9]+)", tfile.read_text())
-        traffic = int(m.group(1)) if m else None
+    # the primary rays and the bounce rays as launches of their own
+    parts = {}
+    for name, (b, c) in (("primary", (0, n_primary)), ("bounce", (n_primary, n - n_primary))):
+        args_ = (o_ptr + 12 * b, d_ptr + 12 * b, c, p_ptr + 4 * b, t_ptr + 4 * b)
+        pc = scene.trace_counts_device(*args_, **tp)
+        pms = launch_stats(ctx, lambda s: scene.trace_closest_device(*args_, sync=s, **tp), 10)
+        parts[name] = part_record(c, pc, pms)
+    parts["bounce"]["note"] = ("the incoherent part: one cosine-weighted diffuse bounce per primary hit, from the "
+                               "hit point into the hemisphere about the facing geometric normal")
+    same = bool(np.array_equal(d_p.cpu().numpy(), res_p) and
+                np.array_equal(d_t.cpu().numpy().view(np.uint32), res_t.view(np.uint32)))
+    parts["results_equal_whole_launch"] = same
 
     value = n * args.steps / wall / 1e6
     res = {
@@ -506,22 +755,26 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"cluster scene: 2^20-triangle Gaussian-cluster soup (64 clusters, seed 1), "
                                f"SAH rope kd-tree leaf<={tune['max_leaf_tris']} depth<=24, 1920x1080 primary rays "
-                               f"({tune['tile']}x{tune['tile']} pixel tiles) + 1 diffuse bounce per hit, closest hit, "
-                               f"leaf-visit cap 50",
+                               f"({tune['tile']}x{tune['tile']} pixel tiles) from 0.8 units outside the cluster "
+                               f"nearest the origin + 1 diffuse bounce per hit, closest hit, leaf-visit cap 50",
+                   "camera": {"pos": list(W.CLUSTER_CAMERA.pos), "view": list(W.CLUSTER_CAMERA.view), "fov": [60, 60]},
                    "launch": {"block": tune["block"], "variant": tune["variant"], "autotuned": tuned is not None},
                    "rays": n, "primary_rays": n_primary, "bounce_rays": n - n_primary,
                    "triangles": int(tri_v.shape[0]), "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "trace_closest_o6_kernel" if tune["variant"] == 3 else
-                               ("trace_persistent_kernel" if tune["variant"] == 1 else "trace_closest_kernel"),
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic.get("bytes_per_launch") if traffic else None,
+                     "kernel": TRACE_KERNEL[tune["variant"]],
                      "kernel_ms": round(kms, 4),
                      "algorithmic_bytes": int(alg_bytes),
                      "reference_layout_bytes": int(ref_layout_bytes),
                      "reference_layout_gbs": round(ref_layout_bytes / (kms * 1e-3) / 1e9, 1),
                      "work": {"node_entries": counts[0], "leaves": counts[1], "tri_tests": counts[2],
                               "rays_hit": counts[3]}},
+        "parts": parts,
     }
+    if traffic is not None:
+        res["traffic_measurement"] = traffic
     # the same call with rays and results in pageable host memory (copied
     # through the context's staging buffer): the PCIe-inclusive rate of the
     # host-pointer boundary, reported beside `value`, never as it
@@ -536,6 +789,24 @@ def main():
                                 "bytes_over_pcie": int(32 * n)}
     res["kd_build"] = kd_gpu
     res["shadow"] = shadow_section(W, torch, ctx, scene, tri_v, po, pd, pp, pt, order)
+    if not args.no_cpu:
+        import oracle
+        nodes, leaves, tris = tree.export()
+        obj = tree.identity_object()
+        log(f"cpu baseline: oracle on {cpu['threads']} threads and 1 core")
+        res["cpu_baseline"] = cpu_rate(
+            lambda th: oracle.trace(nodes, leaves, tris, obj, ro, rd, threads=th), n, "Mrays/s", cpu["threads"],
+            f"the whole workload ({n} rays: {n_primary} primary + {n - n_primary} bounce), CPU oracle (FP32 "
+            f"restatement of the reference traversal, full walks to the cap), OpenMP dynamic/64")
+        res["cpu_baseline"]["cpu"] = cpu
+        po_, to_ = oracle.trace(nodes, leaves, tris, obj, ro, rd, threads=cpu["threads"])
+        res["bit_exact_vs_oracle_all_rays"] = bool(np.array_equal(po_, res_p) and
+                                                   np.array_equal(to_.view(np.uint32), res_t.view(np.uint32)))
+    try:
+        res["small_scene"] = small_scene_section(re, W, torch, ctx, cpu)
+    except Exception as e:
+        res["small_scene"] = {"error": f"{type(e).__name__}: {e}"}
+    scene.release()
     if not args.no_mixed:
         try:
             res["mixed_workload"] = mixed_workload_section(re, W, torch, ctx)
@@ -543,15 +814,9 @@ def main():
             res["mixed_workload"] = {"error": f"{type(e).__name__}: {e}"}
     if not args.no_terrain:
         try:
-            res["terrain"] = terrain_section(re, W, torch, ctx)
+            res["terrain"] = terrain_section(re, W, torch, ctx, cpu, with_cpu=not args.no_cpu)
         except Exception as e:  # the closest-hit line is still reported
             res["terrain"] = {"error": f"{type(e).__name__}: {e}"}
-    if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        res["cpu_baseline"] = cpu_baseline(tree, ro, rd, n_primary, threads)
-        one = cpu_baseline(tree, ro, rd, n_primary, 1, budget_cpu_s=8.0)
-        res["cpu_baseline"]["one_core"] = {"value": one["value"], "sample": one["sample"]}
-        res["cpu_baseline"]["cpu_model"] = _cpu_model()
     print(json.dumps(res), flush=True)
 
 

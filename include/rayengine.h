@@ -325,6 +325,13 @@ RE_API re_status re_volume_save_rsdf(re_ctx* ctx, int32_t vol_id, const char* pa
  * spacing], the point of the last voxel.  RE_ERR_IO if the file is missing,
  * short or longer than its header says. */
 RE_API re_status re_volume_load_rsdf(re_ctx* ctx, const char* path, int32_t* vol_id);
+/* Checks a .rsdf file without a context or a device: RE_OK with its dims and
+ * spacing when the header is valid (dims in [2,4096], spacing > 0) and the
+ * file holds exactly the 24-byte header and dim.x * dim.y * dim.z
+ * (distance, material) pairs; RE_ERR_IO otherwise.  The header is read as
+ * load_map reads it (renderer.cu:873-890); nothing of the size it claims is
+ * allocated before the file's length confirms it. */
+RE_API re_status re_rsdf_info(const char* path, int32_t dim[3], float spacing[3]);
 /* Reads the material channel back as a linear x-fastest grid (zeros if the
  * volume has none). */
 RE_API re_status re_volume_read_material(re_ctx* ctx, int32_t vol_id, float* linear);
@@ -483,8 +490,9 @@ typedef struct re_batch_config {
   int32_t block;     /* threads per block (64, 128, 256), default 256 */
   int32_t mlp_id;    /* network of surrogate tiles, -1 none */
   re_trace_params trace;
-  re_sdf_params sdf;
-  re_hybrid_params hybrid;
+  re_sdf_params sdf;     /* reference sphere-trace tiles (mode 1) */
+  re_hybrid_params hybrid; /* surrogate tiles (mode 2) */
+  re_hybrid_params march;  /* exact-march tiles (mode 0): k_exact, h_min_voxels, n_bisect, max_iters */
 } re_batch_config;
 
 RE_API void re_batch_default_config(re_batch_config* c);
@@ -518,6 +526,11 @@ RE_API re_status re_last_kernel_ms(re_ctx* ctx, float* ms);
 /* Durations of the last min(max, 256) kernel launches, oldest first (waits
  * for them); *count gets the number written. */
 RE_API re_status re_kernel_times(re_ctx* ctx, float* ms, int32_t max, int32_t* count);
+/* For the last k = min(max, 256) launches, oldest first: the time from the
+ * start of the first of them to the end of each (waits for them), so idle
+ * gaps and host issue delays between launches on the stream are included;
+ * *count gets k. */
+RE_API re_status re_launch_end_times(re_ctx* ctx, float* ms, int32_t max, int32_t* count);
 
 #ifdef __cplusplus
 }

@@ -210,6 +210,14 @@ class Context:
         check(lib().re_kernel_times(self._h, buf.ctypes.data_as(_lib._F), max_n, C.byref(k)), self._h)
         return buf[:k.value].copy()
 
+    def launch_end_times(self, max_n: int = 256) -> np.ndarray:
+        """For the last launches, oldest first: ms from the start of the first
+        of them to the end of each (HIP events; gaps between launches count)."""
+        buf = np.zeros(max_n, np.float32)
+        k = C.c_int32()
+        check(lib().re_launch_end_times(self._h, buf.ctypes.data_as(_lib._F), max_n, C.byref(k)), self._h)
+        return buf[:k.value].copy()
+
     # ------------------------------------------------------------ terrain
 
     def noise_volume(self, **kw) -> "Volume":
@@ -259,6 +267,15 @@ class Context:
             self.close()
         except Exception:
             pass
+
+
+def rsdf_info(path):
+    """(dim, spacing) of a well-formed .rsdf file (no device needed); raises
+    ReError for a missing, short, long or malformed file."""
+    dim = (C.c_int32 * 3)()
+    sp = (C.c_float * 3)()
+    check(lib().re_rsdf_info(str(path).encode(), dim, sp), None)
+    return tuple(dim), tuple(sp)
 
 
 def perlin_table(seed: int = 2016) -> np.ndarray:
@@ -511,7 +528,7 @@ class Batch:
     or `surrogate` when the tile names none)."""
 
     def __init__(self, ctx: Context, scenes, tiles, ray_o, ray_d, item_rays=256, block=256, surrogate=None,
-                 trace=None, sdf=None, hybrid=None):
+                 trace=None, sdf=None, hybrid=None, march=None):
         self.ctx = ctx
         o, d = _f32c(ray_o, 3), _f32c(ray_d, 3)
         self.n_rays = o.size // 3
@@ -534,6 +551,8 @@ class Batch:
             cfg.sdf = sdf_params(**sdf)
         if hybrid:
             cfg.hybrid = hybrid_params(**hybrid)
+        if march:
+            cfg.march = hybrid_params(**march)
         bid = C.c_int32()
         check(lib().re_batch_create(ctx.handle, sc, len(scenes), tl, len(tiles), ptr(o), ptr(d), self.n_rays,
                                     C.byref(cfg), C.byref(bid)), ctx.handle)

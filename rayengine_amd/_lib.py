@@ -107,7 +107,7 @@ class re_batch_tile(C.Structure):
 
 class re_batch_config(C.Structure):
     _fields_ = [("item_rays", C.c_int32), ("block", C.c_int32), ("mlp_id", C.c_int32), ("trace", re_trace_params),
-                ("sdf", re_sdf_params), ("hybrid", re_hybrid_params)]
+                ("sdf", re_sdf_params), ("hybrid", re_hybrid_params), ("march", re_hybrid_params)]
 
 
 _P = C.c_void_p
@@ -158,6 +158,7 @@ SIGNATURES = {
     "re_write_ppm": (C.c_int, [C.c_char_p, _P, C.c_int32, C.c_int32]),
     "re_volume_save_rsdf": (C.c_int, [_P, C.c_int32, C.c_char_p]),
     "re_volume_load_rsdf": (C.c_int, [_P, C.c_char_p, _I32]),
+    "re_rsdf_info": (C.c_int, [C.c_char_p, _I32, _F]),
     "re_volume_read_material": (C.c_int, [_P, C.c_int32, _P]),
     "re_brush_default": (None, [_P]),
     "re_volume_edit": (C.c_int, [_P, C.c_int32, _P]),
@@ -188,6 +189,7 @@ SIGNATURES = {
     "re_batch_release": (C.c_int, [_P, C.c_int32]),
     "re_last_kernel_ms": (C.c_int, [_P, _F]),
     "re_kernel_times": (C.c_int, [_P, _F, C.c_int32, _I32]),
+    "re_launch_end_times": (C.c_int, [_P, _F, C.c_int32, _I32]),
 }
 
 _lib: C.CDLL | None = None

@@ -89,6 +89,7 @@ RE_API const char* re_last_error(const re_ctx* ctx) {
 }
 
 RE_API re_status re_scene_upload(re_ctx* ctx, const re_object* objs, int32_t n_obj, int32_t* scene_id) {
+  return guarded(ctx, "re_scene_upload", [&]() -> re_status {
   if (!ctx || !objs || n_obj <= 0 || !scene_id) return set_err(ctx, RE_ERR_INVALID, "re_scene_upload: bad arguments");
   for (int32_t k = 0; k < n_obj; ++k)
     if (!objs[k].tree) return set_err(ctx, RE_ERR_INVALID, "re_scene_upload: object without a tree");
@@ -177,6 +178,7 @@ RE_API re_status re_scene_upload(re_ctx* ctx, const re_object* objs, int32_t n_o
   ctx->scenes[id] = std::move(s);
   *scene_id = id;
   return RE_OK;
+  });
 }
 
 RE_API re_status re_scene_update_objects(re_ctx* ctx, int32_t scene_id, const re_object* objs, int32_t n_obj) {
@@ -263,6 +265,22 @@ RE_API re_status re_kernel_times(re_ctx* ctx, float* ms, int32_t max, int32_t* c
     int64_t idx = (ctx->n_launch - k + i) % re_ctx::kRing;
     HIP_TRY(ctx, hipEventSynchronize(ctx->ring[2 * idx + 1]), "re_kernel_times sync");
     HIP_TRY(ctx, hipEventElapsedTime(&ms[i], ctx->ring[2 * idx], ctx->ring[2 * idx + 1]), "re_kernel_times");
+  }
+  *count = k;
+  return RE_OK;
+}
+
+RE_API re_status re_launch_end_times(re_ctx* ctx, float* ms, int32_t max, int32_t* count) {
+  if (!ctx || !count || max < 0 || (max > 0 && !ms)) return fail(RE_ERR_INVALID, "re_launch_end_times: bad arguments");
+  int64_t avail = ctx->n_launch < re_ctx::kRing ? ctx->n_launch : re_ctx::kRing;
+  int32_t k = (int32_t)(avail < max ? avail : max);
+  if (k > 0) {
+    const int64_t first = (ctx->n_launch - k) % re_ctx::kRing;
+    for (int32_t i = 0; i < k; ++i) {
+      int64_t idx = (ctx->n_launch - k + i) % re_ctx::kRing;
+      HIP_TRY(ctx, hipEventSynchronize(ctx->ring[2 * idx + 1]), "re_launch_end_times sync");
+      HIP_TRY(ctx, hipEventElapsedTime(&ms[i], ctx->ring[2 * first], ctx->ring[2 * idx + 1]), "re_launch_end_times");
+    }
   }
   *count = k;
   return RE_OK;

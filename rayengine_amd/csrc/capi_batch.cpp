@@ -43,11 +43,13 @@ RE_API void re_batch_default_config(re_batch_config* c) {
   re_trace_default_params(&c->trace);
   re_sdf_default_params(&c->sdf);
   re_hybrid_default_params(&c->hybrid);
+  re_hybrid_default_params(&c->march);
 }
 
 RE_API re_status re_batch_create(re_ctx* ctx, const re_batch_scene* scenes, int32_t n_scenes,
                                  const re_batch_tile* tiles, int32_t n_tiles, const float* ray_o, const float* ray_d,
                                  int64_t n_rays, const re_batch_config* config, int32_t* batch_id) {
+  return guarded(ctx, "re_batch_create", [&]() -> re_status {
   if (!ctx || !batch_id || n_scenes < 0 || n_tiles < 0 || n_scenes + n_tiles <= 0 || n_rays <= 0 || !ray_o || !ray_d ||
       (n_scenes > 0 && !scenes) || (n_tiles > 0 && !tiles))
     return fail(RE_ERR_INVALID, "re_batch_create: bad arguments");
@@ -78,11 +80,14 @@ RE_API re_status re_batch_create(re_ctx* ctx, const re_batch_scene* scenes, int3
   // single-workload entry points (re_trace_closest, re_sdf_trace, re_hybrid_trace).
   const re_sdf_params& sp = cfg.sdf;
   const re_hybrid_params& hy = cfg.hybrid;
+  const re_hybrid_params& mp = cfg.march;
   if (sp.max_steps < 1 || sp.max_steps > (1 << 20) || !(sp.step_rule == 0 || sp.step_rule == 1))
     return set_err(ctx, RE_ERR_INVALID, "re_batch_create: sdf max_steps in [1,2^20], step_rule 0/1");
-  if (hy.max_iters < 1 || hy.max_iters > (1 << 20) || hy.n_bisect < 0 || hy.n_bisect > 32 || !(hy.h_min_voxels > 0.0f) ||
-      !(hy.k_exact >= 0.0f))
-    return set_err(ctx, RE_ERR_INVALID, "re_batch_create: hybrid max_iters in [1,2^20], n_bisect in [0,32], h_min > 0");
+  for (const re_hybrid_params* q : {&hy, &mp})
+    if (q->max_iters < 1 || q->max_iters > (1 << 20) || q->n_bisect < 0 || q->n_bisect > 32 ||
+        !(q->h_min_voxels > 0.0f) || !(q->k_exact >= 0.0f))
+      return set_err(ctx, RE_ERR_INVALID,
+                     "re_batch_create: hybrid / march max_iters in [1,2^20], n_bisect in [0,32], h_min > 0");
   if (any_surrogate && !(hy.tau_voxels > hy.margin_voxels))
     return set_err(ctx, RE_ERR_INVALID, "re_batch_create: tau_voxels must exceed margin_voxels");
   HIP_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
@@ -117,7 +122,7 @@ RE_API re_status re_batch_create(re_ctx* ctx, const re_batch_scene* scenes, int3
       htiles[k].box_min[a] = v.box_min[a];
       htiles[k].box_max[a] = v.box_max[a];
     }
-    htiles[k].hp = tile_params(hy, v);
+    htiles[k].hp = tile_params(mp, v);
     b.tile_vol.push_back(tiles[k].vol_id);
     b.tile_mode.push_back(tiles[k].mode);
   }
@@ -249,6 +254,7 @@ RE_API re_status re_batch_create(re_ctx* ctx, const re_batch_scene* scenes, int3
   ctx->batches[id] = b;
   *batch_id = id;
   return RE_OK;
+  });
 }
 
 RE_API re_status re_batch_run(re_ctx* ctx, int32_t batch_id, uint32_t flags) {

@@ -222,29 +222,61 @@ RE_API re_status re_volume_save_rsdf(re_ctx* ctx, int32_t vol_id, const char* pa
   return RE_OK;
 }
 
+namespace {
+
+// Opens an .rsdf file and checks it before anything of the size its header
+// claims is allocated: a 24-byte header (int32 dims in [2,4096], float
+// spacing > 0), then exactly dim.x * dim.y * dim.z (distance, material) float
+// pairs.  On success the file is positioned at the first pair.
+re_status open_rsdf(re_ctx* ctx, const char* path, const char* who, FILE** out, int32_t dim[3], float spacing[3]) {
+  *out = nullptr;
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return set_err(ctx, RE_ERR_IO, std::string(who) + ": cannot open " + path);
+  bool ok = std::fread(dim, sizeof(int32_t), 3, f) == 3 && std::fread(spacing, sizeof(float), 3, f) == 3;
+  const char* why = nullptr;
+  if (!ok) why = "short header";
+  else if (!valid_dims(dim, spacing)) why = "header dims outside [2,4096] or spacing not > 0";
+  int64_t fsize = 0, want = 0;
+  if (!why) {
+    want = 24 + 8 * (int64_t)dim[0] * dim[1] * dim[2];
+    if (fseeko(f, 0, SEEK_END) != 0) why = "cannot seek";
+    else fsize = (int64_t)ftello(f);
+  }
+  if (!why && fsize < want) why = "file shorter than its header says";
+  if (!why && fsize > want) why = "file longer than its header says";
+  if (!why && fseeko(f, 24, SEEK_SET) != 0) why = "cannot seek";
+  if (why) {
+    std::fclose(f);
+    return set_err(ctx, RE_ERR_IO, std::string(who) + ": " + why);
+  }
+  *out = f;
+  return RE_OK;
+}
+
+}  // namespace
+
+RE_API re_status re_rsdf_info(const char* path, int32_t dim[3], float spacing[3]) {
+  if (!path || !dim || !spacing) return fail(RE_ERR_INVALID, "re_rsdf_info: null argument");
+  FILE* f = nullptr;
+  re_status st = open_rsdf(nullptr, path, "re_rsdf_info", &f, dim, spacing);
+  if (f) std::fclose(f);
+  return st;
+}
+
 RE_API re_status re_volume_load_rsdf(re_ctx* ctx, const char* path, int32_t* vol_id) {
   if (!ctx || !path || !vol_id) return fail(RE_ERR_INVALID, "re_volume_load_rsdf: null argument");
-  FILE* f = std::fopen(path, "rb");
-  if (!f) return set_err(ctx, RE_ERR_IO, std::string("re_volume_load_rsdf: cannot open ") + path);
+  return guarded(ctx, "re_volume_load_rsdf", [&]() -> re_status {
+  FILE* f = nullptr;
   int32_t dim[3];
   float spacing[3];
-  bool ok = std::fread(dim, sizeof(int32_t), 3, f) == 3 && std::fread(spacing, sizeof(float), 3, f) == 3;
-  if (!ok) {
-    std::fclose(f);
-    return set_err(ctx, RE_ERR_IO, "re_volume_load_rsdf: short header");
-  }
-  if (!valid_dims(dim, spacing)) {
-    std::fclose(f);
-    return set_err(ctx, RE_ERR_IO, "re_volume_load_rsdf: header dims outside [2,4096] or spacing not > 0");
-  }
+  re_status st0 = open_rsdf(ctx, path, "re_volume_load_rsdf", &f, dim, spacing);
+  if (st0) return st0;
   const int64_t n = (int64_t)dim[0] * dim[1] * dim[2];
   std::vector<float> pairs((size_t)(2 * n));
+  bool ok;
   ok = std::fread(pairs.data(), sizeof(float), pairs.size(), f) == pairs.size();
-  char extra;
-  const bool longer = ok && std::fread(&extra, 1, 1, f) == 1;
   std::fclose(f);
   if (!ok) return set_err(ctx, RE_ERR_IO, "re_volume_load_rsdf: file shorter than its header says");
-  if (longer) return set_err(ctx, RE_ERR_IO, "re_volume_load_rsdf: file longer than its header says");
   std::vector<float> dist((size_t)n), mat((size_t)n);
   for (int64_t i = 0; i < n; ++i) {
     dist[(size_t)i] = pairs[(size_t)(2 * i)];
@@ -272,6 +304,7 @@ RE_API re_status re_volume_load_rsdf(re_ctx* ctx, const char* path, int32_t* vol
   ctx->volumes[id] = v;
   *vol_id = id;
   return RE_OK;
+  });
 }
 
 RE_API re_status re_volume_read_material(re_ctx* ctx, int32_t vol_id, float* linear) {

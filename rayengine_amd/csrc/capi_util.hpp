@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <new>
+#include <string>
 
 #include "host/errors.hpp"
 #include "internal.hpp"
@@ -30,6 +32,20 @@ inline re_status hip_err(re_ctx* ctx, hipError_t e, const char* where) {
     hipError_t _e = (expr);                                \
     if (_e != hipSuccess) return re::hip_err(ctx, _e, where); \
   } while (0)
+
+// Runs an entry point's body; a host allocation that fails inside it (a
+// std::vector sized from the caller's arguments or a file header) becomes
+// RE_ERR_OOM instead of an exception crossing the C ABI.
+template <class F>
+re_status guarded(re_ctx* ctx, const char* what, F&& body) {
+  try {
+    return body();
+  } catch (const std::bad_alloc&) {
+    return set_err(ctx, RE_ERR_OOM, std::string(what) + ": host allocation failed");
+  } catch (const std::length_error&) {
+    return set_err(ctx, RE_ERR_OOM, std::string(what) + ": host allocation failed");
+  }
+}
 
 // Grows the context's staging buffer to at least `bytes`.
 inline re_status ensure_stage(re_ctx* ctx, size_t bytes) {

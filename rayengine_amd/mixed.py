@@ -6,8 +6,8 @@ are issued in ascending ray count and dealt round-robin over the given
 contexts, one HIP stream each, so that small launches overlap.  Inputs and
 outputs are device buffers given as raw pointers.
 
-This is the unbatched way to run the many-scene workload whose single
-persistent work-queue launch (re_batch_*) is switched off in this build.  No
+This is the unbatched way to run the many-scene workload; re_batch_* runs
+it as one persistent work-queue launch in a captured hipGraph.  No
 counterpart in the reference, which traced one scene per frame
 (RayEngine/src/cuda-src/renderer.cu:701-738).
 """
@@ -78,14 +78,16 @@ class MixedRun:
                                     sync=False, **(self.tile_hybrid if net is not None else self.tile_march))
 
     def completion_ms(self) -> np.ndarray:
-        """After a run has finished: per item (in self.items order) the device
-        time from the start of its stream's first launch to the end of its own
-        launch, from the HIP-event durations of each stream's launches."""
+        """After a run has finished (and before any other launch on these
+        contexts): per item (in self.items order) the device time from the
+        start of its stream's first launch to the end of its own launch,
+        from HIP events on that stream, idle gaps between launches
+        included."""
         done = np.zeros(len(self.items))
         for s, cx in enumerate(self.ctxs):
             mine = [i for i in range(len(self.items)) if self.owner[i] == s]
             if mine:
-                done[mine] = np.cumsum(cx.kernel_times(len(mine)))
+                done[mine] = cx.launch_end_times(len(mine))
         return done
 
     def kernel_ms_sum(self) -> float:

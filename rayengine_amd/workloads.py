@@ -5,8 +5,10 @@
                      256x256 pinhole rays from (0,0,5) towards the origin, FOV 60.
 * cluster_scene    - 2^20 triangles in 64 Gaussian clusters (sigma 0.3, centres
                      U[-10,10]^3, edge lengths LogNormal(-3, 0.5), seed 1);
-                     1920x1080 primary rays from (0,0,30) towards the origin plus
-                     one cosine-weighted diffuse bounce ray per primary hit.
+                     1920x1080 primary rays from 0.8 units outside the cluster
+                     nearest the origin, looking at its centre (95 % of the
+                     primaries hit), plus one cosine-weighted diffuse bounce ray
+                     per primary hit: 4.04 M rays, 49 % of them bounces.
 """
 from __future__ import annotations
 
@@ -15,7 +17,16 @@ import numpy as np
 from . import Camera, KDTree, bounce_rays, camera_rays, cluster_soup, icosphere, random_tris
 
 SMALL_CAMERA = Camera(pos=(0.0, 0.0, 5.0), view=(0.0, 0.0, -1.0))
-CLUSTER_CAMERA = Camera(pos=(0.0, 0.0, 30.0), view=(0.0, 0.0, -1.0))
+# The cluster workload's camera: the soup's cluster nearest the origin has its
+# centre near (-0.6805, 1.1570, 0.9132) (mean of the triangle centroids within
+# 1 unit of it); the camera stands 0.8 units further out along the line from
+# the origin through that centre and looks at it, so the cluster fills the
+# view: 1,964,416 of the 2,073,600 primaries hit, and the bounce rays leave a
+# dense cluster in every direction of the hemisphere (the incoherent part).
+CLUSTER_CAMERA = Camera(pos=(-1.016, 1.727, 1.363), view=(0.3353, -0.5701, -0.45))
+# The previous framing, from (0,0,30) towards the origin: 14 % of the
+# primaries hit, so bounces were 9 % of its 2.28 M rays.
+CLUSTER_CAMERA_FAR = Camera(pos=(0.0, 0.0, 30.0), view=(0.0, 0.0, -1.0))
 CLUSTER_TRIS = 1 << 20
 BOUNCE_SEED = 1
 
