@@ -49,9 +49,12 @@ FP32_PEAK_TFLOPS = 157.3
 BF16_PEAK_TFLOPS = 2500.0
 METRIC = "Mrays/s closest-hit; terrain Mrays/s surrogate vs exact SDF at ≤0.5-voxel err"
 TRACE_KERNEL = {0: "trace_closest_kernel", 1: "trace_persistent_kernel", 2: "trace_closest_kernel",
-                3: "trace_closest_o6_kernel"}
-DEFAULT_TUNE = {"max_leaf_tris": 8, "tile": 8, "block": 64, "variant": 3}
+                3: "trace_closest_o6_kernel", 4: "trace_coop_kernel"}
+DEFAULT_TUNE = {"max_leaf_tris": 8, "tile": 8, "block": 64, "variant": 4}
 COPY_PROBE_BYTES = 1 << 28
+# rays of the headline workload at the default tree (leaf 8): 2,073,600
+# primaries + 1,964,416 bounces; checked against the built workload below
+CLUSTER_RAYS = 4038016
 
 
 def log(*a):
@@ -118,8 +121,8 @@ def cpu_rate(fn, units, unit, threads, what, kind="port", one_core_fn=None, one_
 # --------------------------------------------------------------------------- traffic (rocprofv3)
 
 def _dispatch_counters(csv_dir: Path, counter: str):
-    """[(dispatch id, kernel name, value)] of one counter from rocprofv3's
-    counter_collection CSV, in dispatch order."""
+    """[(dispatch id, kernel name, grid size, value)] of one counter from
+    rocprofv3's counter_collection CSV, in dispatch order."""
     rows = {}
     for f in sorted(csv_dir.rglob("*counter_collection.csv")):
         with open(f, newline="") as fh:
@@ -128,12 +131,13 @@ def _dispatch_counters(csv_dir: Path, counter: str):
                     continue
                 key = int(r["Dispatch_Id"])
                 name = r.get("Kernel_Name", "")
-                rows.setdefault(key, [name, 0.0])
-                rows[key][1] += float(r["Counter_Value"])
-    return [(k, rows[k][0], rows[k][1]) for k in sorted(rows)]
+                grid = int(float(r.get("Grid_Size", 0) or 0))
+                rows.setdefault(key, [name, grid, 0.0])
+                rows[key][2] += float(r["Counter_Value"])
+    return [(k, rows[k][0], rows[k][1], rows[k][2]) for k in sorted(rows)]
 
 
-def measure_traffic(tune, timeout_s=150):
+def measure_traffic(tune, n_rays, timeout_s=150):
     """HBM-side bytes per launch of the closest-hit kernel on the headline
     workload, from two rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE) over
     `bench.py --traffic-probe`, each calibrated by a device copy of known size
@@ -158,12 +162,13 @@ def measure_traffic(tune, timeout_s=150):
         except subprocess.TimeoutExpired:
             return {"error": f"{counter} pass timed out after {timeout_s} s"}
         rows = _dispatch_counters(d, counter)
-        trace = [v for _, n, v in rows if kname in n]
-        copy = [v for _, n, v in rows if "copyBuffer" in n or "copy" in n.lower() and kname not in n]
+        whole = -(-n_rays // tune["block"]) * tune["block"]     # grid of the launch over every ray
+        trace = [v for _, n, g, v in rows if kname in n and g == whole]
+        copy = [v for _, n, g, v in rows if "copyBuffer" in n or ("copy" in n.lower() and kname not in n)]
         if len(trace) < 3 or len(copy) < 3:
             return {"error": f"{counter} pass: rc {r.returncode}, {len(trace)} trace and {len(copy)} copy dispatches "
                              f"found; stderr tail: {r.stderr[-300:]}"}
-        trace, copy = trace[-3:], copy[-3:]            # the probe's measured launches come last
+        trace, copy = trace[-3:], copy[-3:]            # the probe's measured launches
         factor = bytes_per_copy / float(np.mean(copy))  # bytes per counter unit
         b = float(np.mean(trace)) * factor
         total += b
@@ -178,8 +183,11 @@ def measure_traffic(tune, timeout_s=150):
 
 
 def traffic_probe(args):
-    """Child process of measure_traffic: the headline workload, the closest-hit
-    kernel launched 3 times, then 3 device copies of COPY_PROBE_BYTES."""
+    """Child process of measure_traffic (and the program the committed counter
+    profiles run): the headline workload; the closest-hit kernel launched 3
+    times over every ray, 3 times over the primary rays alone and 3 times over
+    the bounce rays alone (the three are told apart by their grid sizes); then
+    3 device copies of COPY_PROBE_BYTES."""
     import torch
     import rayengine_amd as re
     from rayengine_amd import workloads as W
@@ -194,9 +202,11 @@ def traffic_probe(args):
     d_p = torch.empty(n, dtype=torch.int32, device=dev)
     d_t = torch.empty(n, dtype=torch.float32, device=dev)
     torch.cuda.synchronize()
-    for _ in range(3):
-        scene.trace_closest_device(d_o.data_ptr(), d_d.data_ptr(), n, d_p.data_ptr(), d_t.data_ptr(),
-                                   block=args.probe_block, variant=args.probe_variant)
+    n_primary = 1920 * 1080
+    for b, c in ((0, n), (0, n_primary), (n_primary, n - n_primary)):
+        for _ in range(3):
+            scene.trace_closest_device(d_o.data_ptr() + 12 * b, d_d.data_ptr() + 12 * b, c, d_p.data_ptr() + 4 * b,
+                                       d_t.data_ptr() + 4 * b, block=args.probe_block, variant=args.probe_variant)
     src = torch.ones(COPY_PROBE_BYTES // 4, dtype=torch.float32, device=dev)
     dst = torch.empty_like(src)
     torch.cuda.synchronize()
@@ -666,7 +676,7 @@ def main():
     traffic = None
     if not args.no_traffic and not args.autotune:
         log("traffic: two rocprofv3 counter passes over the headline launch")
-        traffic = measure_traffic(tune)
+        traffic = measure_traffic(tune, CLUSTER_RAYS)
         log(f"traffic: {traffic.get('bytes_per_launch', traffic.get('error'))}")
 
     import torch
@@ -701,6 +711,8 @@ def main():
     n = len(ro)
     n_primary = len(po)
     log(f"rays: {n} ({n_primary} primary + {n - n_primary} bounce)")
+    if traffic is not None and n != CLUSTER_RAYS:
+        traffic = {"error": f"the counter passes assumed {CLUSTER_RAYS} rays; this tree gives {n}"}
 
     dev = torch.device("cuda:0")
     d_o = torch.from_numpy(ro).to(dev)

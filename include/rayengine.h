@@ -202,7 +202,12 @@ typedef struct re_trace_params {
   int32_t variant;         /* 0: one ray per thread, blocks in ray order;
                               1: persistent waves with per-lane ray refill;
                               2: one ray per thread, last block of rays first;
-                              3: as 2 with a 6-wave register budget (default).
+                              3: as 2 with a 6-wave register budget;
+                              4: as 3, each leaf visit done by the whole wave:
+                                 the triangles of all lanes' leaves fetched as
+                                 one coalesced list and tested one triangle
+                                 per lane, or, when every visiting lane is at
+                                 the same leaf, staged once in LDS (default).
                               Results are identical for every variant. */
 } re_trace_params;
 

@@ -584,7 +584,7 @@ class Batch:
             self.id = None
 
 
-def _trace_params(max_leaf_visits=50, one_sided=True, secondary=False, block=0, variant=3):
+def _trace_params(max_leaf_visits=50, one_sided=True, secondary=False, block=0, variant=4):
     return _lib.re_trace_params(int(max_leaf_visits), int(bool(one_sided)), int(bool(secondary)), int(block),
                                 int(variant))
 

@@ -254,7 +254,7 @@ RE_API void re_trace_default_params(re_trace_params* p) {
   p->one_sided = 1;
   p->secondary = 0;
   p->block = 0;
-  p->variant = 3;
+  p->variant = 4;
 }
 
 RE_API re_status re_kernel_times(re_ctx* ctx, float* ms, int32_t max, int32_t* count) {
@@ -312,7 +312,7 @@ re_status trace_common(re_ctx* ctx, int32_t scene_id, const float* ray_o, const 
   if (params) p = *params;
   if (!(p.block == 0 || p.block == 64 || p.block == 128 || p.block == 256))
     return set_err(ctx, RE_ERR_INVALID, "re_trace_closest: block must be 0, 64, 128 or 256");
-  if (p.variant < 0 || p.variant > 3) return set_err(ctx, RE_ERR_INVALID, "re_trace_closest: variant must be 0..3");
+  if (p.variant < 0 || p.variant > 4) return set_err(ctx, RE_ERR_INVALID, "re_trace_closest: variant must be 0..4");
   if (n == 0) return RE_OK;
   HIP_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
   TraceLaunch L;

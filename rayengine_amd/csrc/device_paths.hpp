@@ -23,18 +23,20 @@ __device__ __forceinline__ TriGeom load_geom(const TriRec* __restrict__ tp, uint
 //
 // ANY: stop at the first accepted hit (any-hit query; the reference's
 // intersection_detected becomes true there and never reverts).
-template <bool ONE_SIDED, bool COUNT, bool ANY = false>
+// BATCH records are loaded per round (4 by default; fewer where registers
+// are short).
+template <bool ONE_SIDED, bool COUNT, bool ANY = false, int BATCH = 4>
 __device__ __forceinline__ void test_leaf(const TriRec* __restrict__ tp, uint32_t count, f3 ro, f3 rd,
                                           float& t_exit, bool& found, float& hit_t, int32_t& hit_prim,
                                           unsigned& c_tri) {
-  for (uint32_t j = 0; j < count; j += 4) {
+  for (uint32_t j = 0; j < count; j += BATCH) {
     if (ANY && found) return;
-    TriGeom g[4];
+    TriGeom g[BATCH];
 #pragma unroll
-    for (int q = 0; q < 4; ++q)
+    for (int q = 0; q < BATCH; ++q)
       if (j + q < count) g[q] = load_geom(tp, j + q);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < BATCH; ++q) {
       if (j + q >= count) break;
       if (COUNT) ++c_tri;
       float t;
@@ -186,6 +188,261 @@ __device__ __forceinline__ void trace_ray(const NodeEntry* __restrict__ nodes, c
       best_obj = k;
     }
     if (ANY && found) return;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Cooperative variant: the wave stages its lanes' leaf triangles together.
+//
+// One ray per lane, as trace_closest_kernel, but every leaf visit is done by
+// the whole wave at once.  Each lane first walks down to its leaf (its own
+// descent, the leaf record, the exit rope), then the triangles of all the
+// lanes' leaves are laid end to end (an exclusive scan of the counts) and
+// tested one triangle per lane, 64 at a time: lane e loads triangle e of
+// that list - so a leaf's records are read by consecutive lanes, one
+// coalesced stream instead of one scattered 40-byte fetch per lane and
+// triangle - and tests it against the ray of the lane owning that leaf
+// (fetched with cross-lane reads).  A hit with t below the owner's t_exit is
+// folded into the owner's LDS slot with an atomic minimum of
+// (t bits << 32 | record index).  For t > 0 the float bits order like the
+// values, so the slot ends at the smallest t and, among equal t, the first
+// record: exactly the hit the sequential in-order loop keeps (a hit is taken
+// when t < t_exit and then lowers t_exit, so the last hit taken is the
+// smallest t, and a later equal t is never taken;
+// RayEngine/src/cuda-src/ray_functions.cuh:295-324).  Results are therefore
+// bit-identical to the other variants.  The rest of the walk (descent, slab
+// exit, rope, visit cap, repeated-state stop) is trace_ray's.
+// ---------------------------------------------------------------------------
+// Per-wave LDS of the cooperative leaf test: each visiting lane's ray, t_exit
+// and record base; the owner lane of every list entry (while the list fits);
+// each lane's hit slot.
+constexpr int kCoopMap = 512;
+// Triangle records one coherent round stages (in the ray area: 2 KB).
+constexpr int kCoopStage = 48;
+struct CoopWave {
+  float4 ray[64][2];             // {ro, t_exit}, {rd, base = first - offset as bits}
+  uint8_t owner[kCoopMap];       // list entry -> lane
+  unsigned long long slot[64];   // (t bits << 32 | record) of the lane's best hit
+};
+
+template <bool ONE_SIDED>
+__device__ __forceinline__ void coop_leaf_test(const TriRec* __restrict__ tris, bool at_leaf, uint32_t first,
+                                               uint32_t cnt, f3 ro, f3 rd, float& t_exit, bool& found, float& hit_t,
+                                               int32_t& hit_prim, CoopWave& W) {
+  const int lane = threadIdx.x & 63;
+  const unsigned long long at_mask = __ballot(at_leaf);
+  const int lead = __ffsll((long long)at_mask) - 1;
+  const uint32_t lead_first = (uint32_t)__builtin_amdgcn_readfirstlane((int)__shfl(first, lead));
+  const uint32_t lc = (uint32_t)__builtin_amdgcn_readfirstlane((int)__shfl(cnt, lead));
+  if (__ballot(at_leaf && (first != lead_first || cnt != lc)) == 0ull) {
+    // Every visiting lane tests the same run of records (the same leaf; an
+    // empty leaf may share its first index with another leaf, so the count is
+    // compared too): the run is staged once in LDS (one coalesced load, up to
+    // kCoopStage records a round) and each lane tests it in record order
+    // against its own ray - the sequential rule itself, read from LDS instead
+    // of one scattered fetch per lane.
+    TriRec* const stage = reinterpret_cast<TriRec*>(&W.ray[0][0]);
+#pragma unroll 1
+    for (uint32_t base = 0; base < lc; base += kCoopStage) {
+      const uint32_t m = lc - base < (uint32_t)kCoopStage ? lc - base : (uint32_t)kCoopStage;
+      if ((uint32_t)lane < m) stage[lane] = tris[lead_first + base + (uint32_t)lane];
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      if (at_leaf) {
+        for (uint32_t j = 0; j < m; ++j) {
+          const float* f = stage[j].v0;
+          float t;
+          const f3 v0{f[0], f[1], f[2]}, e1{f[3], f[4], f[5]}, e2{f[6], f[7], f[8]};
+          const bool h = ONE_SIDED ? ray_tri(ro, rd, v0, e1, e2, t) : ray_tri_2s(ro, rd, v0, e1, e2, t);
+          if (h && t < t_exit) {
+            found = true;
+            hit_t = t;
+            hit_prim = stage[j].prim;
+            t_exit = t;
+          }
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    return;
+  }
+  const uint32_t c = at_leaf ? cnt : 0u;
+  // inclusive scan of the counts over the wave
+  uint32_t incl = c;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t v = __shfl_up(incl, off);
+    if (lane >= off) incl += v;
+  }
+  const uint32_t excl = incl - c;
+  // in a scalar register, so the loops below exit on scalar branches taken by
+  // the whole wave (their bodies hold wave barriers)
+  const uint32_t total = (uint32_t)__builtin_amdgcn_readfirstlane((int)__shfl(incl, 63));
+  W.slot[lane] = ~0ull;
+  if (at_leaf) {
+    W.ray[lane][0] = make_float4(ro.x, ro.y, ro.z, t_exit);
+    W.ray[lane][1] = make_float4(rd.x, rd.y, rd.z, __uint_as_float(first - excl));
+  }
+  // The list is taken kCoopMap entries at a time: the owners write their
+  // lane into the entries of the window, then the wave tests the window's
+  // entries 64 at a time.
+#pragma unroll 1
+  for (uint32_t wb = 0; wb < total; wb += kCoopMap) {
+    const uint32_t we = total - wb < (uint32_t)kCoopMap ? total : wb + (uint32_t)kCoopMap;
+    if (at_leaf) {
+      const uint32_t k0 = excl > wb ? excl : wb, k1 = excl + c < we ? excl + c : we;
+      for (uint32_t k = k0; k < k1; ++k) W.owner[k - wb] = (uint8_t)lane;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll 1
+    for (uint32_t base = wb; base < we; base += 64) {
+      const uint32_t e = base + (uint32_t)lane;
+      if (e < we) {
+        const int L = W.owner[e - wb];
+        const float4 a = W.ray[L][0], b = W.ray[L][1];
+        const uint32_t rec = __float_as_uint(b.w) + e;
+        const TriGeom g = load_geom(tris, rec);
+        const f3 oro{a.x, a.y, a.z}, ord{b.x, b.y, b.z};
+        float t;
+        const bool h =
+            ONE_SIDED ? ray_tri(oro, ord, g.v0, g.e1, g.e2, t) : ray_tri_2s(oro, ord, g.v0, g.e1, g.e2, t);
+        if (h && t < a.w)
+          atomicMin(&W.slot[L], ((unsigned long long)__float_as_uint(t) << 32) | (unsigned long long)rec);
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+  if (at_leaf) {
+    const unsigned long long sv = W.slot[lane];
+    if (sv != ~0ull) {
+      found = true;
+      hit_t = __uint_as_float((uint32_t)(sv >> 32));
+      hit_prim = tris[(uint32_t)sv].prim;
+      t_exit = hit_t;
+    }
+  }
+}
+
+// The closest hit of every lane's ray over objects [k0, k1) (k0 + 1 with
+// ONE_OBJ), the wave walking together: each lane makes its own descent, leaf
+// record read, exit and rope step (trace_ray's walk, same visit cap and
+// repeated-state stop), and every leaf visit's triangle tests are done by
+// coop_leaf_test for all visiting lanes at once.  Lanes with has_ray false take
+// part in the wave's work without a ray of their own.  Results are
+// bit-identical to trace_ray's.  Every lane of the wave must call it (it
+// holds wave-wide operations); the loop ends on a wave-uniform branch.
+template <bool COUNT, bool ONE_SIDED, bool ONE_OBJ>
+__device__ __forceinline__ void coop_trace_wave(const NodeEntry* __restrict__ nodes,
+                                                const DevLeaf* __restrict__ leaves, const TriRec* __restrict__ tris,
+                                                const ObjXform* __restrict__ objs, int32_t k0, int32_t k1,
+                                                bool has_ray, f3 o, f3 d, int32_t cap, bool secondary, float& best_t,
+                                                int32_t& best_prim, int32_t& best_obj, RayCounts& rc, CoopWave& W) {
+  const int32_t k_end = ONE_OBJ ? k0 + 1 : k1;
+  best_t = kInf;
+  best_prim = -1;
+  best_obj = -1;
+  int32_t k_obj = k0 - 1;
+  bool live = false, open = false, done = !has_ray;
+  f3 ro{0, 0, 0}, rd{0, 0, 0}, fr{0, 0, 0};
+  float t_entry = 0.0f, t_exit = 0.0f, hit_t = kInf;
+  int32_t hit_prim = -1, cur = 0, visits = 0;
+  bool found = false, have_window = false;
+  uint4 w = make_uint4(0u, 0u, 0u, 0u);
+  WalkMemo memo;
+  // starts the next object whose root box the ray enters (trace_ray's object
+  // loop); a walk that cannot take a first visit closes at once with no hit
+  auto start_next = [&]() {
+    while (!live && k_obj + 1 < k_end) {
+      ++k_obj;
+      const ObjXform ob = objs[k_obj];
+      xform_ray(ob, o, d, secondary, ro, rd);
+      fr = f3{1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z};
+      if (!ray_box_inv(ob.root_min, ob.root_max, ro, fr, t_entry, t_exit)) continue;
+      cur = ob.root;
+      found = false;
+      hit_t = kInf;
+      hit_prim = -1;
+      visits = 0;
+      have_window = false;
+      memo = WalkMemo{};
+      live = t_entry < t_exit && visits < cap;
+      open = live;
+    }
+    if (!live) done = true;
+  };
+  if (!done) start_next();
+#pragma unroll 1
+  for (;;) {
+    // a lane between walks closes the last one and, with more objects,
+    // starts the next
+    if (!live && !done) {
+      if (open && found && hit_t < best_t) {
+        best_t = hit_t;
+        best_prim = hit_prim;
+        best_obj = k_obj;
+      }
+      open = false;
+      if (ONE_OBJ) done = true;
+      else start_next();
+    }
+    if (__ballot(!done) == 0ull) break;
+    bool at_leaf = false, box_ok = false;
+    uint32_t first = 0, cnt = 0;
+    float tf = 0.0f;
+    int32_t r = kNoRope;
+    if (live) {
+      if (memo.repeats(cur, t_entry, t_exit)) {
+        live = false;
+      } else {
+        ++visits;
+        const f3 pe = add(ro, smul(t_entry, rd));
+        uint32_t leaf_id;
+        if (ref_is_leaf(cur)) {
+          leaf_id = ref_leaf_id(cur);
+        } else {
+          if (!have_window) w = load_window(nodes, cur);
+          leaf_id = descend<COUNT>(nodes, cur, w, pe, rc.ent);
+        }
+        const DevLeaf* lp = leaves + leaf_id;
+        const float4 h0 = *reinterpret_cast<const float4*>(lp->bmin);
+        const float4 h1 = *reinterpret_cast<const float4*>(lp->bmax + 1);
+        const float bmin[3] = {h0.x, h0.y, h0.z}, bmax[3] = {h0.w, h1.x, h1.y};
+        first = __float_as_uint(h1.z);
+        cnt = __float_as_uint(h1.w);
+        if (COUNT) ++rc.leaf;
+        float tn;
+        box_ok = ray_box_inv(bmin, bmax, ro, fr, tn, tf);
+        const int face = box_ok ? exit_face(bmin, bmax, add(ro, smul(tf, rd))) : -1;
+        r = face >= 0 ? lp->rope[face] : kNoRope;
+        have_window = r >= 0;
+        if (have_window) w = load_window(nodes, r);
+        at_leaf = true;
+      }
+    }
+    if (__ballot(at_leaf) != 0ull) {
+      coop_leaf_test<ONE_SIDED>(tris, at_leaf, first, cnt, ro, rd, t_exit, found, hit_t, hit_prim, W);
+      if (COUNT && at_leaf) rc.tri += cnt;
+    }
+    if (at_leaf) {
+      if (!box_ok) {
+        live = false;
+      } else {
+        t_entry = tf;
+        if (r == kNoRope) {
+          live = false;
+        } else {
+          cur = r;
+          live = t_entry < t_exit && visits < cap;
+        }
+      }
+    }
   }
 }
 

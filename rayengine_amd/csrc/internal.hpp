@@ -55,7 +55,7 @@ struct TraceLaunch {
   int32_t* obj;
   TraceCounts* counts;  // device pointer or null
   uint8_t* occluded;    // any-hit query: 1 when some object detects a hit
-  int32_t variant;      // 0 persistent with ray refill, 1 one ray per thread
+  int32_t variant;      // re_trace_params.variant
   int64_t grid_blocks;  // persistent grid size (blocks)
   unsigned long long* queue;  // persistent ray counter, zeroed before launch
 };

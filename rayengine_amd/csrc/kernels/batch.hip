@@ -64,25 +64,40 @@ __device__ __forceinline__ int32_t wave_fetch_sub(int32_t* p, int32_t sub, int l
 
 template <bool ONE_SIDED>
 __global__ void __launch_bounds__(256) batch_kernel(BatchLaunch L) {
+  // one CoopWave per wave of the block (dynamic LDS sized by the launch)
+  extern __shared__ CoopWave waves[];
+  CoopWave& wv = waves[threadIdx.x >> 6];
   const int lane = threadIdx.x & 63;
   uint32_t* const queue = reinterpret_cast<uint32_t*>(L.ctl);  // ctl[0], low word
   for (;;) {
     const uint32_t it = wave_fetch_add(queue, 1u, lane);
     if ((int64_t)it >= L.n_items) break;  // scalar: `it` is wave-uniform
     const BatchItem item = L.items[it];
-    for (int r = lane; r < item.count; r += 64) {
-      const int64_t i = item.begin + r;
-      const f3 o{L.ray_o[3 * i], L.ray_o[3 * i + 1], L.ray_o[3 * i + 2]};
-      const f3 d{L.ray_d[3 * i], L.ray_d[3 * i + 1], L.ray_d[3 * i + 2]};
-      if (item.kind == kItemTri) {
+    if (item.kind == kItemTri) {
+      // a scene's rays, 64 at a time, walked by the whole wave together
+      // (the cooperative traversal of the single-scene kernel's default)
+      for (int r0 = 0; r0 < item.count; r0 += 64) {
+        const int r = r0 + lane;
+        const bool has = r < item.count;
+        const int64_t i = item.begin + (has ? r : 0);
+        const f3 o{L.ray_o[3 * i], L.ray_o[3 * i + 1], L.ray_o[3 * i + 2]};
+        const f3 d{L.ray_d[3 * i], L.ray_d[3 * i + 1], L.ray_d[3 * i + 2]};
         float bt;
         int32_t bp, bo;
         RayCounts rc;
-        trace_ray<false, ONE_SIDED>(L.nodes, L.leaves, L.tris, L.objs, item.target, item.target + 1, o, d, L.cap,
-                                    L.secondary != 0, bt, bp, bo, rc);
-        L.prim[i] = bp;
-        L.t[i] = bt;
-      } else {
+        coop_trace_wave<false, ONE_SIDED, true>(L.nodes, L.leaves, L.tris, L.objs, item.target, item.target + 1, has,
+                                                o, d, L.cap, L.secondary != 0, bt, bp, bo, rc, wv);
+        if (has) {
+          L.prim[i] = bp;
+          L.t[i] = bt;
+        }
+      }
+    }
+    for (int r = lane; item.kind != kItemTri && r < item.count; r += 64) {
+      const int64_t i = item.begin + r;
+      const f3 o{L.ray_o[3 * i], L.ray_o[3 * i + 1], L.ray_o[3 * i + 2]};
+      const f3 d{L.ray_d[3 * i], L.ray_d[3 * i + 1], L.ray_d[3 * i + 2]};
+      {
         const BatchTile& tile = L.tiles[item.target];
         float t;
         int32_t n;
@@ -110,7 +125,9 @@ int64_t batch_grid(int block) {
   int dev = 0, cus = 0, per_cu = 0;
   if (hipGetDevice(&dev) != hipSuccess) return 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel<true>, block, 0) != hipSuccess) return 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel<true>, block,
+                                                    (size_t)(block / 64) * sizeof(CoopWave)) != hipSuccess)
+    return 0;
   return (int64_t)cus * (per_cu > 0 ? per_cu : 1);
 }
 
@@ -139,10 +156,11 @@ hipError_t launch_batch(const BatchLaunch& L, hipStream_t s) {
   int64_t grid = L.grid_blocks > 0 ? L.grid_blocks : 1024;
   const int64_t want = (L.n_items * 64 + block - 1) / block;
   if (grid > want) grid = want;
+  const size_t lds = (size_t)(block / 64) * sizeof(CoopWave);
   if (L.one_sided)
-    hipLaunchKernelGGL(batch_kernel<true>, dim3((unsigned)grid), dim3(block), 0, s, L);
+    hipLaunchKernelGGL(batch_kernel<true>, dim3((unsigned)grid), dim3(block), lds, s, L);
   else
-    hipLaunchKernelGGL(batch_kernel<false>, dim3((unsigned)grid), dim3(block), 0, s, L);
+    hipLaunchKernelGGL(batch_kernel<false>, dim3((unsigned)grid), dim3(block), lds, s, L);
   return hipGetLastError();
 }
 

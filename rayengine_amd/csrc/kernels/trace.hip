@@ -20,7 +20,10 @@ namespace {
 // dispatched work-group: when costly rays sit at the end of the array (the
 // secondary rays a renderer appends after its primaries) they start first and
 // the cheap ones fill in behind them instead of forming the kernel's tail.
-template <bool COUNT, bool ONE_SIDED, bool REVERSE>
+// ONE_OBJ: the scene holds one object, so the object loop runs once and the
+// untransformed ray and the running best hit need no registers of their own
+// (the scenes the benchmarks trace; multi-object scenes take ONE_OBJ false).
+template <bool COUNT, bool ONE_SIDED, bool REVERSE, bool ONE_OBJ = false>
 __device__ __forceinline__ void closest_body(const NodeEntry* __restrict__ nodes, const DevLeaf* __restrict__ leaves,
                                              const TriRec* __restrict__ tris, const ObjXform* __restrict__ objs,
                                              int32_t n_obj, const float* __restrict__ ray_o,
@@ -36,8 +39,8 @@ __device__ __forceinline__ void closest_body(const NodeEntry* __restrict__ nodes
   float best_t;
   int32_t best_prim, best_obj;
   RayCounts rc;
-  trace_ray<COUNT, ONE_SIDED>(nodes, leaves, tris, objs, 0, n_obj, o, d, cap, secondary != 0, best_t, best_prim,
-                              best_obj, rc);
+  trace_ray<COUNT, ONE_SIDED>(nodes, leaves, tris, objs, 0, ONE_OBJ ? 1 : n_obj, o, d, cap, secondary != 0, best_t,
+                              best_prim, best_obj, rc);
   prim_out[i] = best_prim;
   t_out[i] = best_t;
   if (obj_out) obj_out[i] = best_obj;
@@ -64,10 +67,10 @@ __global__ void __launch_bounds__(256) trace_closest_kernel(RE_CLOSEST_PARAMS) {
 }
 
 // The same with the register budget held to 80 VGPRs (6 waves per SIMD).
-template <bool COUNT, bool ONE_SIDED>
+template <bool COUNT, bool ONE_SIDED, bool ONE_OBJ>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6)))
 trace_closest_o6_kernel(RE_CLOSEST_PARAMS) {
-  closest_body<COUNT, ONE_SIDED, true>(RE_CLOSEST_ARGS);
+  closest_body<COUNT, ONE_SIDED, true, ONE_OBJ>(RE_CLOSEST_ARGS);
 }
 
 // Any-hit query (trace_shadow, RayEngine/src/cuda-src/ray_functions.cuh:365-374):
@@ -230,6 +233,38 @@ __global__ void __launch_bounds__(256) trace_persistent_kernel(
   }
 }
 
+// Cooperative variant (see coop_trace_wave): one ray per lane, last block of
+// rays first, 6-wave register budget.
+template <bool COUNT, bool ONE_SIDED, bool ONE_OBJ>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6)))
+trace_coop_kernel(RE_CLOSEST_PARAMS) {
+  // one CoopWave per wave of the block (dynamic LDS sized by the launch)
+  extern __shared__ CoopWave waves[];
+  CoopWave& wv = waves[threadIdx.x >> 6];
+  const int64_t blk = (int64_t)(gridDim.x - 1 - blockIdx.x);  // last block of rays first
+  const int64_t i = blk * blockDim.x + threadIdx.x;
+  const bool has_ray = i < n;
+  const int64_t ic = has_ray ? i : 0;
+  const f3 o{ray_o[3 * ic], ray_o[3 * ic + 1], ray_o[3 * ic + 2]};
+  const f3 d{ray_d[3 * ic], ray_d[3 * ic + 1], ray_d[3 * ic + 2]};
+  float best_t;
+  int32_t best_prim, best_obj;
+  RayCounts rc;
+  coop_trace_wave<COUNT, ONE_SIDED, ONE_OBJ>(nodes, leaves, tris, objs, 0, ONE_OBJ ? 1 : n_obj, has_ray, o, d, cap,
+                                             secondary != 0, best_t, best_prim, best_obj, rc, wv);
+  if (has_ray) {
+    prim_out[i] = best_prim;
+    t_out[i] = best_t;
+    if (obj_out) obj_out[i] = best_obj;
+    if (COUNT) {
+      atomicAdd(&counts->entries, (unsigned long long)rc.ent);
+      atomicAdd(&counts->leaves, (unsigned long long)rc.leaf);
+      atomicAdd(&counts->tris, (unsigned long long)rc.tri);
+      if (best_prim >= 0) atomicAdd(&counts->rays_hit, 1ull);
+    }
+  }
+}
+
 }  // namespace
 
 int64_t trace_persistent_grid(int block) {
@@ -285,8 +320,24 @@ hipError_t launch_trace_closest(const TraceLaunch& L, hipStream_t s) {
       else RE_LAUNCH(K, false, false __VA_ARGS__);                        \
     }                                                                     \
   } while (0)
-  if (L.variant == 2) RE_LAUNCH_ALL(trace_closest_kernel, , true);
-  else if (L.variant == 3) RE_LAUNCH_ALL(trace_closest_o6_kernel);
+  if (L.variant == 4) {
+#define RE_LAUNCH_COOP(C, O, ONE)                                                                             \
+  hipLaunchKernelGGL((trace_coop_kernel<C, O, ONE>), dim3((unsigned)grid), dim3(block),                       \
+                     (block / 64) * sizeof(CoopWave), s, S.nodes, S.leaves,                                      \
+                     S.tris, S.objs, S.n_obj, L.ray_o, L.ray_d, L.n, cap, L.secondary, L.prim, L.t, L.obj,        \
+                     L.counts)
+    const bool one = S.n_obj == 1;
+    if (L.counts) {
+      if (L.one_sided) { if (one) RE_LAUNCH_COOP(true, true, true); else RE_LAUNCH_COOP(true, true, false); }
+      else { if (one) RE_LAUNCH_COOP(true, false, true); else RE_LAUNCH_COOP(true, false, false); }
+    } else {
+      if (L.one_sided) { if (one) RE_LAUNCH_COOP(false, true, true); else RE_LAUNCH_COOP(false, true, false); }
+      else { if (one) RE_LAUNCH_COOP(false, false, true); else RE_LAUNCH_COOP(false, false, false); }
+    }
+#undef RE_LAUNCH_COOP
+  } else if (L.variant == 2) RE_LAUNCH_ALL(trace_closest_kernel, , true);
+  else if (L.variant == 3 && S.n_obj == 1) RE_LAUNCH_ALL(trace_closest_o6_kernel, , true);
+  else if (L.variant == 3) RE_LAUNCH_ALL(trace_closest_o6_kernel, , false);
   else RE_LAUNCH_ALL(trace_closest_kernel, , false);
 #undef RE_LAUNCH_ALL
 #undef RE_LAUNCH

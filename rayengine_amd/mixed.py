@@ -66,7 +66,7 @@ class MixedRun:
             self.release()
             raise
 
-    def run(self, block: int = 64, variant: int = 3):
+    def run(self, block: int = 64, variant: int = 4):
         """Issues every item's launch (asynchronous on its context's stream)."""
         o_ptr, d_ptr, p_ptr, t_ptr, h_ptr = self.ptrs
         for (kind, _, b, c), obj, net in zip(self.items, self.objs, self.nets):

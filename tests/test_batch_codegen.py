@@ -7,9 +7,10 @@ __shfl(it, 0)"; the compiler turned that loop into an inner loop that lanes
 the launch never finished.  The item index is now taken with every lane in
 the atomic and read back with readfirstlane into a scalar register.  This test
 compiles the kernel and checks the properties that make the loop terminate:
-no cross-lane shuffle in the kernel, the item index scaled by the item record
-size (24 bytes) in scalar arithmetic, and the loop exit taken on a comparison
-of that scalar index with the item count.
+the counter's atomic result read back with readfirstlane, no cross-lane
+shuffle between the atomic and the item record, and the item record addressed
+from that scalar index (the index scaled by the 24-byte record size in scalar
+arithmetic).
 """
 import re
 import shutil
@@ -43,9 +44,14 @@ def _kernel_asm(tmp_path, name_part):
 
 def test_batch_kernel_item_loop_is_wave_uniform(tmp_path):
     for name, body in _kernel_asm(tmp_path, "batch_kernel").items():
-        assert "ds_bpermute" not in body and "ds_permute" not in body, f"{name}: cross-lane shuffle in the item loop"
         # the item record address is formed from a scalar index (BatchItem is 24 bytes)
-        assert re.search(r"s_mul_i32\s+s\d+, s\d+, 24", body), f"{name}: item index is not in a scalar register"
+        m = re.search(r"s_mul_i32\s+s\d+, s\d+, 24", body)
+        assert m, f"{name}: item index is not in a scalar register"
         # the counter value is broadcast with readfirstlane right after the atomic
         atom = body.index("global_atomic_add")
+        assert atom < m.start(), f"{name}: item record addressed before the counter is read"
         assert "v_readfirstlane_b32" in body[atom:atom + 400], f"{name}: atomic result not read back uniformly"
+        # and no cross-lane shuffle stands between the atomic and the item
+        # record (where the first version read the index with a shuffle)
+        between = body[atom:m.start()]
+        assert "ds_bpermute" not in between and "ds_permute" not in between, f"{name}: shuffle of the item index"

@@ -25,7 +25,7 @@ def test_tune_trace_small_scene(gpu_ctx):
 
     logs = []
     cfg, meas = AT.tune_trace(gpu_ctx, tri_v, make_rays, 128, 128, leaf_sizes=(4, 8), tiles=(1, 8),
-                              blocks=(64, 256), variants=(2, 3), reps=2, cache=None, log=logs.append)
+                              blocks=(64, 256), variants=(3, 4), reps=2, cache=None, log=logs.append)
     print(cfg, flush=True)
     sys.stdout.flush()
     assert len(meas) == 16 and len(logs) == 16

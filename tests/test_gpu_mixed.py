@@ -52,7 +52,7 @@ def test_mixed_run_matches_oracle_and_single_calls(gpu_ctx):
         for rep in range(2):
             torch.cuda.synchronize()          # the fills above run on torch's stream
             t0 = time.perf_counter()
-            run.run(block=64, variant=3)
+            run.run(block=64, variant=4)
             torch.cuda.synchronize()
             wall_ms = (time.perf_counter() - t0) * 1e3
             # read the run's event times before any other launch on these contexts

@@ -52,13 +52,13 @@ def test_two_sided_and_block_shapes(gpu_ctx, small):
     scene = gpu_ctx.upload_scene([re.SceneObject(tree)])
     po, to = oracle.trace(*tree.export(), tree.identity_object(), o, d, two_sided=True)
     for block in (64, 128, 256):
-        for variant in (0, 1, 2, 3):
+        for variant in (0, 1, 2, 3, 4):
             say(f"two-sided, block {block}, variant {variant}")
             p, t = scene.trace_closest(o, d, one_sided=False, block=block, variant=variant)
             _parity(p, t, po, to)
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4])
 def test_counters_match_oracle(gpu_ctx, small, variant):
     tri_v, tree, o, d = small
     scene = gpu_ctx.upload_scene([re.SceneObject(tree)])
@@ -87,7 +87,7 @@ def test_multi_object_transforms(gpu_ctx):
     o, d = re.camera_rays(re.Camera(pos=(0.3, 0.5, 6.0), view=(0.0, -0.1, -1.0)), 160, 120)
     for secondary in (False, True):
         po, to = oracle.trace(nodes, leaves, tris, obs, o, d, secondary=secondary)
-        for variant in (0, 1, 2, 3):
+        for variant in (0, 1, 2, 3, 4):
             say(f"multi-object, secondary={secondary}, variant {variant}")
             p, t, ob = scene.trace_closest(o, d, with_obj=True, secondary=secondary, variant=variant)
             _parity(p, t, po, to)
@@ -112,7 +112,7 @@ def test_degenerate_rays(gpu_ctx, small):
     d[512:1024] = -0.0
     d[512:1024, 2] = -1.0
     po, to = oracle.trace(nodes, leaves, tris, tree.identity_object(), o, d)
-    for variant in (0, 1):
+    for variant in (0, 1, 4):
         p, t = scene.trace_closest(o, d, variant=variant)
         _parity(p, t, po, to)
 
@@ -152,9 +152,10 @@ def test_device_pointer_path(gpu_ctx, small):
     assert np.array_equal(tp.cpu().numpy(), p) and np.array_equal(tt.cpu().numpy().view(np.uint32), t.view(np.uint32))
 
 
-def test_cluster_scene_subset_bit_exact(gpu_ctx, cluster):
-    """Full-size cluster workload on the GPU; a seeded subset of 65536 rays
-    (primary and bounce) checked against the oracle."""
+def test_cluster_scene_every_ray_bit_exact(gpu_ctx, cluster):
+    """The full cluster workload (2^20 triangles, 1920x1080 primaries and one
+    bounce per hit, about 4 M rays): every ray's prim id and t from every
+    kernel variant equal the CPU oracle's walk."""
     tri_v, tree = cluster
     scene = gpu_ctx.upload_scene([re.SceneObject(tree)])
     o, d = re.camera_rays(W.CLUSTER_CAMERA, 1920, 1080)
@@ -163,15 +164,20 @@ def test_cluster_scene_subset_bit_exact(gpu_ctx, cluster):
     ro, rd = W.cluster_rays(tri_v, p, t)
     say(f"cluster all rays: {len(ro)}")
     pa, ta = scene.trace_closest(ro, rd)
-    pv, tv = scene.trace_closest(ro, rd, variant=1)
-    _parity(pa, ta, pv, tv)
     assert np.array_equal(pa[:len(o)], p) and np.array_equal(ta[:len(o)].view(np.uint32), t.view(np.uint32))
-    rng = np.random.default_rng(11)
-    sel = np.concatenate([rng.choice(len(o), 32768, replace=False),
-                          len(o) + rng.choice(len(ro) - len(o), 32768, replace=False)])
     nodes, leaves, tris = tree.export()
-    po, to = oracle.trace(nodes, leaves, tris, tree.identity_object(), ro[sel], rd[sel])
-    _parity(pa[sel], ta[sel], po, to)
+    say("oracle over every ray")
+    po, to = oracle.trace(nodes, leaves, tris, tree.identity_object(), ro, rd)
+    _parity(pa, ta, po, to)
+    for variant in (0, 1, 2, 4):
+        say(f"cluster all rays, variant {variant}")
+        pv, tv = scene.trace_closest(ro, rd, variant=variant)
+        _parity(pv, tv, po, to)
+    # work counters of the whole launch against the oracle's shortened walks
+    _, _, co = oracle.trace(nodes, leaves, tris, tree.identity_object(), ro, rd, counts=True, cut=True)
+    for variant in (3, 4):
+        _, _, cnt = scene.trace_counts(ro, rd, variant=variant)
+        assert cnt[:3] == [int(x) for x in co[:, :3].sum(0)], f"variant {variant} counters"
     # work counters of the whole launch against the oracle's shortened walks
     _, _, cnt = scene.trace_counts(ro, rd)
     _, _, co = oracle.trace(nodes, leaves, tris, tree.identity_object(), ro, rd, counts=True, cut=True)
