@@ -222,6 +222,24 @@ RE_API re_status re_trace_closest(re_ctx* ctx, int32_t scene_id, const float* ra
                                   const float* ray_d, int64_t n, const re_trace_params* params,
                                   int32_t* prim_id, float* t, int32_t* obj_id, uint32_t flags);
 
+/* re_trace_closest that allocates nothing.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * work counters (counts, may be NULL), the persistent variant's ray counter
+ * and the staging of host-pointer rays and results live in a fixed device
+ * area of the library (about 9 MB per GPU, held until the process ends),
+ * cleared by a kernel on every call; rays and results are the caller's.
+ * Host pointers: at most re_trace_closest_noalloc_max_rays() rays
+ * (RE_ERR_INVALID beyond); device pointers (RE_DEVICE_PTRS): any n.
+ * Results equal re_trace_closest's (and counts re_trace_counts').  Every call
+ * on one GPU shares that GPU's area: calls that overlap on one GPU (two
+ * contexts or two host threads) race. */
+RE_API re_status re_trace_closest_noalloc(re_ctx* ctx, int32_t scene_id, const float* ray_o,
+                                          const float* ray_d, int64_t n, const re_trace_params* params,
+                                          int32_t* prim_id, float* t, int32_t* obj_id, uint64_t counts[4],
+                                          uint32_t flags);
+RE_API int64_t re_trace_closest_noalloc_max_rays(void);
+
 /* Same traversal, also summing its work counters: counts[0] node entries
  * read, counts[1] leaf records read, counts[2] triangle records tested,
  * counts[3] rays with a hit.  These give the algorithmic bytes of a launch:

@@ -146,6 +146,9 @@ SIGNATURES = {
     "re_trace_counts": (C.c_int, [_P, C.c_int32, _P, _P, C.c_int64, C.POINTER(re_trace_params), _P, _P, _U64,
                                   C.c_uint32]),
     "re_trace_occluded": (C.c_int, [_P, C.c_int32, _P, _P, C.c_int64, C.POINTER(re_trace_params), _P, C.c_uint32]),
+    "re_trace_closest_noalloc": (C.c_int, [_P, C.c_int32, _P, _P, C.c_int64, C.POINTER(re_trace_params), _P, _P, _P,
+                                           _U64, C.c_uint32]),
+    "re_trace_closest_noalloc_max_rays": (C.c_int64, []),
     "re_perlin_table": (C.c_int, [C.c_uint32, _P]),
     "re_noise_default_params": (None, [C.POINTER(re_noise_params)]),
     "re_noise_generate": (C.c_int, [_P, C.POINTER(re_noise_params), _I32]),

@@ -49,6 +49,11 @@ RE_API re_status re_create(int device, re_ctx** out) {
     re_destroy(ctx);
     return st;
   }
+  // the device address of the fixed area of re_trace_closest_noalloc on this device
+  if (re_status st = noalloc_setup(ctx)) {
+    re_destroy(ctx);
+    return st;
+  }
   *out = ctx;
   return RE_OK;
 }
@@ -298,10 +303,14 @@ RE_API re_status re_last_kernel_ms(re_ctx* ctx, float* ms) {
 namespace re {
 
 // Shared body of the closest-hit entry points.  counts (host pointer, may be
-// null) receives the summed work counters.
+// null) receives the summed work counters.  With sa given, the work counters,
+// the persistent variant's ray counter and the staging of host-pointer calls
+// live in that fixed device area (cleared by a fill kernel), and nothing is
+// allocated; otherwise they use the context's buffers (the staging buffer
+// grows on demand).
 re_status trace_common(re_ctx* ctx, int32_t scene_id, const float* ray_o, const float* ray_d, int64_t n,
                        const re_trace_params* params, int32_t* prim_id, float* t, int32_t* obj_id,
-                       uint32_t flags, TraceCounts* counts) {
+                       uint32_t flags, TraceCounts* counts, const ScratchArea* sa) {
   if (!ctx) return fail(RE_ERR_INVALID, "re_trace_closest: null ctx");
   if (n < 0 || (n > 0 && (!ray_o || !ray_d || !prim_id || !t)))
     return set_err(ctx, RE_ERR_INVALID, "re_trace_closest: bad arguments");
@@ -324,7 +333,8 @@ re_status trace_common(re_ctx* ctx, int32_t scene_id, const float* ray_o, const 
   L.block = p.block;
   L.counts = nullptr;
   L.variant = p.variant;
-  L.queue = (unsigned long long*)((char*)ctx->counts_buf + 64);
+  char* const header = sa ? (char*)sa->base : (char*)ctx->counts_buf;
+  L.queue = (unsigned long long*)(header + 64);
   {
     const int blk = p.block > 0 ? p.block : 256;
     int64_t& g = ctx->persistent_grid[blk == 64 ? 0 : (blk == 128 ? 1 : 2)];
@@ -336,9 +346,15 @@ re_status trace_common(re_ctx* ctx, int32_t scene_id, const float* ray_o, const 
   char* base = nullptr;
   if (!dev) {
     size_t need = 2 * ray_bytes + (size_t)n * 12 + 256 + 64;
-    re_status st = ensure_stage(ctx, need);
-    if (st) return st;
-    base = (char*)ctx->stage;
+    if (sa) {
+      if (need > sa->bytes - kScratchHeader)
+        return set_err(ctx, RE_ERR_INVALID, "host-pointer rays exceed the fixed staging area");
+      base = (char*)sa->base + kScratchHeader;
+    } else {
+      re_status st = ensure_stage(ctx, need);
+      if (st) return st;
+      base = (char*)ctx->stage;
+    }
     float* d_o = (float*)base;
     float* d_d = (float*)(base + ray_bytes);
     int32_t* d_p = (int32_t*)(base + 2 * ray_bytes);
@@ -360,11 +376,16 @@ re_status trace_common(re_ctx* ctx, int32_t scene_id, const float* ray_o, const 
   }
   TraceCounts* d_counts = nullptr;
   if (counts) {
-    d_counts = (TraceCounts*)ctx->counts_buf;
-    HIP_TRY(ctx, hipMemsetAsync(d_counts, 0, sizeof(TraceCounts), ctx->stream), "counts clear");
+    d_counts = (TraceCounts*)header;
     L.counts = d_counts;
   }
-  if (L.variant == 1) HIP_TRY(ctx, hipMemsetAsync(L.queue, 0, sizeof(unsigned long long), ctx->stream), "queue clear");
+  if (sa) {
+    // counters (bytes 0-31) and ray counter (64-71) of the fixed area start at zero
+    HIP_TRY(ctx, launch_zero_words((uint32_t*)header, 18, ctx->stream), "scratch clear");
+  } else {
+    if (counts) HIP_TRY(ctx, hipMemsetAsync(d_counts, 0, sizeof(TraceCounts), ctx->stream), "counts clear");
+    if (L.variant == 1) HIP_TRY(ctx, hipMemsetAsync(L.queue, 0, sizeof(unsigned long long), ctx->stream), "queue clear");
+  }
   HIP_TRY(ctx, ctx->launch_begin(), "event");
   HIP_TRY(ctx, launch_trace_closest(L, ctx->stream), "trace_closest launch");
   HIP_TRY(ctx, ctx->launch_end(), "event");
@@ -387,7 +408,7 @@ extern "C" {
 RE_API re_status re_trace_closest(re_ctx* ctx, int32_t scene_id, const float* ray_o, const float* ray_d, int64_t n,
                                   const re_trace_params* params, int32_t* prim_id, float* t, int32_t* obj_id,
                                   uint32_t flags) {
-  return trace_common(ctx, scene_id, ray_o, ray_d, n, params, prim_id, t, obj_id, flags, nullptr);
+  return trace_common(ctx, scene_id, ray_o, ray_d, n, params, prim_id, t, obj_id, flags, nullptr, nullptr);
 }
 
 RE_API re_status re_trace_counts(re_ctx* ctx, int32_t scene_id, const float* ray_o, const float* ray_d, int64_t n,
@@ -395,7 +416,7 @@ RE_API re_status re_trace_counts(re_ctx* ctx, int32_t scene_id, const float* ray
                                  uint32_t flags) {
   if (!counts) return set_err(ctx, RE_ERR_INVALID, "re_trace_counts: counts is null");
   TraceCounts c{0, 0, 0, 0};
-  re_status st = trace_common(ctx, scene_id, ray_o, ray_d, n, params, prim_id, t, nullptr, flags, &c);
+  re_status st = trace_common(ctx, scene_id, ray_o, ray_d, n, params, prim_id, t, nullptr, flags, &c, nullptr);
   counts[0] = c.entries;
   counts[1] = c.leaves;
   counts[2] = c.tris;

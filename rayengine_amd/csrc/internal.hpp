@@ -65,6 +65,18 @@ struct TraceLaunch {
 int64_t trace_persistent_grid(int block);
 
 hipError_t launch_trace_closest(const TraceLaunch& L, hipStream_t s);
+// Zeroes n 32-bit words with a kernel (no memset node when a caller captures).
+hipError_t launch_zero_words(uint32_t* p, int32_t n, hipStream_t s);
+
+// A fixed device area standing in for the allocated buffers of a call: work
+// counters at byte 0, the persistent ray counter at byte 64, host-pointer
+// staging from kScratchHeader on.
+constexpr size_t kScratchHeader = 256;
+struct ScratchArea {
+  void* base;
+  size_t bytes;
+};
+
 hipError_t launch_trace_occluded(const TraceLaunch& L, hipStream_t s);
 
 // A volume resident in HBM: 8x8x8 Morton-ordered bricks.
@@ -303,6 +315,7 @@ struct re_ctx {
   std::map<int32_t, re::DevBatch> batches;
   // Staging for host-pointer calls, grown on demand.
   void* counts_buf = nullptr;  // 256 bytes: work counters at 0, ray queue counter at 64
+  void* noalloc_area = nullptr;  // this device's fixed area of re_trace_closest_noalloc (capi_noalloc.cpp)
   int64_t persistent_grid[3] = {0, 0, 0};  // per block size 64, 128, 256
   void* stage = nullptr;
   size_t stage_bytes = 0;
@@ -315,3 +328,14 @@ struct re_ctx {
     return e;
   }
 };
+
+namespace re {
+// Resolves the device address of the fixed area of re_trace_closest_noalloc
+// on the context's device (current), once per device, into ctx->noalloc_area.
+re_status noalloc_setup(re_ctx* ctx);
+
+// The closest-hit entry points' shared body (capi.cpp).
+re_status trace_common(re_ctx* ctx, int32_t scene_id, const float* ray_o, const float* ray_d, int64_t n,
+                       const re_trace_params* params, int32_t* prim_id, float* t, int32_t* obj_id,
+                       uint32_t flags, TraceCounts* counts, const ScratchArea* sa);
+}  // namespace re

@@ -265,7 +265,18 @@ trace_coop_kernel(RE_CLOSEST_PARAMS) {
   }
 }
 
+__global__ void zero_words_kernel(uint32_t* __restrict__ p, int32_t n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
 }  // namespace
+
+hipError_t launch_zero_words(uint32_t* p, int32_t n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(zero_words_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n);
+  return hipGetLastError();
+}
 
 int64_t trace_persistent_grid(int block) {
   int dev = 0, cus = 0, per_cu = 0;
