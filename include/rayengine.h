@@ -222,6 +222,27 @@ RE_API re_status re_trace_closest(re_ctx* ctx, int32_t scene_id, const float* ra
                                   const float* ray_d, int64_t n, const re_trace_params* params,
                                   int32_t* prim_id, float* t, int32_t* obj_id, uint32_t flags);
 
+/* Per-triangle hit attributes of a scene, the per-vertex data of
+ * copy_memory's normals and uvs (RayEngine/src/cuda-src/cuda_memory_functions.cuh:58-211;
+ * packed per triangle as RTScene::init_triangles does, RayEngine/src/World/RTScene.cpp:243-257):
+ * normals holds n0, n1, n2 (9 floats) per prim id, uvs uv0, uv1, uv2 (6
+ * floats) per prim id or NULL; n_prims covers every prim id of the scene.
+ * Host arrays, copied. */
+RE_API re_status re_scene_set_attributes(re_ctx* ctx, int32_t scene_id, const float* normals, const float* uvs,
+                                         int64_t n_prims);
+/* Hit attributes of traced rays (prim_id and obj_id as re_trace_closest
+ * returned them; obj_id may be NULL for a one-object scene; params: the
+ * trace's one_sided and secondary): normal (3 floats per ray) = the vertex
+ * normals interpolated at the hit and normalised (gpu_get_tri_normal,
+ * RayEngine/src/cuda-src/ray_functions.cuh:146-155); uv (2 per ray) = the
+ * vertex uvs interpolated (get_uvs, :157-166), or the barycentrics when the
+ * scene has no uvs (:206-213); bary (2 per ray) = the barycentrics (u, v) of
+ * the hit; hit_point (3 per ray) = ro + t rd in object space (:204).  Every
+ * output may be NULL.  Misses get zeros. */
+RE_API re_status re_hit_attributes(re_ctx* ctx, int32_t scene_id, const float* ray_o, const float* ray_d, int64_t n,
+                                   const re_trace_params* params, const int32_t* prim_id, const int32_t* obj_id,
+                                   float* normal, float* uv, float* bary, float* hit_point, uint32_t flags);
+
 /* re_trace_closest that allocates nothing.  No counterpart in the
  * reference: for callers that cannot allocate device memory (code under HIP
  * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
@@ -277,6 +298,9 @@ typedef struct re_noise_params {
   float gain;         /* default 0.5 */
   float base_freq;    /* default 4.0 (= 4 / extent) */
   float y_offset;     /* default 0.4 */
+  float origin[3];    /* field position of voxel (0,0,0), default 0: voxel (x,y,z) samples the
+                         field at origin + (x,y,z) * spacing.  cuda_update_chunk_gen passes the
+                         chunk's location here (renderer.cu:982-1000) */
 } re_noise_params;
 
 RE_API void re_noise_default_params(re_noise_params* p);
@@ -289,6 +313,12 @@ RE_API re_status re_noise_generate(re_ctx* ctx, const re_noise_params* params, i
  * (RayEngine/src/cuda-src/sphere_tracing.cuh:617-755). */
 RE_API re_status re_volume_upload(re_ctx* ctx, const int32_t* dim, const float* spacing,
                                   const float* linear, int32_t* vol_id);
+/* Uploads a chunk's grid as linear x-fastest (distance, material) float
+ * pairs, 2 * dim.x*dim.y*dim.z floats, keeping the material channel.
+ * Replaces cuda_update_chunk's upload of the chunk's Grid voxels
+ * (RayEngine/src/cuda-src/sphere_tracing.cuh:565-615).  Box [0, dim * spacing]. */
+RE_API re_status re_volume_upload_pairs(re_ctx* ctx, const int32_t* dim, const float* spacing,
+                                        const float* pairs, int32_t* vol_id);
 /* Reads a volume back as a linear x-fastest grid (save_map's D2H copy,
  * RayEngine/src/cuda-src/renderer.cu:838-870). */
 RE_API re_status re_volume_read(re_ctx* ctx, int32_t vol_id, float* linear);

@@ -25,6 +25,7 @@ _SIGS = {
     "orc_occluded": (None, [_P, _P, _P, _P, C.c_int, _P, _P, C.c_int64, C.c_int, C.c_int, C.c_int, _P, C.c_int]),
     "orc_brute_f64": (None, [_P, C.c_int64, _P, _P, C.c_int64, C.c_int, _P, _P, C.c_int]),
     "orc_ray_tri": (C.c_int, [_P, _P, _P, _P, _P, C.c_int, _P, _P]),
+    "orc_hit_attributes": (None, [_P, _P, _P, _P, _P, _P, C.c_int64, _P, _P, C.c_int, C.c_int, _P, _P, _P, _P]),
     "orc_ray_box": (C.c_int, [_P, _P, _P, _P, _P, _P]),
     "orc_neighbour_face": (C.c_int, [_P, _P, _P]),
     "orc_perlin_table": (None, [C.c_uint32, _P, _P]),
@@ -32,8 +33,8 @@ _SIGS = {
     "orc_grad_dot": (C.c_float, [C.c_uint32, C.c_float, C.c_float, C.c_float]),
     "orc_noise_eval": (C.c_float, [_P, C.c_float, C.c_float, C.c_float]),
     "orc_noise_eval_f64": (C.c_double, [_P, C.c_double, C.c_double, C.c_double]),
-    "orc_noise_fill": (None, [_P, _P, _P, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, _P, C.c_int]),
-    "orc_noise_fill_f64": (None, [_P, _P, _P, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, _P, C.c_int]),
+    "orc_noise_fill": (None, [_P, _P, _P, _P, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, _P, C.c_int]),
+    "orc_noise_fill_f64": (None, [_P, _P, _P, _P, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, _P, C.c_int]),
     "orc_trilinear": (C.c_float, [_P, _P, _P, C.c_float, C.c_float, C.c_float]),
     "orc_sky": (None, [_P, _P, C.c_float, _P, C.c_int64, _P]),
     "orc_sky_f64": (None, [_P, _P, C.c_float, _P, C.c_int64, _P]),
@@ -134,6 +135,25 @@ def ray_tri(o, d, v0, e1, e2, two_sided=False):
     return bool(hit), float(t[0]), uv
 
 
+def hit_attributes(geom, normals, uvs, objs, o, d, prim, obj=None, two_sided=False, secondary=False):
+    """(normal (n,3), uv (n,2), bary (n,2), hit_point (n,3)) of traced rays:
+    geom (n_prims, 9) v0, e1, e2; normals (n_prims, 9) n0, n1, n2; uvs
+    (n_prims, 6) or None; objs an OBJ_DT array."""
+    geom, normals = _f(geom), _f(normals)
+    uvs = None if uvs is None else _f(uvs)
+    o, d = _f(o), _f(d)
+    n = o.size // 3
+    prim = np.ascontiguousarray(prim, np.int32)
+    obj = None if obj is None else np.ascontiguousarray(obj, np.int32)
+    nrm = np.empty((n, 3), np.float32)
+    uv = np.empty((n, 2), np.float32)
+    bary = np.empty((n, 2), np.float32)
+    hp = np.empty((n, 3), np.float32)
+    lib().orc_hit_attributes(_p(geom), _p(normals), _p(uvs), _p(objs), _p(o), _p(d), n, _p(prim), _p(obj),
+                             int(two_sided), int(secondary), _p(nrm), _p(uv), _p(bary), _p(hp))
+    return nrm, uv, bary, hp
+
+
 def ray_box(bmin, bmax, o, d):
     tn = np.zeros(1, np.float32)
     tf = np.zeros(1, np.float32)
@@ -179,23 +199,24 @@ def _terrain_args(perm, kw):
     p.update(kw)
     dim = np.array(p["dim"], np.int32)
     sp = np.array(p["spacing"], np.float32)
+    origin = np.array(p.get("origin", (0.0, 0.0, 0.0)), np.float32)
     perm = np.ascontiguousarray(perm, np.uint32)
-    return perm, dim, sp, p
+    return perm, dim, sp, origin, p
 
 
 def noise_fill(perm, threads=0, **kw):
     """FP32 terrain grid as (dz, dy, dx)."""
-    perm, dim, sp, p = _terrain_args(perm, kw)
+    perm, dim, sp, origin, p = _terrain_args(perm, kw)
     out = np.empty((dim[2], dim[1], dim[0]), np.float32)
-    lib().orc_noise_fill(_p(perm), _p(dim), _p(sp), p["octaves"], p["lacunarity"], p["gain"], p["base_freq"],
+    lib().orc_noise_fill(_p(perm), _p(dim), _p(sp), _p(origin), p["octaves"], p["lacunarity"], p["gain"], p["base_freq"],
                          p["y_offset"], _p(out), threads)
     return out
 
 
 def noise_fill_f64(perm, threads=0, **kw):
-    perm, dim, sp, p = _terrain_args(perm, kw)
+    perm, dim, sp, origin, p = _terrain_args(perm, kw)
     out = np.empty((dim[2], dim[1], dim[0]), np.float64)
-    lib().orc_noise_fill_f64(_p(perm), _p(dim), _p(sp), p["octaves"], p["lacunarity"], p["gain"], p["base_freq"],
+    lib().orc_noise_fill_f64(_p(perm), _p(dim), _p(sp), _p(origin), p["octaves"], p["lacunarity"], p["gain"], p["base_freq"],
                              p["y_offset"], _p(out), threads)
     return out
 

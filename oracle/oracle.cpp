@@ -415,6 +415,70 @@ ORC_API int orc_neighbour_face(const float* bmin, const float* bmax, const float
   return neighbour(L, {p[0], p[1], p[2]});
 }
 
+// ------------------------------------------------------------ hit attributes
+
+// Hit attributes of a traced ray (prim, obj as the traversal returned them):
+// the Moller-Trumbore test of the hit repeated on the object-space ray gives
+// the barycentrics (u, v) and t; then
+//   normal = normalize(w n0 + u n1 + v n2), w = 1 - (u + v)
+//     (gpu_get_tri_normal, RayEngine/src/cuda-src/ray_functions.cuh:146-155),
+//   uv = w' uv0 + u uv1 + v uv2, w' = 1 - u - v (get_uvs, :157-166), or (u, v)
+//     when there are no vertex uvs (:206-213),
+//   hit point = ro + t rd in object space (:204).
+// Misses (prim < 0) give zeros.  Pinned by closed forms: flat vertex normals
+// give the face normal, a ray aimed at v0 + a e1 + b e2 gives barycentrics
+// (a, b) and the linear interpolation of the uvs and normals there, and the
+// hit point lies on the triangle (tests/test_attr_oracle.py).
+ORC_API void orc_hit_attributes(const float* geom, const float* nrm, const float* uvs, const void* objs,
+                                const float* ray_o, const float* ray_d, int64_t n, const int32_t* prim,
+                                const int32_t* obj, int two_sided, int secondary, float* normal, float* uv,
+                                float* bary, float* hit_point) {
+  const Obj* O = static_cast<const Obj*>(objs);
+  for (int64_t i = 0; i < n; ++i) {
+    float out[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};  // normal, uv, bary, hit point
+    const int32_t p = prim[i];
+    if (p >= 0) {
+      const Obj& ob = O[obj ? obj[i] : 0];
+      V3<float> no, nd;
+      transform(ob, V3<float>{ray_o[3 * i], ray_o[3 * i + 1], ray_o[3 * i + 2]},
+                V3<float>{ray_d[3 * i], ray_d[3 * i + 1], ray_d[3 * i + 2]}, secondary != 0, no, nd);
+      const float* g = geom + 9 * p;
+      float t = 0.0f, b[2] = {0.0f, 0.0f};
+      if (tri_test<float>(no, nd, {g[0], g[1], g[2]}, {g[3], g[4], g[5]}, {g[6], g[7], g[8]}, two_sided != 0, t, b)) {
+        const float u = b[0], v = b[1];
+        const float* q = nrm + 9 * p;
+        const float w = 1.0f - (u + v);
+        const V3<float> nn = unit((w * V3<float>{q[0], q[1], q[2]} + u * V3<float>{q[3], q[4], q[5]}) +
+                                  v * V3<float>{q[6], q[7], q[8]});
+        out[0] = nn.x;
+        out[1] = nn.y;
+        out[2] = nn.z;
+        if (uvs) {
+          const float* c = uvs + 6 * p;
+          const float w2 = 1.0f - u - v;
+          out[3] = (w2 * c[0] + u * c[2]) + v * c[4];
+          out[4] = (w2 * c[1] + u * c[3]) + v * c[5];
+        } else {
+          out[3] = u;
+          out[4] = v;
+        }
+        out[5] = u;
+        out[6] = v;
+        const V3<float> hp = no + t * nd;
+        out[7] = hp.x;
+        out[8] = hp.y;
+        out[9] = hp.z;
+      } else {
+        for (float& x : out) x = std::nanf("");
+      }
+    }
+    if (normal) std::memcpy(normal + 3 * i, out, 12);
+    if (uv) std::memcpy(uv + 2 * i, out + 3, 8);
+    if (bary) std::memcpy(bary + 2 * i, out + 5, 8);
+    if (hit_point) std::memcpy(hit_point + 3 * i, out + 7, 12);
+  }
+}
+
 // ------------------------------------------------------------ Perlin table
 
 // RPerlinNoise constructor, RayEngine/src/World/PerlinNoise.h:16-52, with the

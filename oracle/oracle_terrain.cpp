@@ -193,10 +193,14 @@ ORC_API double orc_noise_eval_f64(const uint32_t* perm, double x, double y, doub
   return eval<double>(perm, x, y, z);
 }
 
-// Terrain field sampled at every voxel p = (x,y,z) * spacing (the
-// generate_noise loop, RayEngine/src/cuda-src/renderer.cu:105-126).
-ORC_API void orc_noise_fill(const uint32_t* perm, const int32_t* dim, const float* spacing, int octaves,
-                            float lacunarity, float gain, float base_freq, float y_offset, float* out, int threads) {
+// Terrain field sampled at every voxel p = origin + (x,y,z) * spacing (the
+// generate_noise loop, RayEngine/src/cuda-src/renderer.cu:105-126; origin is
+// the chunk location cuda_update_chunk_gen passes, :982-1000; null = 0).
+ORC_API void orc_noise_fill(const uint32_t* perm, const int32_t* dim, const float* spacing, const float* origin,
+                            int octaves, float lacunarity, float gain, float base_freq, float y_offset, float* out,
+                            int threads) {
+  const float zero[3] = {0.0f, 0.0f, 0.0f};
+  const float* o = origin ? origin : zero;
   Fbm fp{octaves, lacunarity, gain, base_freq, y_offset};
   const int64_t n = (int64_t)dim[0] * dim[1] * dim[2];
 #ifdef _OPENMP
@@ -205,13 +209,16 @@ ORC_API void orc_noise_fill(const uint32_t* perm, const int32_t* dim, const floa
 #endif
   for (int64_t i = 0; i < n; ++i) {
     int x = (int)(i % dim[0]), y = (int)((i / dim[0]) % dim[1]), z = (int)(i / ((int64_t)dim[0] * dim[1]));
-    out[i] = field<float>(perm, fp, (float)x * spacing[0], (float)y * spacing[1], (float)z * spacing[2]);
+    out[i] = field<float>(perm, fp, o[0] + (float)x * spacing[0], o[1] + (float)y * spacing[1],
+                          o[2] + (float)z * spacing[2]);
   }
 }
 
-ORC_API void orc_noise_fill_f64(const uint32_t* perm, const int32_t* dim, const float* spacing, int octaves,
-                                float lacunarity, float gain, float base_freq, float y_offset, double* out,
+ORC_API void orc_noise_fill_f64(const uint32_t* perm, const int32_t* dim, const float* spacing, const float* origin,
+                                int octaves, float lacunarity, float gain, float base_freq, float y_offset, double* out,
                                 int threads) {
+  const float zero[3] = {0.0f, 0.0f, 0.0f};
+  const float* o = origin ? origin : zero;
   Fbm fp{octaves, lacunarity, gain, base_freq, y_offset};
   const int64_t n = (int64_t)dim[0] * dim[1] * dim[2];
 #ifdef _OPENMP
@@ -220,7 +227,8 @@ ORC_API void orc_noise_fill_f64(const uint32_t* perm, const int32_t* dim, const 
 #endif
   for (int64_t i = 0; i < n; ++i) {
     int x = (int)(i % dim[0]), y = (int)((i / dim[0]) % dim[1]), z = (int)(i / ((int64_t)dim[0] * dim[1]));
-    out[i] = field<double>(perm, fp, (double)x * spacing[0], (double)y * spacing[1], (double)z * spacing[2]);
+    out[i] = field<double>(perm, fp, (double)o[0] + (double)x * spacing[0], (double)o[1] + (double)y * spacing[1],
+                           (double)o[2] + (double)z * spacing[2]);
   }
 }
 

@@ -243,6 +243,20 @@ class Context:
         check(lib().re_volume_upload(self._h, ptr(dim), ptr(sp), ptr(g), C.byref(vid)), self._h)
         return Volume(self, vid.value)
 
+    def upload_volume_pairs(self, grid: np.ndarray, material: np.ndarray, spacing) -> "Volume":
+        """Uploads (dz, dy, dx) distance and material grids (x fastest) as one
+        volume with a material channel (re_volume_upload_pairs)."""
+        g = np.ascontiguousarray(grid, np.float32)
+        m = np.ascontiguousarray(material, np.float32)
+        if g.shape != m.shape or g.ndim != 3:
+            raise ValueError("distance and material grids must have one (dz, dy, dx) shape")
+        pairs = np.ascontiguousarray(np.stack([g, m], axis=-1))
+        dim = np.array([g.shape[2], g.shape[1], g.shape[0]], np.int32)
+        sp = np.array(spacing, np.float32).reshape(3)
+        vid = C.c_int32()
+        check(lib().re_volume_upload_pairs(self._h, ptr(dim), ptr(sp), ptr(pairs), C.byref(vid)), self._h)
+        return Volume(self, vid.value)
+
     def upload_scene(self, objects: list[SceneObject]) -> "Scene":
         arr = (_lib.re_object * len(objects))()
         for i, ob in enumerate(objects):
@@ -291,8 +305,8 @@ def noise_params(**kw) -> _lib.re_noise_params:
     for k, v in kw.items():
         if k == "dim":
             p.dim[:] = [int(x) for x in (v if np.ndim(v) else (v, v, v))]
-        elif k == "spacing":
-            p.spacing[:] = [float(x) for x in (v if np.ndim(v) else (v, v, v))]
+        elif k in ("spacing", "origin"):
+            getattr(p, k)[:] = [float(x) for x in (v if np.ndim(v) else (v, v, v))]
         else:
             setattr(p, k, v)
     return p
@@ -646,6 +660,29 @@ class Scene:
         flags = RE_DEVICE_PTRS | (0 if sync else RE_ASYNC)
         check(lib().re_trace_closest(self.ctx.handle, self.id, o_ptr, d_ptr, n, C.byref(p), prim_ptr, t_ptr, None,
                                      flags), self.ctx.handle)
+
+    def set_attributes(self, normals, uvs=None):
+        """Per-prim vertex normals ((n_prims, 9): n0, n1, n2) and optional
+        vertex uvs ((n_prims, 6): uv0, uv1, uv2) for hit_attributes."""
+        nr = _f32c(normals, 9)
+        uv = None if uvs is None else _f32c(uvs, 6)
+        check(lib().re_scene_set_attributes(self.ctx.handle, self.id, ptr(nr), ptr(uv), nr.size // 9), self.ctx.handle)
+
+    def hit_attributes(self, o, d, prim, obj=None, **params):
+        """(normal (n,3), uv (n,2), bary (n,2), hit_point (n,3)) of traced
+        rays with their prim ids (and object ids for several objects)."""
+        o, d = _f32c(o, 3), _f32c(d, 3)
+        n = o.size // 3
+        prim = np.ascontiguousarray(prim, np.int32)
+        obj_a = None if obj is None else np.ascontiguousarray(obj, np.int32)
+        nrm = np.empty((n, 3), np.float32)
+        uv = np.empty((n, 2), np.float32)
+        bary = np.empty((n, 2), np.float32)
+        hp = np.empty((n, 3), np.float32)
+        p = _trace_params(**params)
+        check(lib().re_hit_attributes(self.ctx.handle, self.id, ptr(o), ptr(d), n, C.byref(p), ptr(prim), ptr(obj_a),
+                                      ptr(nrm), ptr(uv), ptr(bary), ptr(hp), 0), self.ctx.handle)
+        return nrm, uv, bary, hp
 
     def trace_counts(self, o, d, **params):
         """(prim, t, counts) with counts = [entries, leaves, tris, rays_hit] summed over rays."""

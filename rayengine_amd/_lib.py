@@ -65,7 +65,8 @@ class re_trace_params(C.Structure):
 
 class re_noise_params(C.Structure):
     _fields_ = [("dim", C.c_int32 * 3), ("spacing", C.c_float * 3), ("seed", C.c_uint32), ("octaves", C.c_int32),
-                ("lacunarity", C.c_float), ("gain", C.c_float), ("base_freq", C.c_float), ("y_offset", C.c_float)]
+                ("lacunarity", C.c_float), ("gain", C.c_float), ("base_freq", C.c_float), ("y_offset", C.c_float),
+                ("origin", C.c_float * 3)]
 
 
 class re_sdf_params(C.Structure):
@@ -149,10 +150,14 @@ SIGNATURES = {
     "re_trace_closest_noalloc": (C.c_int, [_P, C.c_int32, _P, _P, C.c_int64, C.POINTER(re_trace_params), _P, _P, _P,
                                            _U64, C.c_uint32]),
     "re_trace_closest_noalloc_max_rays": (C.c_int64, []),
+    "re_scene_set_attributes": (C.c_int, [_P, C.c_int32, _P, _P, C.c_int64]),
+    "re_hit_attributes": (C.c_int, [_P, C.c_int32, _P, _P, C.c_int64, C.POINTER(re_trace_params), _P, _P, _P, _P,
+                                    _P, _P, C.c_uint32]),
     "re_perlin_table": (C.c_int, [C.c_uint32, _P]),
     "re_noise_default_params": (None, [C.POINTER(re_noise_params)]),
     "re_noise_generate": (C.c_int, [_P, C.POINTER(re_noise_params), _I32]),
     "re_volume_upload": (C.c_int, [_P, _P, _P, _P, _I32]),
+    "re_volume_upload_pairs": (C.c_int, [_P, _P, _P, _P, _I32]),
     "re_volume_read": (C.c_int, [_P, C.c_int32, _P]),
     "re_volume_info": (C.c_int, [_P, C.c_int32, _P, _P, _I64]),
     "re_volume_release": (C.c_int, [_P, C.c_int32]),

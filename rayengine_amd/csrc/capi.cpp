@@ -17,6 +17,9 @@ void free_scene(DevScene& s) {
   (void)hipFree(s.leaves);
   (void)hipFree(s.tris);
   (void)hipFree(s.objs);
+  if (s.attr_geom) (void)hipFree(s.attr_geom);
+  if (s.attr_nrm) (void)hipFree(s.attr_nrm);
+  if (s.attr_uv) (void)hipFree(s.attr_uv);
   s = DevScene{};
 }
 
@@ -67,15 +70,7 @@ RE_API void re_destroy(re_ctx* ctx) {
     (void)hipFree(kv.second.data);
     if (kv.second.mat) (void)hipFree(kv.second.mat);
   }
-  for (auto& kv : ctx->batches) {
-    DevBatch& b = kv.second;
-    if (b.exec) (void)hipGraphExecDestroy(b.exec);
-    if (b.graph) (void)hipGraphDestroy(b.graph);
-    for (void* p : {(void*)b.arena.nodes, (void*)b.arena.leaves, (void*)b.arena.tris, (void*)b.arena.objs,
-                    (void*)b.tiles, (void*)b.ray_o, (void*)b.ray_d, (void*)b.prim, (void*)b.t, (void*)b.hit_scratch,
-                    (void*)b.items, (void*)b.ctl, (void*)b.remaining, (void*)b.remaining_init, (void*)b.done_ts})
-      if (p) (void)hipFree(p);
-  }
+  for (auto& kv : ctx->batches) release_batch(kv.second);
   for (auto& kv : ctx->mlps)
     for (void* p : {(void*)kv.second.params, (void*)kv.second.m1, (void*)kv.second.m2, kv.second.pack,
                     (void*)kv.second.small, (void*)kv.second.partial, (void*)kv.second.grad, (void*)kv.second.loss})
@@ -470,6 +465,131 @@ RE_API re_status re_trace_occluded(re_ctx* ctx, int32_t scene_id, const float* r
   if (!dev)
     HIP_TRY(ctx, hipMemcpyAsync(occluded, L.occluded, (size_t)n, hipMemcpyDeviceToHost, ctx->stream), "copy occluded");
   if (!(flags & RE_ASYNC) || !dev) HIP_TRY(ctx, hipStreamSynchronize(ctx->stream), "trace_occluded sync");
+  return RE_OK;
+}
+
+
+RE_API re_status re_scene_set_attributes(re_ctx* ctx, int32_t scene_id, const float* normals, const float* uvs,
+                                         int64_t n_prims) {
+  if (!ctx) return fail(RE_ERR_INVALID, "re_scene_set_attributes: null ctx");
+  if (!normals || n_prims <= 0) return set_err(ctx, RE_ERR_INVALID, "re_scene_set_attributes: bad arguments");
+  auto it = ctx->scenes.find(scene_id);
+  if (it == ctx->scenes.end()) return set_err(ctx, RE_ERR_NOT_FOUND, "re_scene_set_attributes: unknown scene");
+  DevScene& S = it->second;
+  return guarded(ctx, "re_scene_set_attributes", [&]() -> re_status {
+    HIP_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+    // every prim's geometry from the uploaded triangle records (each prim id
+    // appears in one record or, clipped into several leaves, in identical
+    // copies)
+    std::vector<TriRec> recs((size_t)S.n_trirecs);
+    if (S.n_trirecs)
+      HIP_TRY(ctx, hipMemcpyAsync(recs.data(), S.tris, recs.size() * sizeof(TriRec), hipMemcpyDeviceToHost, ctx->stream),
+              "read triangle records");
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream), "read sync");
+    std::vector<float> geom((size_t)n_prims * 9, 0.0f);
+    std::vector<uint8_t> seen((size_t)n_prims, 0);
+    for (const TriRec& r : recs) {
+      if (r.prim < 0 || r.prim >= n_prims)
+        return set_err(ctx, RE_ERR_INVALID, "re_scene_set_attributes: n_prims is below a prim id of the scene");
+      float* dst = &geom[(size_t)r.prim * 9];
+      if (seen[(size_t)r.prim] && std::memcmp(dst, r.v0, 9 * sizeof(float)) != 0)
+        return set_err(ctx, RE_ERR_INVALID,
+                       "re_scene_set_attributes: two objects give one prim id different triangles (set prim offsets)");
+      std::memcpy(dst, r.v0, 9 * sizeof(float));
+      seen[(size_t)r.prim] = 1;
+    }
+    float *g = nullptr, *nr = nullptr, *uv = nullptr;
+    hipError_t e = hipMalloc((void**)&g, geom.size() * 4);
+    if (e == hipSuccess) e = hipMalloc((void**)&nr, (size_t)n_prims * 36);
+    if (e == hipSuccess && uvs) e = hipMalloc((void**)&uv, (size_t)n_prims * 24);
+    if (e == hipSuccess) e = hipMemcpyAsync(g, geom.data(), geom.size() * 4, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(nr, normals, (size_t)n_prims * 36, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess && uvs) e = hipMemcpyAsync(uv, uvs, (size_t)n_prims * 24, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+      for (void* p : {(void*)g, (void*)nr, (void*)uv})
+        if (p) (void)hipFree(p);
+      return hip_err(ctx, e, "re_scene_set_attributes");
+    }
+    for (void* p : {(void*)S.attr_geom, (void*)S.attr_nrm, (void*)S.attr_uv})
+      if (p) (void)hipFree(p);
+    S.attr_geom = g;
+    S.attr_nrm = nr;
+    S.attr_uv = uv;
+    S.n_attr = n_prims;
+    return RE_OK;
+  });
+}
+
+RE_API re_status re_hit_attributes(re_ctx* ctx, int32_t scene_id, const float* ray_o, const float* ray_d, int64_t n,
+                                   const re_trace_params* params, const int32_t* prim_id, const int32_t* obj_id,
+                                   float* normal, float* uv, float* bary, float* hit_point, uint32_t flags) {
+  if (!ctx) return fail(RE_ERR_INVALID, "re_hit_attributes: null ctx");
+  if (n < 0 || (n > 0 && (!ray_o || !ray_d || !prim_id)))
+    return set_err(ctx, RE_ERR_INVALID, "re_hit_attributes: bad arguments");
+  auto it = ctx->scenes.find(scene_id);
+  if (it == ctx->scenes.end()) return set_err(ctx, RE_ERR_NOT_FOUND, "re_hit_attributes: unknown scene");
+  const DevScene& S = it->second;
+  if (!S.attr_geom) return set_err(ctx, RE_ERR_INVALID, "re_hit_attributes: no re_scene_set_attributes for this scene");
+  if (S.n_obj > 1 && !obj_id) return set_err(ctx, RE_ERR_INVALID, "re_hit_attributes: obj_id needed with several objects");
+  re_trace_params p;
+  re_trace_default_params(&p);
+  if (params) p = *params;
+  if (n == 0) return RE_OK;
+  HIP_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+  AttrLaunch L{};
+  L.scene = &S;
+  L.n = n;
+  L.one_sided = p.one_sided;
+  L.secondary = p.secondary;
+  const bool dev = (flags & RE_DEVICE_PTRS) != 0;
+  if (!dev) {
+    const size_t rb = (size_t)n * 12;
+    re_status st = ensure_stage(ctx, stage_size({rb, rb, (size_t)n * 4, (size_t)n * 4, rb, (size_t)n * 8,
+                                                 (size_t)n * 8, rb}));
+    if (st) return st;
+    StageCursor c{(char*)ctx->stage};
+    float* so = c.take<float>(3 * n);
+    float* sd = c.take<float>(3 * n);
+    int32_t* sp = c.take<int32_t>(n);
+    int32_t* sob = c.take<int32_t>(n);
+    float* snr = c.take<float>(3 * n);
+    float* suv = c.take<float>(2 * n);
+    float* sba = c.take<float>(2 * n);
+    float* shp = c.take<float>(3 * n);
+    HIP_TRY(ctx, hipMemcpyAsync(so, ray_o, rb, hipMemcpyHostToDevice, ctx->stream), "copy ray_o");
+    HIP_TRY(ctx, hipMemcpyAsync(sd, ray_d, rb, hipMemcpyHostToDevice, ctx->stream), "copy ray_d");
+    HIP_TRY(ctx, hipMemcpyAsync(sp, prim_id, (size_t)n * 4, hipMemcpyHostToDevice, ctx->stream), "copy prim");
+    if (obj_id) HIP_TRY(ctx, hipMemcpyAsync(sob, obj_id, (size_t)n * 4, hipMemcpyHostToDevice, ctx->stream), "copy obj");
+    L.ray_o = so;
+    L.ray_d = sd;
+    L.prim = sp;
+    L.obj = obj_id ? sob : nullptr;
+    L.normal = normal ? snr : nullptr;
+    L.uv = uv ? suv : nullptr;
+    L.bary = bary ? sba : nullptr;
+    L.hit_point = hit_point ? shp : nullptr;
+  } else {
+    L.ray_o = ray_o;
+    L.ray_d = ray_d;
+    L.prim = prim_id;
+    L.obj = obj_id;
+    L.normal = normal;
+    L.uv = uv;
+    L.bary = bary;
+    L.hit_point = hit_point;
+  }
+  HIP_TRY(ctx, ctx->launch_begin(), "event");
+  HIP_TRY(ctx, launch_hit_attributes(L, ctx->stream), "hit attributes");
+  HIP_TRY(ctx, ctx->launch_end(), "event");
+  if (!dev) {
+    if (normal) HIP_TRY(ctx, hipMemcpyAsync(normal, L.normal, (size_t)n * 12, hipMemcpyDeviceToHost, ctx->stream), "copy normal");
+    if (uv) HIP_TRY(ctx, hipMemcpyAsync(uv, L.uv, (size_t)n * 8, hipMemcpyDeviceToHost, ctx->stream), "copy uv");
+    if (bary) HIP_TRY(ctx, hipMemcpyAsync(bary, L.bary, (size_t)n * 8, hipMemcpyDeviceToHost, ctx->stream), "copy bary");
+    if (hit_point)
+      HIP_TRY(ctx, hipMemcpyAsync(hit_point, L.hit_point, (size_t)n * 12, hipMemcpyDeviceToHost, ctx->stream), "copy point");
+  }
+  if (!(flags & RE_ASYNC) || !dev) HIP_TRY(ctx, hipStreamSynchronize(ctx->stream), "attributes sync");
   return RE_OK;
 }
 

@@ -21,17 +21,22 @@ HybridParams tile_params(const re_hybrid_params& p, const DevVolume& v) {
                       p.rejoin_voxels * vox, p.n_bisect, p.max_iters};
 }
 
-void free_batch(DevBatch& b) {
+void free_batch(DevBatch& b) { release_batch(b); }
+
+}  // namespace
+
+void re::release_batch(DevBatch& b) {
   if (b.exec) (void)hipGraphExecDestroy(b.exec);
   if (b.graph) (void)hipGraphDestroy(b.graph);
+  if (b.ev_fork) (void)hipEventDestroy(b.ev_fork);
+  if (b.ev_join) (void)hipEventDestroy(b.ev_join);
+  if (b.side) (void)hipStreamDestroy(b.side);
   for (void* p : {(void*)b.arena.nodes, (void*)b.arena.leaves, (void*)b.arena.tris, (void*)b.arena.objs,
                   (void*)b.tiles, (void*)b.ray_o, (void*)b.ray_d, (void*)b.prim, (void*)b.t, (void*)b.hit_scratch,
                   (void*)b.items, (void*)b.ctl, (void*)b.remaining, (void*)b.remaining_init, (void*)b.done_ts})
     if (p) (void)hipFree(p);
   b = DevBatch{};
 }
-
-}  // namespace
 
 extern "C" {
 
@@ -227,20 +232,37 @@ RE_API re_status re_batch_create(re_ctx* ctx, const re_batch_scene* scenes, int3
   L.block = cfg.block;
   L.grid_blocks = batch_grid(cfg.block);
 
-  // Capture reset + persistent kernel (+ surrogate tiles) once.
-  e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+  // Capture once: the reset, then two branches - the surrogate tiles' hybrid
+  // launches (largest first) on a side stream forked from the capture, and
+  // the persistent kernel on the context's stream - joined at the end.  The
+  // surrogate branch is captured first so its blocks are issued first; the
+  // persistent kernel fills the rest of the device and, its items being
+  // taken from a counter, absorbs whatever share the hybrid launches leave.
+  const bool fork = !b.surrogate.empty();
+  if (fork) {
+    e = hipStreamCreateWithFlags(&b.side, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b.ev_fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b.ev_join, hipEventDisableTiming);
+    std::stable_sort(b.surrogate.begin(), b.surrogate.end(),
+                     [](const SurrogateTile& x, const SurrogateTile& y) { return x.count > y.count; });
+  }
+  if (e == hipSuccess) e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
   if (e == hipSuccess) e = launch_batch_reset(L, ctx->stream);
-  if (e == hipSuccess) e = launch_batch(L, ctx->stream);
+  if (e == hipSuccess && fork) e = hipEventRecord(b.ev_fork, ctx->stream);
+  if (e == hipSuccess && fork) e = hipStreamWaitEvent(b.side, b.ev_fork, 0);
   for (const SurrogateTile& st : b.surrogate) {
     if (e != hipSuccess) break;
     const DevVolume& v = ctx->volumes[b.tile_vol[st.tile]];
     const DevMlp& m = ctx->mlps[st.mlp_id];
     const HybridParams hp = tile_params(hy, v);
-    e = launch_hybrid_trace(v, infer_pack(m), infer_small(m), b.ray_o + 3 * st.begin, b.ray_d + 3 * st.begin, st.count, hp, true,
-                            b.t + st.begin, b.hit_scratch + st.begin, nullptr, nullptr, ctx->stream);
-    if (e == hipSuccess) e = launch_hit_to_prim(b.hit_scratch + st.begin, b.prim + st.begin, st.count, ctx->stream);
-    if (e == hipSuccess) e = launch_batch_stamp(b.done_ts, st.slot, b.remaining, ctx->stream);
+    e = launch_hybrid_trace(v, infer_pack(m), infer_small(m), b.ray_o + 3 * st.begin, b.ray_d + 3 * st.begin, st.count,
+                            hp, true, b.t + st.begin, b.hit_scratch + st.begin, nullptr, nullptr, b.side);
+    if (e == hipSuccess) e = launch_hit_to_prim(b.hit_scratch + st.begin, b.prim + st.begin, st.count, b.side);
+    if (e == hipSuccess) e = launch_batch_stamp(b.done_ts, st.slot, b.remaining, b.side);
   }
+  if (e == hipSuccess && fork) e = hipEventRecord(b.ev_join, b.side);
+  if (e == hipSuccess) e = launch_batch(L, ctx->stream);
+  if (e == hipSuccess && fork) e = hipStreamWaitEvent(ctx->stream, b.ev_join, 0);
   hipGraph_t g = nullptr;
   hipError_t e2 = hipStreamEndCapture(ctx->stream, &g);
   if (e == hipSuccess) e = e2;

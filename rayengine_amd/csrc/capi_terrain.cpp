@@ -90,6 +90,7 @@ RE_API void re_noise_default_params(re_noise_params* p) {
   p->gain = 0.5f;
   p->base_freq = 4.0f;
   p->y_offset = 0.4f;
+  for (float& o : p->origin) o = 0.0f;
 }
 
 RE_API re_status re_noise_generate(re_ctx* ctx, const re_noise_params* params, int32_t* vol_id) {
@@ -97,8 +98,10 @@ RE_API re_status re_noise_generate(re_ctx* ctx, const re_noise_params* params, i
   re_noise_params p;
   re_noise_default_params(&p);
   if (params) p = *params;
-  if (!valid_dims(p.dim, p.spacing) || p.octaves < 0 || p.octaves > 16)
-    return set_err(ctx, RE_ERR_INVALID, "re_noise_generate: dims in [2,4096], spacing > 0, octaves in [0,16]");
+  if (!valid_dims(p.dim, p.spacing) || p.octaves < 0 || p.octaves > 16 || !std::isfinite(p.origin[0]) ||
+      !std::isfinite(p.origin[1]) || !std::isfinite(p.origin[2]))
+    return set_err(ctx, RE_ERR_INVALID,
+                   "re_noise_generate: dims in [2,4096], spacing > 0, octaves in [0,16], finite origin");
   HIP_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
   DevVolume v;
   re_status st = alloc_volume(ctx, p.dim, p.spacing, v);
@@ -118,6 +121,7 @@ RE_API re_status re_noise_generate(re_ctx* ctx, const re_noise_params* params, i
     L.dim[a] = v.dim[a];
     L.bdim[a] = v.bdim[a];
     L.spacing[a] = v.spacing[a];
+    L.origin[a] = p.origin[a];
   }
   L.fbm = FbmParams{p.octaves, p.lacunarity, p.gain, p.base_freq, p.y_offset};
   if (e == hipSuccess) e = ctx->launch_begin();
@@ -263,20 +267,16 @@ RE_API re_status re_rsdf_info(const char* path, int32_t dim[3], float spacing[3]
   return st;
 }
 
-RE_API re_status re_volume_load_rsdf(re_ctx* ctx, const char* path, int32_t* vol_id) {
-  if (!ctx || !path || !vol_id) return fail(RE_ERR_INVALID, "re_volume_load_rsdf: null argument");
-  return guarded(ctx, "re_volume_load_rsdf", [&]() -> re_status {
-  FILE* f = nullptr;
-  int32_t dim[3];
-  float spacing[3];
-  re_status st0 = open_rsdf(ctx, path, "re_volume_load_rsdf", &f, dim, spacing);
-  if (st0) return st0;
+}  // extern "C"
+
+namespace {
+
+// A new volume from linear x-fastest (distance, material) pairs; box_max_last
+// bounds the box by the point of the last voxel (load_map, Grid.cpp:61)
+// instead of dim * spacing.
+re_status upload_pairs(re_ctx* ctx, const int32_t* dim, const float* spacing, const float* pairs, bool box_max_last,
+                       int32_t* vol_id) {
   const int64_t n = (int64_t)dim[0] * dim[1] * dim[2];
-  std::vector<float> pairs((size_t)(2 * n));
-  bool ok;
-  ok = std::fread(pairs.data(), sizeof(float), pairs.size(), f) == pairs.size();
-  std::fclose(f);
-  if (!ok) return set_err(ctx, RE_ERR_IO, "re_volume_load_rsdf: file shorter than its header says");
   std::vector<float> dist((size_t)n), mat((size_t)n);
   for (int64_t i = 0; i < n; ++i) {
     dist[(size_t)i] = pairs[(size_t)(2 * i)];
@@ -286,8 +286,8 @@ RE_API re_status re_volume_load_rsdf(re_ctx* ctx, const char* path, int32_t* vol
   DevVolume v;
   re_status st = alloc_volume(ctx, dim, spacing, v);
   if (st) return st;
-  // load_map bounds the volume by the point of its last voxel (Grid.cpp:61).
-  for (int a = 0; a < 3; ++a) v.box_max[a] = (float)(dim[a] - 1) * spacing[a];
+  if (box_max_last)
+    for (int a = 0; a < 3; ++a) v.box_max[a] = (float)(dim[a] - 1) * spacing[a];
   std::vector<float> bd, bm;
   to_bricked(v, dist.data(), bd);
   to_bricked(v, mat.data(), bm);
@@ -298,12 +298,42 @@ RE_API re_status re_volume_load_rsdf(re_ctx* ctx, const char* path, int32_t* vol
   if (e != hipSuccess) {
     (void)hipFree(v.data);
     if (v.mat) (void)hipFree(v.mat);
-    return hip_err(ctx, e, "re_volume_load_rsdf");
+    v.mat = nullptr;
+    return hip_err(ctx, e, "volume upload");
   }
   int32_t id = ctx->next_id++;
   ctx->volumes[id] = v;
   *vol_id = id;
   return RE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+RE_API re_status re_volume_upload_pairs(re_ctx* ctx, const int32_t* dim, const float* spacing, const float* pairs,
+                                        int32_t* vol_id) {
+  if (!ctx || !dim || !spacing || !pairs || !vol_id) return fail(RE_ERR_INVALID, "re_volume_upload_pairs: null argument");
+  if (!valid_dims(dim, spacing))
+    return set_err(ctx, RE_ERR_INVALID, "re_volume_upload_pairs: dims in [2,4096], spacing > 0");
+  return guarded(ctx, "re_volume_upload_pairs",
+                 [&]() -> re_status { return upload_pairs(ctx, dim, spacing, pairs, false, vol_id); });
+}
+
+RE_API re_status re_volume_load_rsdf(re_ctx* ctx, const char* path, int32_t* vol_id) {
+  if (!ctx || !path || !vol_id) return fail(RE_ERR_INVALID, "re_volume_load_rsdf: null argument");
+  return guarded(ctx, "re_volume_load_rsdf", [&]() -> re_status {
+    FILE* f = nullptr;
+    int32_t dim[3];
+    float spacing[3];
+    re_status st0 = open_rsdf(ctx, path, "re_volume_load_rsdf", &f, dim, spacing);
+    if (st0) return st0;
+    const int64_t n = (int64_t)dim[0] * dim[1] * dim[2];
+    std::vector<float> pairs((size_t)(2 * n));
+    const bool ok = std::fread(pairs.data(), sizeof(float), pairs.size(), f) == pairs.size();
+    std::fclose(f);
+    if (!ok) return set_err(ctx, RE_ERR_IO, "re_volume_load_rsdf: file shorter than its header says");
+    return upload_pairs(ctx, dim, spacing, pairs.data(), true, vol_id);
   });
 }
 

@@ -33,7 +33,27 @@ struct DevScene {
   int64_t n_nodes = 0, n_leaves = 0, n_trirecs = 0;
   int64_t bytes = 0;
   std::vector<LeafRec> h_leaves;
+  // hit attributes (re_scene_set_attributes), indexed by prim id
+  float* attr_geom = nullptr;  // v0, e1, e2 (9 floats) per prim
+  float* attr_nrm = nullptr;   // vertex normals n0, n1, n2 (9 floats) per prim
+  float* attr_uv = nullptr;    // vertex uvs uv0, uv1, uv2 (6 floats) per prim, or null
+  int64_t n_attr = 0;
 };
+
+struct AttrLaunch {
+  const DevScene* scene;
+  const float* ray_o;
+  const float* ray_d;
+  const int32_t* prim;
+  const int32_t* obj;  // null: object 0
+  int64_t n;
+  int32_t one_sided, secondary;
+  float* normal;     // 3 per ray or null
+  float* uv;         // 2 per ray or null
+  float* bary;       // 2 per ray or null
+  float* hit_point;  // 3 per ray or null
+};
+hipError_t launch_hit_attributes(const AttrLaunch& L, hipStream_t s);
 
 // Per-ray work counters of a traversal: node entries read, leaf records read,
 // triangle records tested.
@@ -107,6 +127,7 @@ struct NoiseLaunch {
   int32_t dim[3];
   int32_t bdim[3];
   float spacing[3];
+  float origin[3];
   FbmParams fbm;
 };
 
@@ -278,8 +299,16 @@ struct DevBatch {
   BatchLaunch launch;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  // surrogate tiles run on a branch of the graph beside the persistent kernel
+  // (captured from this stream between a fork and a join event)
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   double wall_clock_khz = 100000.0;
 };
+
+// Frees everything a batch holds (graph, branch stream and events, device
+// buffers) and resets it.
+void release_batch(DevBatch& b);
 
 hipError_t launch_batch_reset(const BatchLaunch& L, hipStream_t s);
 hipError_t launch_hit_to_prim(const uint8_t* hit, int32_t* prim, int64_t n, hipStream_t s);

@@ -27,7 +27,8 @@ struct LdsTable {
 
 __global__ void __launch_bounds__(256) noise_fill_kernel(float* __restrict__ out, const uint32_t* __restrict__ perm,
                                                          int32_t dx, int32_t dy, int32_t dz, int32_t bx, int32_t by,
-                                                         float sx, float sy, float sz, FbmParams fp) {
+                                                         float sx, float sy, float sz, float ox, float oy,
+                                                         float oz, FbmParams fp) {
   __shared__ uint16_t P[512];
   for (int i = threadIdx.x; i < 512; i += blockDim.x) P[i] = (uint16_t)perm[i];
   __syncthreads();
@@ -40,7 +41,7 @@ __global__ void __launch_bounds__(256) noise_fill_kernel(float* __restrict__ out
     const int z = gz * kBrick + (int)compact3((uint32_t)m >> 2);
     float v = 0.0f;
     if (x < dx && y < dy && z < dz) {
-      const f3 p{(float)x * sx, (float)y * sy, (float)z * sz};
+      const f3 p{ox + (float)x * sx, oy + (float)y * sy, oz + (float)z * sz};
       v = terrain_field(T, fp, p);
     }
     out[brick * kBrickVox + m] = v;
@@ -161,7 +162,8 @@ hipError_t launch_volume_edit(const EditLaunch& L, hipStream_t s) {
 hipError_t launch_noise_fill(const NoiseLaunch& L, hipStream_t s) {
   const int64_t bricks = (int64_t)L.bdim[0] * L.bdim[1] * L.bdim[2];
   hipLaunchKernelGGL(noise_fill_kernel, dim3((unsigned)bricks), dim3(256), 0, s, L.out, L.perm, L.dim[0], L.dim[1],
-                     L.dim[2], L.bdim[0], L.bdim[1], L.spacing[0], L.spacing[1], L.spacing[2], L.fbm);
+                     L.dim[2], L.bdim[0], L.bdim[1], L.spacing[0], L.spacing[1], L.spacing[2], L.origin[0], L.origin[1],
+                     L.origin[2], L.fbm);
   return hipGetLastError();
 }
 

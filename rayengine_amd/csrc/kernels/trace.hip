@@ -265,12 +265,102 @@ trace_coop_kernel(RE_CLOSEST_PARAMS) {
   }
 }
 
+// Hit attributes of traced rays: the barycentrics of the hit triangle (the
+// Moller-Trumbore test of the hit repeated on the same transformed ray and
+// record, so u, v are the traversal's own), the interpolated vertex normal
+// (gpu_get_tri_normal, RayEngine/src/cuda-src/ray_functions.cuh:146-155), the
+// interpolated uv (get_uvs, :157-166) or, without vertex uvs, (u, v) itself
+// (the reference's material-without-uvs case, :206-213), and the hit point
+// ro + t rd in object space (:204).  Misses get zeros.
+__global__ void __launch_bounds__(256) hit_attr_kernel(const ObjXform* __restrict__ objs,
+                                                       const float* __restrict__ geom, const float* __restrict__ nrm,
+                                                       const float* __restrict__ uvs, const float* __restrict__ ray_o,
+                                                       const float* __restrict__ ray_d, const int32_t* __restrict__ prim,
+                                                       const int32_t* __restrict__ obj, int64_t n, int64_t n_attr,
+                                                       int32_t n_obj, int32_t one_sided,
+                                                       int32_t secondary, float* __restrict__ normal,
+                                                       float* __restrict__ uv, float* __restrict__ bary,
+                                                       float* __restrict__ hit_point) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int32_t p = prim[i];
+  float nx = 0.0f, ny = 0.0f, nz = 0.0f, uu = 0.0f, vv = 0.0f, tu = 0.0f, tv = 0.0f, hx = 0.0f, hy = 0.0f, hz = 0.0f;
+  const int32_t k = obj ? obj[i] : 0;
+  if (p >= 0 && (p >= n_attr || k < 0 || k >= n_obj)) {
+    // ids outside the scene's prims or objects: not from a traversal of it
+    nx = ny = nz = uu = vv = tu = tv = hx = hy = hz = __builtin_nanf("");
+  } else if (p >= 0) {
+    const ObjXform ob = objs[k];
+    const f3 o{ray_o[3 * i], ray_o[3 * i + 1], ray_o[3 * i + 2]};
+    const f3 d{ray_d[3 * i], ray_d[3 * i + 1], ray_d[3 * i + 2]};
+    f3 ro, rd;
+    xform_ray(ob, o, d, secondary != 0, ro, rd);
+    const float* g = geom + 9 * (int64_t)p;
+    float t, u, v;
+    if (ray_tri_uv(ro, rd, f3{g[0], g[1], g[2]}, f3{g[3], g[4], g[5]}, f3{g[6], g[7], g[8]}, one_sided == 0, t, u,
+                   v)) {
+      const float* q = nrm + 9 * (int64_t)p;
+      const float w = 1.0f - (u + v);
+      const f3 nn = normalize(add(add(smul(w, f3{q[0], q[1], q[2]}), smul(u, f3{q[3], q[4], q[5]})),
+                                  smul(v, f3{q[6], q[7], q[8]})));
+      nx = nn.x;
+      ny = nn.y;
+      nz = nn.z;
+      if (uvs) {
+        const float* c = uvs + 6 * (int64_t)p;
+        const float w2 = 1.0f - u - v;
+        uu = (w2 * c[0] + u * c[2]) + v * c[4];
+        vv = (w2 * c[1] + u * c[3]) + v * c[5];
+      } else {
+        uu = u;
+        vv = v;
+      }
+      tu = u;
+      tv = v;
+      const f3 hp = add(ro, smul(t, rd));
+      hx = hp.x;
+      hy = hp.y;
+      hz = hp.z;
+    } else {
+      // the ray does not hit its prim: inputs not from one traversal
+      nx = ny = nz = uu = vv = tu = tv = hx = hy = hz = __builtin_nanf("");
+    }
+  }
+  if (normal) {
+    normal[3 * i] = nx;
+    normal[3 * i + 1] = ny;
+    normal[3 * i + 2] = nz;
+  }
+  if (uv) {
+    uv[2 * i] = uu;
+    uv[2 * i + 1] = vv;
+  }
+  if (bary) {
+    bary[2 * i] = tu;
+    bary[2 * i + 1] = tv;
+  }
+  if (hit_point) {
+    hit_point[3 * i] = hx;
+    hit_point[3 * i + 1] = hy;
+    hit_point[3 * i + 2] = hz;
+  }
+}
+
 __global__ void zero_words_kernel(uint32_t* __restrict__ p, int32_t n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0u;
 }
 
 }  // namespace
+
+hipError_t launch_hit_attributes(const AttrLaunch& L, hipStream_t s) {
+  if (L.n <= 0) return hipSuccess;
+  const DevScene& S = *L.scene;
+  hipLaunchKernelGGL(hit_attr_kernel, dim3((unsigned)((L.n + 255) / 256)), dim3(256), 0, s, S.objs, S.attr_geom,
+                     S.attr_nrm, S.attr_uv, L.ray_o, L.ray_d, L.prim, L.obj, L.n, S.n_attr, S.n_obj, L.one_sided,
+                     L.secondary, L.normal, L.uv, L.bary, L.hit_point);
+  return hipGetLastError();
+}
 
 hipError_t launch_zero_words(uint32_t* p, int32_t n, hipStream_t s) {
   if (n <= 0) return hipSuccess;

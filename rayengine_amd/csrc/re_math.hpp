@@ -111,6 +111,29 @@ RE_HD bool ray_tri(f3 o, f3 d, f3 v0, f3 e1, f3 e2, float& t_out) {
   return false;
 }
 
+// ray_tri / ray_tri_2s that also return the barycentrics (u, v) of the hit,
+// the same operations in the same order (RayEngine/src/cuda-src/ray_functions.cuh:173-223).
+RE_HD bool ray_tri_uv(f3 o, f3 d, f3 v0, f3 e1, f3 e2, bool two_sided, float& t_out, float& u_out, float& v_out) {
+  f3 h = cross(d, e2);
+  float a = dot(e1, h);
+  if (two_sided ? (a > -kEps && a < kEps) : (a < kEps)) return false;
+  float f = 1.0f / a;
+  f3 s = sub(o, v0);
+  float u = f * dot(s, h);
+  if (u < 0.0f || u > 1.0f) return false;
+  f3 q = cross(s, e1);
+  float v = f * dot(d, q);
+  if (v < 0.0f || u + v > 1.0f) return false;
+  float t = f * dot(e2, q);
+  if (t > kEps) {
+    t_out = t;
+    u_out = u;
+    v_out = v;
+    return true;
+  }
+  return false;
+}
+
 // Two-sided variant (|det| < kEps rejects), used when the one_sided
 // trace parameter is 0.  Same operation order as ray_tri.
 RE_HD bool ray_tri_2s(f3 o, f3 d, f3 v0, f3 e1, f3 e2, float& t_out) {

@@ -48,6 +48,94 @@ def test_compat_trace_matches_python_api(demo, gpu_ctx, tmp_path):
     assert np.array_equal(rec[:, 0].view(np.int32), p)
     assert np.array_equal(rec[:, 1], t.view(np.uint32))
     assert (p >= 0).sum() > 100
+    # HitResult normal, uv and hit point from the packed vertex normals and
+    # uvs of the full copy_memory call, equal to the C-ABI attributes and to
+    # the oracle's; ray_o / ray_dir are the camera rays
+    n_prims = len(re.icosphere(2, 1.0))
+    raw = np.fromfile(str(out) + ".attr_in", np.float32)
+    normals = raw[:n_prims * 9].reshape(n_prims, 9)
+    uvs = raw[n_prims * 9:].reshape(n_prims, 6)
+    attr = np.fromfile(str(out) + ".attr", np.float32).reshape(-1, 14)
+    scene.set_attributes(normals, uvs)
+    nrm, uv, _, hp = scene.hit_attributes(o, d, p)
+    for got, want in ((attr[:, 0:3], nrm), (attr[:, 3:5], uv), (attr[:, 5:8], hp)):
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    assert np.array_equal(attr[:, 8:11], o) and np.array_equal(attr[:, 11:14], d)
+    nodes, leaves, tris, obs = scene.export()
+    geom = np.zeros((n_prims, 9), np.float32)
+    geom[tris["prim"]] = np.concatenate([tris["v0"], tris["e1"], tris["e2"]], axis=1)
+    import oracle
+    on, ouv, _, ohp = oracle.hit_attributes(geom, normals, uvs, obs, o, d, p)
+    assert np.array_equal(on.view(np.uint32), nrm.view(np.uint32))
+    assert np.array_equal(ouv.view(np.uint32), uv.view(np.uint32))
+    # smooth normals of the unit sphere: close to the unit hit point direction
+    h = p >= 0
+    unit_hp = hp[h] / np.linalg.norm(hp[h], axis=1, keepdims=True)
+    assert np.abs(nrm[h] - unit_hp).max() < 0.05
+    assert "bruteforce refused 1" in r.stdout
+
+
+def _read_volume(path):
+    raw = np.fromfile(path, np.float32)
+    dim = raw[:3].view(np.int32)
+    sp = raw[3:6]
+    n = int(np.prod(dim))
+    shape = (dim[2], dim[1], dim[0])
+    return dim, sp, raw[6:6 + n].reshape(shape), raw[6 + n:6 + 2 * n].reshape(shape)
+
+
+@pytest.mark.gpu
+def test_compat_volume_lifecycle(demo, gpu_ctx, tmp_path):
+    """initialize_volume_render, cuda_update_chunk, cuda_update_chunk_gen,
+    generate_noise(pos), update_render_settings and cuda_render_frame in
+    SDFScene's order (SDFScene.cpp:42-59, UILayer.cpp:262-318)."""
+    import oracle
+    r = subprocess.run([str(demo), "volume", str(tmp_path)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "volume ok" in r.stdout
+    f32 = np.float32
+    # initialize_volume_render: the chunk's grid with its materials
+    dim, sp, g, m = _read_volume(tmp_path / "init.bin")
+    assert list(dim) == [24, 24, 24] and np.all(sp == f32(1) / f32(24))
+    idx = np.arange(24, dtype=np.float32) * (f32(1) / f32(24)) - f32(0.5)
+    z, y, x = np.meshgrid(idx, idx, idx, indexing="ij")
+    want = np.sqrt(x * x + y * y + z * z) - f32(0.3)
+    assert np.abs(g - want).max() < 1e-6
+    assert np.array_equal(m, np.where(g < 0, f32(1), f32(2)))
+    # its frame: camera rays taken into the chunk's frame (origin - location)
+    vol = gpu_ctx.upload_volume_pairs(g, m, (1 / 24, 1 / 24, 1 / 24))
+    o, d = re.camera_rays(re.Camera(pos=(0.75, 0.5, 2.5), view=(0.0, 0.0, -1.0)), 64, 48)
+    o = (o - np.float32([0.25, 0.0, -0.5])).astype(np.float32)
+    _, h, _, nrm = vol.trace(o, d, normals=True)
+    px = re.frame_shade(gpu_ctx, d, h, nrm, 64, 48)
+    assert np.array_equal(np.fromfile(tmp_path / "init_frame.bin", np.uint32).reshape(48, 64), px)
+    assert h.sum() > 100
+    vol.release()
+    # cuda_update_chunk: the edited grid
+    _, _, g2, m2 = _read_volume(tmp_path / "update.bin")
+    assert m2.flat[0] == 7 and np.array_equal(g2, g)
+    # cuda_update_chunk_gen: the field at the chunk location, seed 7,
+    # resolution and world size of the scene settings
+    dim, sp, g3, _ = _read_volume(tmp_path / "gen.bin")
+    assert list(dim) == [40, 24, 33]
+    want_sp = np.float32([f32(2) / f32(40), f32(1) / f32(24), f32(1.5) / f32(33)])
+    assert np.array_equal(sp, want_sp)
+    ref = oracle.noise_fill(re.perlin_table(7), dim=(40, 24, 33), spacing=tuple(want_sp), origin=(3.0, 0.0, -2.0))
+    assert np.array_equal(g3.view(np.uint32), ref.view(np.uint32))
+    # generate_noise(pos) after update_render_settings: new resolution, origin 0, seed 2016
+    dim, sp, g4, _ = _read_volume(tmp_path / "noise.bin")
+    assert list(dim) == [32, 16, 20]
+    want_sp = np.float32([f32(1) / f32(32), f32(0.5) / f32(16), f32(0.75) / f32(20)])
+    ref = oracle.noise_fill(re.perlin_table(2016), dim=(32, 16, 20), spacing=tuple(want_sp))
+    assert np.array_equal(g4.view(np.uint32), ref.view(np.uint32))
+    # and its frame with the settings' gamma and vignetting
+    vol = gpu_ctx.upload_volume(g4, tuple(want_sp))
+    o, d = re.camera_rays(re.Camera(pos=(0.5, 0.9, -0.3), view=(0.0, -0.55, 1.0)), 64, 48)
+    _, h, _, nrm = vol.trace(o, d, normals=True)
+    px = re.frame_shade(gpu_ctx, d, h, nrm, 64, 48, gamma=1, vignetting=1, vignetting_k=0.4)
+    assert np.array_equal(np.fromfile(tmp_path / "noise_frame.bin", np.uint32).reshape(48, 64), px)
+    assert not np.array_equal(px, re.frame_shade(gpu_ctx, d, h, nrm, 64, 48))
+    vol.release()
 
 
 @pytest.mark.gpu

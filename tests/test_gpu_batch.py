@@ -75,3 +75,47 @@ def test_batch_rejects_bad_ranges(gpu_ctx, small_batch):
         re.Batch(gpu_ctx, [], [(vols[0], 0, 0, 10)], ro, rd, hybrid=dict(max_iters=0))   # unbounded march
     with pytest.raises(re.ReError):
         re.Batch(gpu_ctx, [], [(vols[0], 1, 0, 10)], ro, rd, sdf=dict(max_steps=0))
+
+
+@pytest.mark.timeout(120, method="thread")
+def test_batch_with_surrogate_tile_equals_single_calls(gpu_ctx, small_batch):
+    """A surrogate tile runs on its own branch of the graph beside the
+    persistent kernel; every ray still equals the single-call result."""
+    trees, vols, ro, rd, sr, tr = small_batch
+    dv = re.distance_volume(vols[1])
+    net = re.Surrogate(gpu_ctx, seed=2)
+    net.train(dv, n_samples=1 << 16, batch=4096, steps=200, lr=1e-3, seed=3)
+    dv.release()
+    hyb = dict(tau_voxels=4.0, margin_voxels=1.0, k_exact=0.1, n_bisect=4)
+    mar = dict(k_exact=0.05, h_min_voxels=1.0, n_bisect=1)
+    modes = [0, 2, 1]
+    say("test_batch_with_surrogate_tile_equals_single_calls: create")
+    b = re.Batch(gpu_ctx, [(t, s, c) for t, (s, c) in zip(trees, sr)],
+                 [(v, m, s, c) + ((net,) if m == 2 else ()) for v, m, (s, c) in zip(vols, modes, tr)], ro, rd,
+                 item_rays=256, block=128, hybrid=hyb, march=mar)
+    refs = []
+    for tree, (s, c) in zip(trees, sr):
+        scene = gpu_ctx.upload_scene([re.SceneObject(tree)])
+        p, t = scene.trace_closest(ro[s:s + c], rd[s:s + c])
+        refs.append((s, c, p, t))
+        scene.release()
+    for v, m, (s, c) in zip(vols, modes, tr):
+        if m == 0:
+            t, h, _, _ = re.hybrid_trace(v, None, ro[s:s + c], rd[s:s + c], **mar)
+        elif m == 2:
+            t, h, ev, _ = re.hybrid_trace(v, net, ro[s:s + c], rd[s:s + c], **hyb)
+            assert ev.sum() > 0
+        else:
+            t, h, _ = v.trace(ro[s:s + c], rd[s:s + c])
+        refs.append((s, c, np.where(h == 1, 0, -1).astype(np.int32), t))
+    for rep in range(3):
+        say(f"replay {rep}")
+        b.run()
+        p, t = b.results()
+        for s, c, pr, tr_ in refs:
+            assert np.array_equal(p[s:s + c], pr), f"replay {rep}: prim mismatch in [{s}, {s + c})"
+            assert np.array_equal(_bits(t[s:s + c]), _bits(tr_)), f"replay {rep}: t mismatch in [{s}, {s + c})"
+        assert (b.completion_ms() > 0).all()
+        gpu_ctx.noise_volume(dim=(16, 16, 16), spacing=(1 / 16,) * 3).release()
+    b.release()
+    net.release()

@@ -49,6 +49,26 @@ def test_noise_volume_odd_dims_and_seeds(gpu_ctx, dims, seed):
     vol.release()
 
 
+def test_noise_volume_with_origin(gpu_ctx, perm):
+    """The chunk-location origin (cuda_update_chunk_gen, renderer.cu:982-1000)."""
+    for origin in ((0.25, -0.5, 3.0), (-17.125, 0.0, 40.5)):
+        say(f"noise origin {origin}")
+        vol = gpu_ctx.noise_volume(dim=(40, 24, 33), spacing=(0.05, 0.02, 0.03), origin=origin)
+        ref = oracle.noise_fill(perm, dim=(40, 24, 33), spacing=(0.05, 0.02, 0.03), origin=origin)
+        assert np.array_equal(vol.read().view(np.uint32), ref.view(np.uint32))
+        vol.release()
+
+
+def test_volume_upload_pairs_keeps_material(gpu_ctx):
+    rng = np.random.default_rng(1)
+    g = rng.standard_normal((9, 14, 11)).astype(np.float32)
+    m = rng.integers(0, 5, g.shape).astype(np.float32)
+    vol = gpu_ctx.upload_volume_pairs(g, m, (0.1, 0.2, 0.3))
+    assert np.array_equal(vol.read(), g) and np.array_equal(vol.read_material(), m)
+    assert list(vol.dim) == [11, 14, 9]
+    vol.release()
+
+
 def test_volume_upload_roundtrip(gpu_ctx):
     rng = np.random.default_rng(0)
     g = rng.standard_normal((13, 27, 10)).astype(np.float32)

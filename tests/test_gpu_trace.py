@@ -186,3 +186,77 @@ def test_cluster_scene_every_ray_bit_exact(gpu_ctx, cluster):
     hit = pa >= 0
     assert (ta[hit] > 1e-4).all() and (ta[~hit] == np.float32(1e20)).all()
     assert hit[len(o):].mean() > 0.3
+
+
+def _attr_parity(got, want):
+    for name, g, w in zip(("normal", "uv", "bary", "hit_point"), got, want):
+        gb, wb = g.view(np.uint32), w.view(np.uint32)
+        same = (gb == wb) | (np.isnan(g) & np.isnan(w))
+        assert same.all(), f"{name}: {int((~same).sum())} values differ"
+
+
+def _prim_geometry(tris, n_prims):
+    geom = np.zeros((n_prims, 9), np.float32)
+    geom[tris["prim"], 0:3] = tris["v0"]
+    geom[tris["prim"], 3:6] = tris["e1"]
+    geom[tris["prim"], 6:9] = tris["e2"]
+    return geom
+
+
+def test_hit_attributes_bit_exact(gpu_ctx, small):
+    """Normals, uvs, barycentrics and hit points of traced rays
+    (ray_functions.cuh:146-166,204-213) equal the oracle's bit for bit, with
+    and without vertex uvs; misses give zeros."""
+    tri_v, tree, o, d = small
+    n_prims = len(tri_v)
+    rng = np.random.default_rng(8)
+    normals = rng.normal(size=(n_prims, 9)).astype(np.float32)
+    uvs = rng.uniform(0, 1, (n_prims, 6)).astype(np.float32)
+    scene = gpu_ctx.upload_scene([re.SceneObject(tree)])
+    nodes, leaves, tris, obs = scene.export()
+    geom = _prim_geometry(tris, n_prims)
+    p, t = scene.trace_closest(o, d)
+    for with_uv in (True, False):
+        say(f"hit attributes, uvs={with_uv}")
+        scene.set_attributes(normals, uvs if with_uv else None)
+        got = scene.hit_attributes(o, d, p)
+        want = oracle.hit_attributes(geom, normals, uvs if with_uv else None, obs, o, d, p)
+        _attr_parity(got, want)
+        assert np.all(got[0][p < 0] == 0) and not np.isnan(got[0][p >= 0]).any()
+    # the hit point lies at the traced t along the unit direction
+    h = p >= 0
+    dn = d / np.linalg.norm(d, axis=1, keepdims=True)
+    assert np.abs(got[3][h] - (o[h] + t[h, None] * dn[h])).max() < 1e-4
+    # prim ids outside the scene give NaN, not a read past the attribute arrays
+    bad = p.copy()
+    bad[:10] = n_prims + 5
+    nrm, _, _, _ = scene.hit_attributes(o, d, bad)
+    assert np.isnan(nrm[:10]).all()
+    scene.release()
+
+
+def test_hit_attributes_multi_object(gpu_ctx):
+    a = re.icosphere(3, 1.0)
+    b = re.random_tris(5, 3000, -1.5, 1.5, 0.05, 0.3)
+    ta, tb = re.KDTree(a), re.KDTree(b)
+    objs = [re.SceneObject(ta, location=(0.5, -0.2, 0.1), rotation_y=0.7),
+            re.SceneObject(tb, location=(-0.3, 0.4, 0.0), rotation_y=-1.3, prim_offset=len(a)),
+            re.SceneObject(ta, location=(0.2, 0.1, -0.4), rotation_y=2.1, is_character=True,
+                           prim_offset=len(a) + len(b))]
+    n_prims = 2 * len(a) + len(b)
+    scene = gpu_ctx.upload_scene(objs)
+    nodes, leaves, tris, obs = scene.export()
+    geom = _prim_geometry(tris, n_prims)
+    rng = np.random.default_rng(9)
+    normals = rng.normal(size=(n_prims, 9)).astype(np.float32)
+    uvs = rng.uniform(0, 1, (n_prims, 6)).astype(np.float32)
+    scene.set_attributes(normals, uvs)
+    o, d = re.camera_rays(re.Camera(pos=(0.3, 0.5, 6.0), view=(0.0, -0.1, -1.0)), 160, 120)
+    for secondary in (False, True):
+        say(f"hit attributes, multi-object, secondary={secondary}")
+        p, t, ob = scene.trace_closest(o, d, with_obj=True, secondary=secondary)
+        got = scene.hit_attributes(o, d, p, obj=ob, secondary=secondary)
+        want = oracle.hit_attributes(geom, normals, uvs, obs, o, d, p, obj=ob, secondary=secondary)
+        _attr_parity(got, want)
+        assert not np.isnan(got[0][p >= 0]).any()
+    scene.release()

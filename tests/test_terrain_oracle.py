@@ -165,3 +165,17 @@ def test_fp32_march_against_fp64(perm):
     both = (h == 1) & (h64 == 1)
     err = np.abs(t[both] - t64[both])
     assert np.percentile(err, 99) < 1e-3
+
+
+def test_noise_origin_shifts_the_sampled_window(perm):
+    """A chunk generated at origin k * spacing is the window of the grid at
+    origin 0 that starts at voxel k (power-of-two spacing keeps both sample
+    positions exact), and the origin moves y against the y_offset plane."""
+    sp = (1.0 / 64, 1.0 / 64, 1.0 / 64)
+    big = oracle.noise_fill(perm, dim=(24, 20, 28), spacing=sp)
+    k = (4, 8, 12)
+    chunk = oracle.noise_fill(perm, dim=(16, 12, 16), spacing=sp, origin=tuple(np.float32(ki / 64) for ki in k))
+    assert np.array_equal(chunk, big[k[2]:k[2] + 16, k[1]:k[1] + 12, k[0]:k[0] + 16])
+    # origin 0 is the default grid bit for bit
+    assert np.array_equal(oracle.noise_fill(perm, dim=(8, 8, 8), spacing=sp, origin=(0, 0, 0)),
+                          oracle.noise_fill(perm, dim=(8, 8, 8), spacing=sp))
