@@ -246,7 +246,8 @@ bool traverse(const Node* nodes, const Leaf* leaves, const Tri* tris, const Obj&
 
 // trace_scene (RayEngine/src/cuda-src/ray_functions.cuh:376-396) over every
 // object, for n rays, OpenMP over rays with schedule(dynamic, 64) as in
-// RayEngine/src/Engine/RayTracer.cpp:154.  counts (nullable) gets 3 per ray.
+// RayEngine/src/Engine/RayTracer.cpp:154.  counts (nullable) gets 4 per ray:
+// entries, leaves, triangles tested and the end reason of the last loop.
 // cap <= 0 means the ceiling 2^20 (a ray whose exit point lies within
 // K_EPSILON of two faces can alternate between two leaves until the cap).  Pinned by brute-force equality at cap = inf
 // (orc_brute), the icosphere closed form and the primitive KATs.  cut: see

@@ -9,18 +9,13 @@
 #include <vector>
 
 #include "capi_util.hpp"
+#include "host/rsdf.hpp"
 
 using namespace re;
 
 namespace {
 
-bool valid_dims(const int32_t* dim, const float* spacing) {
-  for (int a = 0; a < 3; ++a) {
-    if (dim[a] < 2 || dim[a] > 4096) return false;
-    if (!(spacing[a] > 0.0f) || !std::isfinite(spacing[a])) return false;
-  }
-  return true;
-}
+bool valid_dims(const int32_t* dim, const float* spacing) { return valid_volume_dims(dim, spacing); }
 
 re_status alloc_volume(re_ctx* ctx, const int32_t* dim, const float* spacing, DevVolume& v) {
   for (int a = 0; a < 3; ++a) {
@@ -226,50 +221,16 @@ RE_API re_status re_volume_save_rsdf(re_ctx* ctx, int32_t vol_id, const char* pa
   return RE_OK;
 }
 
-namespace {
-
-// Opens an .rsdf file and checks it before anything of the size its header
-// claims is allocated: a 24-byte header (int32 dims in [2,4096], float
-// spacing > 0), then exactly dim.x * dim.y * dim.z (distance, material) float
-// pairs.  On success the file is positioned at the first pair.
-re_status open_rsdf(re_ctx* ctx, const char* path, const char* who, FILE** out, int32_t dim[3], float spacing[3]) {
-  *out = nullptr;
-  FILE* f = std::fopen(path, "rb");
-  if (!f) return set_err(ctx, RE_ERR_IO, std::string(who) + ": cannot open " + path);
-  bool ok = std::fread(dim, sizeof(int32_t), 3, f) == 3 && std::fread(spacing, sizeof(float), 3, f) == 3;
-  const char* why = nullptr;
-  if (!ok) why = "short header";
-  else if (!valid_dims(dim, spacing)) why = "header dims outside [2,4096] or spacing not > 0";
-  int64_t fsize = 0, want = 0;
-  if (!why) {
-    want = 24 + 8 * (int64_t)dim[0] * dim[1] * dim[2];
-    if (fseeko(f, 0, SEEK_END) != 0) why = "cannot seek";
-    else fsize = (int64_t)ftello(f);
-  }
-  if (!why && fsize < want) why = "file shorter than its header says";
-  if (!why && fsize > want) why = "file longer than its header says";
-  if (!why && fseeko(f, 24, SEEK_SET) != 0) why = "cannot seek";
-  if (why) {
-    std::fclose(f);
-    return set_err(ctx, RE_ERR_IO, std::string(who) + ": " + why);
-  }
-  *out = f;
-  return RE_OK;
-}
-
-}  // namespace
-
-RE_API re_status re_rsdf_info(const char* path, int32_t dim[3], float spacing[3]) {
-  if (!path || !dim || !spacing) return fail(RE_ERR_INVALID, "re_rsdf_info: null argument");
-  FILE* f = nullptr;
-  re_status st = open_rsdf(nullptr, path, "re_rsdf_info", &f, dim, spacing);
-  if (f) std::fclose(f);
-  return st;
-}
-
 }  // extern "C"
 
 namespace {
+
+re_status open_rsdf(re_ctx* ctx, const char* path, const char* who, FILE** out, int32_t dim[3], float spacing[3]) {
+  const char* why = nullptr;
+  const re_status st = rsdf_open(path, out, dim, spacing, &why);
+  if (st) return set_err(ctx, st, std::string(who) + ": " + why + ": " + path);
+  return RE_OK;
+}
 
 // A new volume from linear x-fastest (distance, material) pairs; box_max_last
 // bounds the box by the point of the last voxel (load_map, Grid.cpp:61)
