@@ -334,9 +334,11 @@ NOISE_FLOP_PER_VOXEL = 5 * 70
 TRAIN_STEPS = 20000
 
 
-def terrain_cpu(cpu, o, d, grid32):
+def terrain_cpu(cpu, o, d, grid32, net_params):
     """CPU baselines of the terrain path: the host noise fill (oracle,
-    voxels/s) and the host reference sphere tracer on the same rays."""
+    voxels/s), the host reference sphere tracer, the host exact-field march
+    and the host surrogate hybrid (the trained network in FP32) on the same
+    rays (every k-th ray where the whole set would take minutes)."""
     import oracle
     perm, _ = oracle.perlin_table(2016)
     out = {}
@@ -351,7 +353,32 @@ def terrain_cpu(cpu, o, d, grid32):
     out["reference_march"] = cpu_rate(
         lambda th: oracle.sdf_trace(grid32, (1 / 256,) * 3, o, d, threads=th, **params), n, "Mrays/s",
         cpu["threads"], f"all {n} terrain rays, CPU oracle sphere trace (step 0.8, eps 1e-3, <= 256 steps)")
+    sp = (1 / 256,) * 3
+    ex = oracle.hybrid_world_params(1 / 256, **_hybrid_voxel_params(TERRAIN_METHODS["exact_march"][1]))
+    out["exact_march"] = cpu_rate(
+        lambda th: oracle.exact_march(grid32, sp, o, d, ex["k_exact"], ex["h_min"], ex["n_bisect"], ex["max_iters"],
+                                      threads=th), n, "Mrays/s", cpu["threads"],
+        f"all {n} terrain rays, CPU oracle exact-field march", one_core_fn=lambda th: oracle.exact_march(
+            grid32, sp, o[::8], d[::8], ex["k_exact"], ex["h_min"], ex["n_bisect"], ex["max_iters"], threads=th),
+        one_core_units=len(o[::8]), one_core_what=f"every 8th terrain ray ({len(o[::8])} rays)")
+    hy = oracle.hybrid_world_params(1 / 256, **_hybrid_voxel_params(TERRAIN_METHODS["surrogate_hybrid"][1]))
+    os_, ds_ = o[::64], d[::64]
+    o1, d1 = o[::1024], d[::1024]
+    out["surrogate_hybrid"] = cpu_rate(
+        lambda th: oracle.hybrid_trace(grid32, sp, net_params, os_, ds_, threads=th, **hy), len(os_), "Mrays/s",
+        cpu["threads"], f"every 64th terrain ray ({len(os_)} rays), CPU oracle hybrid, the trained network in FP32",
+        one_core_fn=lambda th: oracle.hybrid_trace(grid32, sp, net_params, o1, d1, threads=th, **hy),
+        one_core_units=len(o1), one_core_what=f"every 1024th terrain ray ({len(o1)} rays)")
     return out
+
+
+def _hybrid_voxel_params(kw):
+    """re_hybrid_params fields (voxel units) with the defaults filled in."""
+    import rayengine_amd as re
+    p = re.hybrid_params(**kw)
+    return dict(tau_voxels=p.tau_voxels, margin_voxels=p.margin_voxels, k_exact=p.k_exact,
+                h_min_voxels=p.h_min_voxels, rejoin_voxels=p.rejoin_voxels, n_bisect=p.n_bisect,
+                max_iters=p.max_iters)
 
 
 def terrain_fp64(o, d, grid32, t_gpu, h_gpu, cpu):
@@ -503,7 +530,7 @@ def terrain_section(re, W, torch, ctx, cpu, with_cpu=True):
         except Exception as e:
             out["fp32_vs_fp64"] = {"error": f"{type(e).__name__}: {e}"}
         try:
-            out["cpu_baseline"] = terrain_cpu(cpu, o, d, grid32)
+            out["cpu_baseline"] = terrain_cpu(cpu, o, d, grid32, net.params()[0])
         except Exception as e:
             out["cpu_baseline"] = {"error": f"{type(e).__name__}: {e}"}
     for x in (net, dv, vol):
@@ -511,7 +538,50 @@ def terrain_section(re, W, torch, ctx, cpu, with_cpu=True):
     return out
 
 
-def mixed_workload_section(re, W, torch, ctx, reps=3, n_streams=4):
+def mixed_cpu(cpu, trees, vols, tiles, ro, rd, sr, tr, stride=32, one_core_stride=512):
+    """CPU baseline of the mixed workload: the scenes and tiles one after
+    another, each on all threads (CPU oracle traversal for scenes, exact-field
+    march for even tiles, the hybrid with the tile's network in FP32 for odd
+    tiles), on every stride-th ray of every item."""
+    import oracle
+    from rayengine_amd.mixed import TILE_HYBRID, TILE_MARCH
+    exports = [(t.export(), t.identity_object()) for t in trees]
+    grids = [v.read() for v in vols]
+    hy = oracle.hybrid_world_params(1.0, **_hybrid_voxel_params(TILE_HYBRID))
+    ex = oracle.hybrid_world_params(1.0, **_hybrid_voxel_params(TILE_MARCH))
+
+    def world(P, dm):
+        vox = np.float32(1.0 / dm)
+        return {k: (float(np.float32(v) * vox) if k in ("tau", "margin", "h_min", "rejoin") else v)
+                for k, v in P.items()}
+
+    def run(th, step):
+        n = 0
+        for (arrays, obj), (b, c) in zip(exports, sr):
+            o, d = ro[b:b + c:step], rd[b:b + c:step]
+            oracle.trace(*arrays, obj, o, d, threads=th)
+            n += len(o)
+        for k, (g, (b, c)) in enumerate(zip(grids, tr)):
+            dm = g.shape[0]
+            o, d = ro[b:b + c:step], rd[b:b + c:step]
+            sp = (1.0 / dm,) * 3
+            if k % 2 == 0:
+                P = world(ex, dm)
+                oracle.exact_march(g, sp, o, d, P["k_exact"], P["h_min"], P["n_bisect"], P["max_iters"], threads=th)
+            else:
+                oracle.hybrid_trace(g, sp, tiles[k][2], o, d, threads=th, **world(hy, dm))
+            n += len(o)
+        return n
+
+    n_s = run(cpu["threads"], stride)
+    n_1 = sum(len(range(0, c, one_core_stride)) for _, c in list(sr) + list(tr))
+    return cpu_rate(lambda th: run(th, stride), n_s, "Mrays/s", cpu["threads"],
+                    f"every {stride}th ray of every scene and tile ({n_s} rays), items one after another",
+                    one_core_fn=lambda th: run(th, one_core_stride), one_core_units=n_1,
+                    one_core_what=f"every {one_core_stride}th ray of every scene and tile ({n_1} rays)")
+
+
+def mixed_workload_section(re, W, torch, ctx, cpu=None, reps=3, n_streams=4):
     """The many-scene mixed workload (workloads.batch_workload, seed 7: 64
     triangle scenes of 1k-500k triangles with 16k-2M incoherent rays each, 16
     noise-terrain tiles of 64^3-256^3 with their own rays; even tiles traced
@@ -637,6 +707,11 @@ def mixed_workload_section(re, W, torch, ctx, reps=3, n_streams=4):
         run.release()
     for c in ctxs[1:]:
         c.close()
+    if cpu is not None:
+        try:
+            out["cpu_baseline"] = mixed_cpu(cpu, utrees, vols, tiles, ro, rd, sr, tr)
+        except Exception as e:
+            out["cpu_baseline"] = {"error": f"{type(e).__name__}: {e}"}
     for x in nets + vols:
         if x is not None:
             x.release()
@@ -821,7 +896,7 @@ def main():
     scene.release()
     if not args.no_mixed:
         try:
-            res["mixed_workload"] = mixed_workload_section(re, W, torch, ctx)
+            res["mixed_workload"] = mixed_workload_section(re, W, torch, ctx, cpu=None if args.no_cpu else cpu)
         except Exception as e:  # the closest-hit line is still reported
             res["mixed_workload"] = {"error": f"{type(e).__name__}: {e}"}
     if not args.no_terrain:

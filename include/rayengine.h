@@ -483,9 +483,25 @@ typedef struct re_mlp_train_params {
 
 RE_API void re_mlp_default_train_params(re_mlp_train_params* p);
 /* Adam training on the GPU on samples drawn from the distance volume;
- * loss_history (nullable) receives one loss per step. */
+ * loss_history (nullable) receives one loss per step.  The sample pool is
+ * drawn from (volume, seed, n_samples, near band); step s reads batch s of
+ * it, wrapping.  A call whose seed, n_samples, batch and near band equal the
+ * network's last call continues at the batch where that call stopped (the
+ * caller passes the same volume), so N steps and then M steps train exactly
+ * as N + M steps in one call. */
 RE_API re_status re_mlp_train(re_ctx* ctx, int32_t mlp_id, int32_t dist_vol_id,
                               const re_mlp_train_params* params, float* loss_history);
+/* Training checkpoint: parameters, Adam moments, Adam step and the sample
+ * pool's key and batch cursor, so that training resumed from it reproduces
+ * an uninterrupted run bit for bit.  The file is a 64-byte header ("RESURR"
+ * 0 1, param count, step, pool seed, pool size, cursor, batch, near band)
+ * and three float32 vectors of re_mlp_param_count() values.  The reference
+ * persists nothing but .rsdf volumes (renderer.cu:838-870); the network
+ * itself has no counterpart there. */
+RE_API re_status re_mlp_save(re_ctx* ctx, int32_t mlp_id, const char* path);
+/* A new network from a checkpoint; RE_ERR_IO when the file is missing, is
+ * not a checkpoint of this network or has the wrong length. */
+RE_API re_status re_mlp_load(re_ctx* ctx, const char* path, int32_t* mlp_id);
 
 /* Hybrid terrain tracer.  Surrogate phase: while the network's distance s
  * exceeds tau, t += s - margin.  Exact phase: march the trilinear field with

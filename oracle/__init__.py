@@ -33,6 +33,11 @@ _SIGS = {
     "orc_grad_dot": (C.c_float, [C.c_uint32, C.c_float, C.c_float, C.c_float]),
     "orc_noise_eval": (C.c_float, [_P, C.c_float, C.c_float, C.c_float]),
     "orc_noise_eval_f64": (C.c_double, [_P, C.c_double, C.c_double, C.c_double]),
+    "orc_exact_march": (None, [_P, _P, _P, _P, _P, C.c_int64, C.c_float, C.c_float, C.c_int, C.c_int, _P, _P, _P,
+                               C.c_int]),
+    "orc_hybrid_trace": (None, [_P, _P, _P, _P, C.c_float, _P, _P, C.c_int64, C.c_float, C.c_float, C.c_float,
+                                C.c_float, C.c_float, C.c_int, C.c_int, _P, _P, _P, _P, C.c_int]),
+    "orc_mlp_eval_f32": (C.c_float, [_P, C.c_float, C.c_float, C.c_float]),
     "orc_noise_fill": (None, [_P, _P, _P, _P, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, _P, C.c_int]),
     "orc_noise_fill_f64": (None, [_P, _P, _P, _P, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, _P, C.c_int]),
     "orc_trilinear": (C.c_float, [_P, _P, _P, C.c_float, C.c_float, C.c_float]),
@@ -395,3 +400,56 @@ def frame_shade(params, ray_d, hit, normal, width, height):
     out = np.empty((height, width), np.uint32)
     lib().orc_frame_shade(_p(d), _p(h), _p(nrm), int(width), int(height), C.byref(params), _p(out))
     return out
+
+
+def exact_march(grid, spacing, o, d, k_exact, h_min, n_bisect, max_iters, threads=0):
+    """(t, hit, samples) of the exact-field march over a (dz, dy, dx) FP32
+    grid; k_exact and h_min in world units as the library applies them."""
+    g = np.ascontiguousarray(grid, np.float32)
+    dim = np.array([g.shape[2], g.shape[1], g.shape[0]], np.int32)
+    sp = np.array(spacing, np.float32).reshape(3)
+    o, d = _f(o), _f(d)
+    n = o.size // 3
+    t = np.empty(n, np.float32)
+    hit = np.empty(n, np.uint8)
+    s = np.empty(n, np.int32)
+    lib().orc_exact_march(_p(g), _p(dim), _p(sp), _p(o), _p(d), n, k_exact, h_min, n_bisect, max_iters, _p(t),
+                          _p(hit), _p(s), threads)
+    return t, hit, s
+
+
+def hybrid_world_params(spacing, tau_voxels, margin_voxels, k_exact, h_min_voxels, rejoin_voxels, n_bisect,
+                        max_iters):
+    """re_hybrid_params in world units, rounded as the library rounds them
+    (float32 products with the x spacing)."""
+    vox = np.float32(np.float32(spacing[0]) if np.ndim(spacing) else spacing)
+    f = np.float32
+    return dict(tau=float(f(tau_voxels) * vox), margin=float(f(margin_voxels) * vox), k_exact=float(f(k_exact)),
+                h_min=float(f(h_min_voxels) * vox), rejoin=float(f(rejoin_voxels) * vox), n_bisect=int(n_bisect),
+                max_iters=int(max_iters))
+
+
+def hybrid_trace(grid, spacing, net_params, o, d, tau, margin, k_exact, h_min, rejoin, n_bisect, max_iters,
+                 const_s=0.0, threads=0):
+    """(t, hit, evals, samples) of the hybrid tracer with the network in FP32
+    (net_params: the flat parameter vector) or, with net_params None, a
+    network of constant value const_s; distances in world units."""
+    g = np.ascontiguousarray(grid, np.float32)
+    dim = np.array([g.shape[2], g.shape[1], g.shape[0]], np.int32)
+    sp = np.array(spacing, np.float32).reshape(3)
+    net = None if net_params is None else np.ascontiguousarray(net_params, np.float32)
+    o, d = _f(o), _f(d)
+    n = o.size // 3
+    t = np.empty(n, np.float32)
+    hit = np.empty(n, np.uint8)
+    ev = np.empty(n, np.int32)
+    sm = np.empty(n, np.int32)
+    lib().orc_hybrid_trace(_p(g), _p(dim), _p(sp), _p(net), const_s, _p(o), _p(d), n, tau, margin, k_exact, h_min,
+                           rejoin, n_bisect, max_iters, _p(t), _p(hit), _p(ev), _p(sm), threads)
+    return t, hit, ev, sm
+
+
+def mlp_eval_f32(params, pts):
+    p = np.ascontiguousarray(params, np.float32)
+    pts = np.asarray(pts, np.float32).reshape(-1, 3)
+    return np.array([lib().orc_mlp_eval_f32(_p(p), float(x), float(y), float(z)) for x, y, z in pts], np.float32)

@@ -152,3 +152,37 @@ ORC_API void orc_adam(float* params, float* m, float* v, const float* grad, int 
     params[i] = (float)((double)params[i] - (double)lr * (mi / c1) / (std::sqrt(vi / c2) + (double)eps));
   }
 }
+
+// The network in FP32 (float weights, features, accumulation and SiLU):
+// the CPU stand-in for the device's BF16 evaluation in the hybrid tracer.
+ORC_API float orc_mlp_eval_f32(const float* P, float x, float y, float z) {
+  float in[kIn], h[kH], g[kH];
+  const float p[3] = {x, y, z};
+  const float two_pi = 6.28318530717958647692f;
+  for (int d = 0; d < 3; ++d)
+    for (int k = 0; k < kFreq; ++k) {
+      const float a = two_pi * std::exp2(0.5f * (float)k) * p[d];
+      in[32 * d + 2 * k] = std::sin(a);
+      in[32 * d + 2 * k + 1] = std::cos(a);
+    }
+  in[96] = x;
+  in[97] = y;
+  in[98] = z;
+  const int oW[4] = {oW0, oW1, oW2, oW3}, ob[4] = {ob0, ob1, ob2, ob3};
+  const float* src = in;
+  for (int l = 0; l < 4; ++l) {
+    const int nin = l == 0 ? kIn : kH;
+    for (int i = 0; i < kH; ++i) {
+      const float* w = P + oW[l] + i * nin;
+      float s = 0.0f;
+      for (int j = 0; j < nin; ++j) s += w[j] * src[j];
+      s += P[ob[l] + i];
+      g[i] = s / (1.0f + std::exp(-s));
+    }
+    std::copy(g, g + kH, h);
+    src = h;
+  }
+  float o = 0.0f;
+  for (int j = 0; j < kH; ++j) o += P[oW4 + j] * h[j];
+  return o + P[ob4];
+}

@@ -184,7 +184,53 @@ void march(const Grid<T>& G, const T* bmin, const T* bmax, const T* o, const T* 
   }
 }
 
+// The library's exact-field march (device_paths.hpp exact_march; the
+// reference has no counterpart, its tracer is march above): from the box
+// entry step max(f * k_exact, h_min) while the trilinear field stays
+// positive, bisect the first sign change n_bisect times, and take the
+// midpoint of the last bracket.  A ray entering at f <= 0 hits at the entry.
+template <typename T>
+bool exact_march(const Grid<T>& G, const T* bmin, const T* bmax, const T* o, const T* d, T k_exact, T h_min,
+                 int n_bisect, int max_iters, T& t_out, int32_t& samples) {
+  T tn, tf;
+  samples = 0;
+  t_out = (T)1e20f;
+  if (!slab<T>(bmin, bmax, o, d, tn, tf)) return false;
+  auto f = [&](T t) { return trilerp<T>(G, o[0] + t * d[0], o[1] + t * d[1], o[2] + t * d[2]); };
+  T t = tn;
+  T f_prev = f(t);
+  ++samples;
+  if (f_prev <= T(0)) {
+    t_out = t;
+    return true;
+  }
+  for (int it = 0; it < max_iters; ++it) {
+    const T h = std::fmax(f_prev * k_exact, h_min);
+    const T t_next = t + h;
+    const T f_next = f(t_next);
+    ++samples;
+    if (f_next <= T(0)) {
+      T a = t, b = t_next;
+      for (int k = 0; k < n_bisect; ++k) {
+        const T mid = T(0.5) * (a + b);
+        const T fm = f(mid);
+        ++samples;
+        if (fm > T(0)) a = mid; else b = mid;
+      }
+      t_out = T(0.5) * (a + b);
+      return true;
+    }
+    t = t_next;
+    f_prev = f_next;
+    if (t >= tf) return false;
+  }
+  return false;
+}
+
 }  // namespace
+
+// Surrogate network value at one point in FP32 (oracle_mlp.cpp).
+extern "C" float orc_mlp_eval_f32(const float* params, float x, float y, float z);
 
 ORC_API float orc_grad_dot(uint32_t h, float x, float y, float z) { return gradient_dot<float>(h, x, y, z); }
 
@@ -377,4 +423,116 @@ ORC_API void orc_volume_edit(float* dist, float* mat, const int32_t* dim, const 
         dist[i] = r.d;
         mat[i] = r.m;
       }
+}
+
+// Exact-field march over a linear FP32 grid, volume box [0, dim * spacing];
+// distances in world units.
+ORC_API void orc_exact_march(const float* grid, const int32_t* dim, const float* spacing, const float* ray_o,
+                             const float* ray_d, int64_t n, float k_exact, float h_min, int n_bisect, int max_iters,
+                             float* t, uint8_t* hit, int32_t* samples, int threads) {
+  Grid<float> G{grid, dim[0], dim[1], dim[2], spacing[0], spacing[1], spacing[2]};
+  const float bmin[3] = {0, 0, 0};
+  const float bmax[3] = {(float)dim[0] * spacing[0], (float)dim[1] * spacing[1], (float)dim[2] * spacing[2]};
+#ifdef _OPENMP
+  if (threads <= 0) threads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 64) num_threads(threads)
+#endif
+  for (int64_t i = 0; i < n; ++i) {
+    float tt;
+    int32_t s;
+    const bool h = exact_march<float>(G, bmin, bmax, ray_o + 3 * i, ray_d + 3 * i, k_exact, h_min, n_bisect,
+                                      max_iters, tt, s);
+    t[i] = tt;
+    hit[i] = h ? 1 : 0;
+    if (samples) samples[i] = s;
+  }
+}
+
+// The library's hybrid tracer per ray (surrogate.hip hybrid_trace_kernel,
+// whose wave scheduling leaves every ray's steps as written here; no
+// reference counterpart), with the network evaluated in FP32 instead of the
+// device's BF16.  Each advance: in the surrogate phase one network value s at
+// the current point: s > tau steps t += s - margin (t >= t_far ends with a
+// miss), else the exact phase starts here (f <= 0 at this point is a hit at
+// t); in the exact phase one exact-march step (as exact_march) in the same
+// advance; after rejoin of exact travel without a crossing the surrogate
+// phase resumes.  At most max_iters advances.  net = NULL gives a network
+// whose value is the constant const_s (tests pin the state machine with it).
+ORC_API void orc_hybrid_trace(const float* grid, const int32_t* dim, const float* spacing, const float* net,
+                              float const_s, const float* ray_o, const float* ray_d, int64_t n, float tau,
+                              float margin, float k_exact, float h_min, float rejoin, int n_bisect, int max_iters,
+                              float* t_out, uint8_t* hit_out, int32_t* evals_out, int32_t* samples_out,
+                              int threads) {
+  Grid<float> G{grid, dim[0], dim[1], dim[2], spacing[0], spacing[1], spacing[2]};
+  const float bmin[3] = {0, 0, 0};
+  const float bmax[3] = {(float)dim[0] * spacing[0], (float)dim[1] * spacing[1], (float)dim[2] * spacing[2]};
+#ifdef _OPENMP
+  if (threads <= 0) threads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 16) num_threads(threads)
+#endif
+  for (int64_t i = 0; i < n; ++i) {
+    const float* o = ray_o + 3 * i;
+    const float* d = ray_d + 3 * i;
+    auto f = [&](float t) { return trilerp<float>(G, o[0] + t * d[0], o[1] + t * d[1], o[2] + t * d[2]); };
+    enum { SURR, EXACT, DONE };
+    int phase = DONE;
+    bool hit = false;
+    float t = 0.0f, t_far = 0.0f, f_prev = 0.0f, travel = 0.0f, tn, tf;
+    int32_t evals = 0, samples = 0, advances = 0;
+    if (slab<float>(bmin, bmax, o, d, tn, tf)) {
+      t = tn;
+      t_far = tf;
+      phase = SURR;
+    }
+    while (phase != DONE) {
+      if (advances >= max_iters) break;
+      if (phase == SURR) {
+        const float px = o[0] + t * d[0], py = o[1] + t * d[1], pz = o[2] + t * d[2];
+        const float s = net ? orc_mlp_eval_f32(net, px, py, pz) : const_s;
+        ++evals;
+        if (s > tau) {
+          t += s - margin;
+          if (t >= t_far) phase = DONE;
+        } else {
+          phase = EXACT;
+          travel = 0.0f;
+          f_prev = f(t);
+          ++samples;
+          if (f_prev <= 0.0f) {
+            hit = true;
+            phase = DONE;
+          }
+        }
+      }
+      if (phase == EXACT) {
+        const float h = std::fmax(f_prev * k_exact, h_min);
+        const float t_next = t + h;
+        const float f_next = f(t_next);
+        ++samples;
+        if (f_next <= 0.0f) {
+          float a = t, b = t_next;
+          for (int k = 0; k < n_bisect; ++k) {
+            const float mid = 0.5f * (a + b);
+            const float fm = f(mid);
+            ++samples;
+            if (fm > 0.0f) a = mid; else b = mid;
+          }
+          t = 0.5f * (a + b);
+          hit = true;
+          phase = DONE;
+        } else {
+          t = t_next;
+          f_prev = f_next;
+          travel += h;
+          if (t >= t_far) phase = DONE;
+          else if (travel > rejoin) phase = SURR;
+        }
+      }
+      ++advances;
+    }
+    t_out[i] = hit ? t : 1e20f;
+    hit_out[i] = hit ? 1 : 0;
+    if (evals_out) evals_out[i] = evals;
+    if (samples_out) samples_out[i] = samples;
+  }
 }

@@ -483,6 +483,19 @@ class Surrogate:
         check(lib().re_mlp_train(self.ctx.handle, self.id, dist_volume.id, C.byref(p), ptr(hist)), self.ctx.handle)
         return hist[:p.steps]
 
+    def save(self, path):
+        """Writes a training checkpoint (re_mlp_save)."""
+        check(lib().re_mlp_save(self.ctx.handle, self.id, str(path).encode()), self.ctx.handle)
+
+    @classmethod
+    def load(cls, ctx: Context, path) -> "Surrogate":
+        """A network restored from a checkpoint (re_mlp_load)."""
+        mid = C.c_int32()
+        check(lib().re_mlp_load(ctx.handle, str(path).encode(), C.byref(mid)), ctx.handle)
+        s = cls.__new__(cls)
+        s.ctx, s.id = ctx, mid.value
+        return s
+
     def release(self):
         if self.id is not None:
             check(lib().re_mlp_release(self.ctx.handle, self.id), self.ctx.handle)

@@ -184,6 +184,8 @@ SIGNATURES = {
     "re_mlp_sample": (C.c_int, [_P, C.c_int32, C.c_int64, C.c_uint64, C.c_float, _P, C.c_uint32]),
     "re_mlp_default_train_params": (None, [C.POINTER(re_mlp_train_params)]),
     "re_mlp_train": (C.c_int, [_P, C.c_int32, C.c_int32, C.POINTER(re_mlp_train_params), _P]),
+    "re_mlp_save": (C.c_int, [_P, C.c_int32, C.c_char_p]),
+    "re_mlp_load": (C.c_int, [_P, C.c_char_p, _I32]),
     "re_hybrid_default_params": (None, [C.POINTER(re_hybrid_params)]),
     "re_hybrid_trace": (C.c_int, [_P, C.c_int32, C.c_int32, C.POINTER(re_hybrid_params), _P, _P, C.c_int64, _P, _P,
                                   _P, _P, C.c_uint32]),

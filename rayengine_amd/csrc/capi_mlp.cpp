@@ -4,6 +4,9 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
 #include <vector>
 
 #include "capi_util.hpp"
@@ -129,6 +132,7 @@ RE_API re_status re_mlp_set_params(re_ctx* ctx, int32_t mlp_id, const float* par
   HIP_TRY(ctx, hipMemcpyAsync(m->m1, zeros.data(), (size_t)kParams * 4, hipMemcpyHostToDevice, ctx->stream), "reset m");
   HIP_TRY(ctx, hipMemcpyAsync(m->m2, zeros.data(), (size_t)kParams * 4, hipMemcpyHostToDevice, ctx->stream), "reset v");
   m->step = 0;
+  m->cursor = -1;
   st = repack(ctx, *m);
   if (st) return st;
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream), "set params sync");
@@ -362,12 +366,27 @@ RE_API re_status re_mlp_train(re_ctx* ctx, int32_t mlp_id, int32_t dist_vol_id, 
     HIP_TRY(ctx, hipMalloc((void**)&m->loss, (size_t)(p.steps > 0 ? p.steps : 1) * 4), "loss history");
     m->loss_cap = p.steps;
   }
+  // The pool is a pure function of (volume, seed, n_samples, band): a call
+  // with the key of the last one continues at its batch cursor.
+  uint32_t band_bits;
+  std::memcpy(&band_bits, &p.near_band_voxels, 4);
+  const bool resume = m->cursor >= 0 && m->pool_seed == p.seed && m->pool_n == p.n_samples &&
+                      m->pool_batch == p.batch && m->pool_band_bits == band_bits;
+  if (!resume) {
+    m->pool_seed = p.seed;
+    m->pool_n = p.n_samples;
+    m->pool_batch = p.batch;
+    m->pool_band_bits = band_bits;
+    m->cursor = 0;
+  }
+  const int64_t first = m->cursor;
   float4* samples = nullptr;
   HIP_TRY(ctx, hipMalloc((void**)&samples, (size_t)p.n_samples * 16), "sample pool");
   hipError_t e = launch_sample(dv, p.n_samples, p.seed, p.near_band_voxels * dv.spacing[0], 256, samples, ctx->stream);
   for (int32_t s = 0; e == hipSuccess && s < p.steps; ++s) {
-    e = launch_mlp_train(m->pack, m->small, samples, p.n_samples, (int64_t)s * p.batch, p.batch, p.over_weight,
-                         m->partial, m->partial_blocks, ctx->stream);
+    const int64_t start = (int64_t)(((unsigned __int128)(first + s) * (uint64_t)p.batch) % (uint64_t)p.n_samples);
+    e = launch_mlp_train(m->pack, m->small, samples, p.n_samples, start, p.batch, p.over_weight, m->partial,
+                         m->partial_blocks, ctx->stream);
     if (e == hipSuccess) {
       ++m->step;
       e = launch_mlp_update(m->params, m->m1, m->m2, m->partial, m->partial_blocks, m->step, p.lr, p.beta1, p.beta2,
@@ -378,8 +397,109 @@ RE_API re_status re_mlp_train(re_ctx* ctx, int32_t mlp_id, int32_t dist_vol_id, 
     e = hipMemcpyAsync(loss_history, m->loss, (size_t)p.steps * 4, hipMemcpyDeviceToHost, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   (void)hipFree(samples);
-  if (e != hipSuccess) return hip_err(ctx, e, "re_mlp_train");
+  if (e != hipSuccess) {
+    m->cursor = -1;  // the state is not a clean continuation any more
+    return hip_err(ctx, e, "re_mlp_train");
+  }
+  m->cursor = first + p.steps;
   return RE_OK;
+}
+
+// Checkpoint file: a 64-byte header, then params, Adam m and Adam v as
+// little-endian float32 (param_count each).
+struct MlpCheckpointHeader {
+  char magic[8];        // "RESURR\0\1"
+  int32_t n_params;
+  int32_t step;
+  uint64_t pool_seed;
+  int64_t pool_n;
+  int64_t cursor;
+  int32_t pool_batch;
+  uint32_t pool_band_bits;
+  uint8_t reserved[16];
+};
+static_assert(sizeof(MlpCheckpointHeader) == 64, "checkpoint header");
+static const char kMlpMagic[8] = {'R', 'E', 'S', 'U', 'R', 'R', 0, 1};
+
+RE_API re_status re_mlp_save(re_ctx* ctx, int32_t mlp_id, const char* path) {
+  if (!ctx || !path) return fail(RE_ERR_INVALID, "re_mlp_save: null argument");
+  DevMlp* m;
+  re_status st = get_mlp(ctx, mlp_id, &m);
+  if (st) return st;
+  return guarded(ctx, "re_mlp_save", [&]() -> re_status {
+    std::vector<float> buf((size_t)kParams * 3);
+    HIP_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+    HIP_TRY(ctx, hipMemcpyAsync(buf.data(), m->params, (size_t)kParams * 4, hipMemcpyDeviceToHost, ctx->stream),
+            "read params");
+    HIP_TRY(ctx, hipMemcpyAsync(buf.data() + kParams, m->m1, (size_t)kParams * 4, hipMemcpyDeviceToHost, ctx->stream),
+            "read m");
+    HIP_TRY(ctx,
+            hipMemcpyAsync(buf.data() + 2 * kParams, m->m2, (size_t)kParams * 4, hipMemcpyDeviceToHost, ctx->stream),
+            "read v");
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream), "read sync");
+    MlpCheckpointHeader h{};
+    std::memcpy(h.magic, kMlpMagic, 8);
+    h.n_params = kParams;
+    h.step = m->step;
+    h.pool_seed = m->pool_seed;
+    h.pool_n = m->pool_n;
+    h.cursor = m->cursor;
+    h.pool_batch = m->pool_batch;
+    h.pool_band_bits = m->pool_band_bits;
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return set_err(ctx, RE_ERR_IO, std::string("re_mlp_save: cannot open ") + path);
+    bool ok = std::fwrite(&h, sizeof(h), 1, f) == 1 && std::fwrite(buf.data(), 4, buf.size(), f) == buf.size();
+    ok = (std::fclose(f) == 0) && ok;
+    if (!ok) return set_err(ctx, RE_ERR_IO, std::string("re_mlp_save: write failed: ") + path);
+    return RE_OK;
+  });
+}
+
+RE_API re_status re_mlp_load(re_ctx* ctx, const char* path, int32_t* mlp_id) {
+  if (!ctx || !path || !mlp_id) return fail(RE_ERR_INVALID, "re_mlp_load: null argument");
+  return guarded(ctx, "re_mlp_load", [&]() -> re_status {
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return set_err(ctx, RE_ERR_IO, std::string("re_mlp_load: cannot open ") + path);
+    MlpCheckpointHeader h{};
+    std::vector<float> buf((size_t)kParams * 3);
+    bool ok = std::fread(&h, sizeof(h), 1, f) == 1 && std::memcmp(h.magic, kMlpMagic, 8) == 0 &&
+              h.n_params == kParams && h.step >= 0 && std::fread(buf.data(), 4, buf.size(), f) == buf.size();
+    ok = ok && std::fgetc(f) == EOF;  // nothing after the three vectors
+    std::fclose(f);
+    if (!ok)
+      return set_err(ctx, RE_ERR_IO,
+                     std::string("re_mlp_load: not a checkpoint of this network (header, size or length): ") + path);
+    int32_t id = -1;
+    re_status st = re_mlp_create(ctx, 0, &id);
+    if (st) return st;
+    DevMlp& m = ctx->mlps[id];
+    hipError_t e = hipMemcpyAsync(m.params, buf.data(), (size_t)kParams * 4, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(m.m1, buf.data() + kParams, (size_t)kParams * 4, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(m.m2, buf.data() + 2 * kParams, (size_t)kParams * 4, hipMemcpyHostToDevice, ctx->stream);
+    if (e != hipSuccess) {
+      (void)re_mlp_release(ctx, id);
+      return hip_err(ctx, e, "re_mlp_load");
+    }
+    m.step = h.step;
+    m.pool_seed = h.pool_seed;
+    m.pool_n = h.pool_n;
+    m.cursor = h.cursor;
+    m.pool_batch = h.pool_batch;
+    m.pool_band_bits = h.pool_band_bits;
+    st = repack(ctx, m);
+    if (st == RE_OK) {
+      e = hipStreamSynchronize(ctx->stream);
+      if (e != hipSuccess) st = hip_err(ctx, e, "re_mlp_load sync");
+    }
+    if (st) {
+      (void)re_mlp_release(ctx, id);
+      return st;
+    }
+    *mlp_id = id;
+    return RE_OK;
+  });
 }
 
 }  // extern "C"

@@ -193,6 +193,14 @@ struct DevMlp {
   int64_t loss_cap = 0;
   int32_t partial_blocks = 0;
   int32_t step = 0;          // Adam steps taken
+  // Training sample pool: the key it was drawn with and the batches taken
+  // from it, so that a later re_mlp_train call with the same key continues
+  // where the last one stopped.
+  uint64_t pool_seed = 0;
+  int64_t pool_n = 0;
+  int32_t pool_batch = 0;
+  uint32_t pool_band_bits = 0;  // near_band_voxels as float bits
+  int64_t cursor = -1;          // batches taken; -1: no pool yet
 };
 
 size_t mlp_train_lds_bytes();

@@ -131,9 +131,15 @@ __global__ void __launch_bounds__(256) trace_persistent_kernel(
     const unsigned long long need = __ballot(ray < 0);
     if (need != 0ull && !drained) {
       if (pool_next >= pool_end) {
-        unsigned long long b = 0;
-        if (lane == 0) b = atomicAdd(queue, (unsigned long long)kChunk);
-        b = __shfl(b, 0);
+        // every lane takes part in the atomic (lane 0 adds the chunk, the
+        // others 0) and the result is read back with readfirstlane, so the
+        // chunk start is a scalar and no lane-dependent branch precedes the
+        // loop's exit (a lane-0-only atomic followed by a shuffle can be
+        // structured into an inner loop that lanes 1-63 never leave)
+        const unsigned long long v = atomicAdd(queue, lane == 0 ? (unsigned long long)kChunk : 0ull);
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
+        const unsigned long long b = ((unsigned long long)hi << 32) | lo;
         pool_next = (int64_t)b;
         pool_end = (int64_t)b + kChunk < n ? (int64_t)b + kChunk : n;
         if ((int64_t)b >= n) drained = true;

@@ -10,7 +10,8 @@ compiles the kernel and checks the properties that make the loop terminate:
 the counter's atomic result read back with readfirstlane, no cross-lane
 shuffle between the atomic and the item record, and the item record addressed
 from that scalar index (the index scaled by the 24-byte record size in scalar
-arithmetic).
+arithmetic).  The persistent closest-hit kernel takes its ray chunks from a
+counter the same way and is checked likewise.
 """
 import re
 import shutil
@@ -24,21 +25,21 @@ SRC = ROOT / "rayengine_amd" / "csrc" / "kernels" / "batch.hip"
 HIPCC = Path("/opt/rocm/bin/hipcc")
 
 
-def _kernel_asm(tmp_path, name_part):
+def _kernel_asm(tmp_path, name_part, src=SRC):
     hipcc = str(HIPCC) if HIPCC.exists() else shutil.which("hipcc")
     if hipcc is None:
         pytest.skip("hipcc not found")
     from rayengine_amd.build import COMMON, HIP_FLAGS
-    out = tmp_path / "batch.s"
-    cmd = [hipcc, *HIP_FLAGS, *COMMON, "--cuda-device-only", "-S", str(SRC), "-o", str(out)]
+    out = tmp_path / (src.stem + ".s")
+    cmd = [hipcc, *HIP_FLAGS, *COMMON, "--cuda-device-only", "-S", str(src), "-o", str(out)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
     text = out.read_text()
     bodies = {}
-    for m in re.finditer(r"^(_Z\w*batch_kernel\w*):", text, re.M):
+    for m in re.finditer(r"^(_Z\w*" + name_part + r"\w*):", text, re.M):
         end = text.index("s_endpgm", m.end())
         bodies[m.group(1)] = text[m.end():end]
-    assert bodies, "batch_kernel not found in the device assembly"
+    assert bodies, f"{name_part} not found in the device assembly"
     return bodies
 
 
@@ -55,3 +56,14 @@ def test_batch_kernel_item_loop_is_wave_uniform(tmp_path):
         # record (where the first version read the index with a shuffle)
         between = body[atom:m.start()]
         assert "ds_bpermute" not in between and "ds_permute" not in between, f"{name}: shuffle of the item index"
+
+
+def test_persistent_trace_kernel_chunk_fetch_is_wave_uniform(tmp_path):
+    """trace_persistent_kernel (variant 1) takes chunks of rays from a device
+    counter with the same every-lane atomic and readfirstlane read-back."""
+    src = ROOT / "rayengine_amd" / "csrc" / "kernels" / "trace.hip"
+    for name, body in _kernel_asm(tmp_path, "trace_persistent_kernel", src).items():
+        atom = body.index("global_atomic_add_x2")
+        window = body[atom:atom + 600]
+        assert window.count("v_readfirstlane_b32") >= 2, f"{name}: chunk start not read back uniformly"
+        assert "ds_bpermute" not in window, f"{name}: shuffle of the chunk start"

@@ -172,6 +172,57 @@ def test_training_reduces_loss(gpu_ctx):
     net.release()
 
 
+def test_checkpoint_resume_is_bit_exact(gpu_ctx, tmp_path):
+    """Training N steps, saving, loading into a new network and training N
+    more reproduces 2N uninterrupted steps bit for bit: parameters, Adam
+    moments, step and the losses of the second half.  Two calls of N steps on
+    one network do the same (the sample pool's batch cursor continues)."""
+    vol = gpu_ctx.noise_volume(dim=(48, 48, 48), spacing=(1 / 48,) * 3)
+    dv = re.distance_volume(vol)
+    kw = dict(n_samples=1 << 16, batch=4096, lr=2e-3, seed=9)
+    say("uninterrupted 2N steps")
+    a = re.Surrogate(gpu_ctx, seed=21)
+    hist_a = a.train(dv, steps=24, **kw)   # 24 batches of 4096 wrap the 65536-sample pool
+    pa = a.params()
+    say("N steps, checkpoint, load, N steps")
+    b = re.Surrogate(gpu_ctx, seed=21)
+    b.train(dv, steps=12, **kw)
+    ck = tmp_path / "net.ckpt"
+    b.save(ck)
+    b.release()
+    c = re.Surrogate.load(gpu_ctx, ck)
+    hist_c = c.train(dv, steps=12, **kw)
+    pc = c.params()
+    for x, y, name in zip(pa, pc, ("params", "adam_m", "adam_v", "step")):
+        assert np.array_equal(np.asarray(x), np.asarray(y)), name
+    assert np.array_equal(hist_a[12:], hist_c)
+    say("two calls on one network")
+    d = re.Surrogate(gpu_ctx, seed=21)
+    d.train(dv, steps=12, **kw)
+    d.train(dv, steps=12, **kw)
+    pd = d.params()
+    assert all(np.array_equal(np.asarray(x), np.asarray(y)) for x, y in zip(pa, pd))
+    # a new seed starts a new pool from its first batch
+    e = re.Surrogate(gpu_ctx, seed=21)
+    e.train(dv, steps=12, **kw)
+    e.train(dv, steps=12, **dict(kw, seed=10))
+    assert not np.array_equal(e.params()[0], pa[0])
+    # files that are not checkpoints are refused
+    bad = tmp_path / "bad.ckpt"
+    bad.write_bytes(ck.read_bytes()[:-4])
+    with pytest.raises(re.ReError):
+        re.Surrogate.load(gpu_ctx, bad)
+    bad.write_bytes(b"RESURR\x00\x02" + ck.read_bytes()[8:])
+    with pytest.raises(re.ReError):
+        re.Surrogate.load(gpu_ctx, bad)
+    with pytest.raises(re.ReError):
+        re.Surrogate.load(gpu_ctx, tmp_path / "missing.ckpt")
+    for net in (a, c, d, e):
+        net.release()
+    dv.release()
+    vol.release()
+
+
 # Hybrid settings of the terrain benchmark.
 HYBRID = dict(tau_voxels=4.0, margin_voxels=1.0, k_exact=0.1, n_bisect=4)
 
@@ -198,6 +249,127 @@ def test_hybrid_accuracy_and_what_the_network_saves(gpu_ctx):
     assert np.mean(hh == th) >= 0.999
     assert np.percentile(e, 99) <= 0.5
     assert sm.mean() < 0.9 * se.mean()
+    net.release()
+    dv.release()
+    vol.release()
+
+
+def _world(vol_spacing, **kw):
+    p = re.hybrid_params(**kw)
+    return oracle.hybrid_world_params(vol_spacing, p.tau_voxels, p.margin_voxels, p.k_exact, p.h_min_voxels,
+                                      p.rejoin_voxels, p.n_bisect, p.max_iters)
+
+
+def test_exact_march_bit_exact_with_oracle(gpu_ctx):
+    """The exact-field march (re_hybrid_trace without a network) equals the
+    oracle's restatement bit for bit: t, hit and samples per ray."""
+    vol = gpu_ctx.noise_volume(dim=(128, 128, 128), spacing=(1 / 128,) * 3)
+    g = vol.read()
+    o, d = W.terrain_rays(480, 270)
+    for kw in (dict(), dict(k_exact=0.1, n_bisect=4, h_min_voxels=1.0), dict(max_iters=7)):
+        say(f"exact march {kw}")
+        t, h, ev, sm = re.hybrid_trace(vol, None, o, d, **kw)
+        P = _world(1 / 128, **kw)
+        to, ho, so = oracle.exact_march(g, (1 / 128,) * 3, o, d, P["k_exact"], P["h_min"], P["n_bisect"],
+                                        P["max_iters"])
+        assert np.array_equal(h, ho) and np.array_equal(t.view(np.uint32), to.view(np.uint32))
+        assert np.array_equal(sm, so) and not ev.any()
+    vol.release()
+
+
+def test_hybrid_state_machine_with_constant_networks(gpu_ctx):
+    """With all weights zero the network's value is b4.  Below tau the hybrid
+    is the exact march bit for bit; above tau every ray steps s - margin to
+    the box exit; both equal the oracle's hybrid with that constant, network
+    evaluations and field samples included."""
+    vol = gpu_ctx.noise_volume(dim=(96, 96, 96), spacing=(1 / 96,) * 3)
+    g = vol.read()
+    o, d = W.terrain_rays(320, 180)
+    net = re.Surrogate(gpu_ctx, seed=0)
+    kw = dict(tau_voxels=4.0, margin_voxels=3.0, k_exact=0.05, n_bisect=6)
+    P = _world(1 / 96, **kw)
+    te, he, _, _ = re.hybrid_trace(vol, None, o, d, **kw)
+    for s in (0.5 * P["tau"], 2.0 * P["tau"]):
+        say(f"constant network {s:.4f} (tau {P['tau']:.4f})")
+        p = np.zeros(NP, np.float32)
+        p[-1] = s
+        net.set_params(p)
+        t, h, ev, sm = re.hybrid_trace(vol, net, o, d, **kw)
+        to, ho, eo, so = oracle.hybrid_trace(g, (1 / 96,) * 3, None, o, d, const_s=float(np.float32(s)), **P)
+        assert np.array_equal(h, ho) and np.array_equal(t.view(np.uint32), to.view(np.uint32))
+        assert np.array_equal(ev, eo) and np.array_equal(sm, so)
+        if s < P["tau"]:
+            assert np.array_equal(h, he) and np.array_equal(t.view(np.uint32), te.view(np.uint32))
+        else:
+            assert not h.any() and ev.max() > 10
+    net.release()
+    vol.release()
+
+
+def test_trained_hybrid_against_fp32_oracle_network(gpu_ctx):
+    """The trained hybrid on the device (BF16 network) against the oracle's
+    hybrid with the same parameters in FP32: the network values differ by
+    the BF16 rounding, so the surrogate steps differ, but both stop at the
+    exact field's crossing.  Bar: hit agreement >= 0.999 and the 99th
+    percentile of |t_gpu - t_oracle| <= 0.25 voxel."""
+    vol = gpu_ctx.noise_volume(dim=(128, 128, 128), spacing=(1 / 128,) * 3)
+    g = vol.read()
+    dv = re.distance_volume(vol)
+    net = re.Surrogate(gpu_ctx, seed=1)
+    net.train(dv, n_samples=1 << 20, batch=65536, steps=1500, lr=1e-3, seed=1, over_weight=4.0)
+    o, d = W.terrain_rays(240, 135)
+    P = _world(1 / 128, **HYBRID)
+    t, h, ev, sm = re.hybrid_trace(vol, net, o, d, **HYBRID)
+    to, ho, eo, so = oracle.hybrid_trace(g, (1 / 128,) * 3, net.params()[0], o, d, **P)
+    both = (h == 1) & (ho == 1)
+    err = np.abs(t[both].astype(np.float64) - to[both]) * 128
+    say(f"trained hybrid vs FP32 oracle: agree {np.mean(h == ho):.5f}, p99 {np.percentile(err, 99):.4f} voxel, "
+        f"evals/ray {ev.mean():.2f} vs {eo.mean():.2f}")
+    assert np.mean(h == ho) >= 0.999 and both.sum() > 10000
+    assert np.percentile(err, 99) <= 0.25
+    assert abs(ev.mean() - eo.mean()) <= 0.1 * eo.mean()
+    net.release()
+    dv.release()
+    vol.release()
+
+
+def test_hybrid_is_network_bound(gpu_ctx):
+    """The hybrid's throughput relation on the 256^3 terrain, 1920x1080 rays:
+    its Mrays/s is bounded by the network's own evaluation rate divided by
+    the network evaluations per ray (the per-ray cost floor), it reaches a
+    fair share of that bound (the exact phase and the wave scheduling cost
+    less than the network), and at 8+ evaluations per ray that bound lies
+    below the exact-field march it is meant to beat."""
+    vol = gpu_ctx.noise_volume()
+    dv = re.distance_volume(vol)
+    net = re.Surrogate(gpu_ctx, seed=1)
+    net.train(dv, n_samples=1 << 22, batch=65536, steps=4000, lr=1e-3, seed=1, over_weight=4.0)
+    o, d = W.terrain_rays(1920, 1080)
+    n = len(o)
+
+    def best_ms(fn, reps=4):
+        fn()
+        ts = []
+        for _ in range(reps):
+            fn()
+            ts.append(gpu_ctx.last_kernel_ms())
+        return float(np.median(ts))
+
+    hy_ms = best_ms(lambda: re.hybrid_trace(vol, net, o, d, **HYBRID))
+    _, _, ev, _ = re.hybrid_trace(vol, net, o, d, **HYBRID)
+    ex_ms = best_ms(lambda: re.hybrid_trace(vol, None, o, d, k_exact=0.05, h_min_voxels=1.0, n_bisect=1))
+    pts = np.random.default_rng(0).uniform(0, 1, (1 << 22, 3)).astype(np.float32)
+    fw_ms = best_ms(lambda: net.forward(pts))
+    evals_per_ray = float(ev.mean())
+    net_gevals = (1 << 22) / (fw_ms * 1e-3) / 1e9
+    bound = net_gevals * 1e3 / evals_per_ray          # Mrays/s the network alone allows
+    hy, ex = n / (hy_ms * 1e-3) / 1e6, n / (ex_ms * 1e-3) / 1e6
+    say(f"hybrid {hy:.0f} Mrays/s, exact march {ex:.0f}, network {net_gevals:.2f} Gevals/s x "
+        f"{evals_per_ray:.2f} evals/ray -> bound {bound:.0f} Mrays/s ({hy / bound:.2f} of it)")
+    assert evals_per_ray > 4
+    assert hy <= 1.1 * bound
+    assert hy >= 0.3 * bound
+    assert bound < ex
     net.release()
     dv.release()
     vol.release()
