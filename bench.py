@@ -634,7 +634,8 @@ def mixed_workload_section(re, W, torch, ctx, cpu=None, reps=3, n_streams=4):
                            hybrid=TILE_HYBRID, march=TILE_MARCH)
     trees = [re.KDTree(s, max_leaf_tris=bcfg["max_leaf_tris"], ctx=ctx) for s in soups]
     batch = re.Batch(ctx, [(t, b, c) for t, (b, c) in zip(trees, sr)], btiles, ro, rd, item_rays=bcfg["item_rays"],
-                     block=bcfg["block"], hybrid=TILE_HYBRID, march=TILE_MARCH)
+                     block=bcfg["block"], hybrid=TILE_HYBRID, march=TILE_MARCH,
+                     side_share=bcfg.get("side_share", 0.0))
     replay_ms = batch_time_ms(ctx, batch, 5)
     walls = []
     for _ in range(reps):
@@ -652,7 +653,7 @@ def mixed_workload_section(re, W, torch, ctx, cpu=None, reps=3, n_streams=4):
                                     "p99": round(float(np.percentile(comp[:len(sr)], 99)), 3)},
             "tile_completion_ms": {"p50": round(float(np.percentile(comp[len(sr):], 50)), 3),
                                    "p99": round(float(np.percentile(comp[len(sr):], 99)), 3)},
-            "autotune": {"chosen": {k: bcfg[k] for k in ("max_leaf_tris", "block", "item_rays")},
+            "autotune": {"chosen": {k: bcfg.get(k) for k in ("max_leaf_tris", "block", "item_rays", "side_share")},
                          "seconds": round(time.perf_counter() - t_tune, 1),
                          "measurements": bmeas}}
     out["batched_graph"] = info

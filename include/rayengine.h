@@ -562,6 +562,9 @@ typedef struct re_batch_config {
   re_sdf_params sdf;     /* reference sphere-trace tiles (mode 1) */
   re_hybrid_params hybrid; /* surrogate tiles (mode 2) */
   re_hybrid_params march;  /* exact-march tiles (mode 0): k_exact, h_min_voxels, n_bisect, max_iters */
+  float side_share;        /* with surrogate tiles: the fraction of the device's work-group slots the
+                              persistent kernel leaves to the surrogate tiles' launches, which run
+                              beside it on their own graph branch; in [0, 0.9], default 0 */
 } re_batch_config;
 
 RE_API void re_batch_default_config(re_batch_config* c);

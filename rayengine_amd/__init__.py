@@ -555,7 +555,7 @@ class Batch:
     or `surrogate` when the tile names none)."""
 
     def __init__(self, ctx: Context, scenes, tiles, ray_o, ray_d, item_rays=256, block=256, surrogate=None,
-                 trace=None, sdf=None, hybrid=None, march=None):
+                 trace=None, sdf=None, hybrid=None, march=None, side_share=0.0):
         self.ctx = ctx
         o, d = _f32c(ray_o, 3), _f32c(ray_d, 3)
         self.n_rays = o.size // 3
@@ -571,6 +571,7 @@ class Batch:
         cfg = _lib.re_batch_config()
         lib().re_batch_default_config(C.byref(cfg))
         cfg.item_rays, cfg.block = int(item_rays), int(block)
+        cfg.side_share = float(side_share)
         cfg.mlp_id = surrogate.id if surrogate is not None else -1
         if trace:
             cfg.trace = _trace_params(**trace)

@@ -108,7 +108,8 @@ class re_batch_tile(C.Structure):
 
 class re_batch_config(C.Structure):
     _fields_ = [("item_rays", C.c_int32), ("block", C.c_int32), ("mlp_id", C.c_int32), ("trace", re_trace_params),
-                ("sdf", re_sdf_params), ("hybrid", re_hybrid_params), ("march", re_hybrid_params)]
+                ("sdf", re_sdf_params), ("hybrid", re_hybrid_params), ("march", re_hybrid_params),
+                ("side_share", C.c_float)]
 
 
 _P = C.c_void_p

@@ -49,6 +49,7 @@ RE_API void re_batch_default_config(re_batch_config* c) {
   re_sdf_default_params(&c->sdf);
   re_hybrid_default_params(&c->hybrid);
   re_hybrid_default_params(&c->march);
+  c->side_share = 0.0f;
 }
 
 RE_API re_status re_batch_create(re_ctx* ctx, const re_batch_scene* scenes, int32_t n_scenes,
@@ -61,8 +62,10 @@ RE_API re_status re_batch_create(re_ctx* ctx, const re_batch_scene* scenes, int3
   re_batch_config cfg;
   re_batch_default_config(&cfg);
   if (config) cfg = *config;
-  if (cfg.item_rays < 1 || cfg.item_rays > (1 << 16) || !(cfg.block == 64 || cfg.block == 128 || cfg.block == 256))
-    return set_err(ctx, RE_ERR_INVALID, "re_batch_create: item_rays in [1,65536], block in {64,128,256}");
+  if (cfg.item_rays < 1 || cfg.item_rays > (1 << 16) || !(cfg.block == 64 || cfg.block == 128 || cfg.block == 256) ||
+      !(cfg.side_share >= 0.0f && cfg.side_share <= 0.9f))
+    return set_err(ctx, RE_ERR_INVALID,
+                   "re_batch_create: item_rays in [1,65536], block in {64,128,256}, side_share in [0,0.9]");
   for (int32_t k = 0; k < n_scenes; ++k)
     if (!scenes[k].tree || scenes[k].ray_begin < 0 || scenes[k].ray_count < 0 ||
         scenes[k].ray_begin + scenes[k].ray_count > n_rays)
@@ -231,6 +234,13 @@ RE_API re_status re_batch_create(re_ctx* ctx, const re_batch_scene* scenes, int3
   L.sdf = BatchSdf{cfg.sdf.max_steps, cfg.sdf.eps, cfg.sdf.step_scale, cfg.sdf.step_rule};
   L.block = cfg.block;
   L.grid_blocks = batch_grid(cfg.block);
+  if (!b.surrogate.empty() && cfg.side_share > 0.0f) {
+    // leave side_share of the slots to the surrogate branch; the persistent
+    // kernel's items are taken from a counter, so fewer blocks only changes
+    // who runs them
+    const int64_t keep = (int64_t)((double)L.grid_blocks * (1.0 - (double)cfg.side_share));
+    L.grid_blocks = keep > 0 ? keep : 1;
+  }
 
   // Capture once: the reset, then two branches - the surrogate tiles' hybrid
   // launches (largest first) on a side stream forked from the capture, and

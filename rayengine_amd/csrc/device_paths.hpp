@@ -71,7 +71,10 @@ __device__ __forceinline__ uint4 load_window(const NodeEntry* __restrict__ nodes
 // pe (the down-walk of ray_functions.cuh:289-291: below if pe < split, else
 // above).  The below child of an entry is the next entry, so a "below" step
 // from the window's first entry needs no new load; any other step does.
-// Returns the leaf id.
+// Returns the leaf id.  The loop is bounded by the tree: the builders number
+// entries in preorder, so both children of an entry lie after it and inside
+// the array (checked on the host for every built tree shape), and every step
+// goes one level down; a descent ends at a leaf within the tree's depth.
 template <bool COUNT>
 __device__ __forceinline__ uint32_t descend(const NodeEntry* __restrict__ nodes, int32_t cur, uint4 q, f3 pe,
                                             unsigned& c_ent) {

@@ -63,7 +63,7 @@ __device__ __forceinline__ int32_t wave_fetch_sub(int32_t* p, int32_t sub, int l
 }
 
 template <bool ONE_SIDED>
-__global__ void __launch_bounds__(256) batch_kernel(BatchLaunch L) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) batch_kernel(BatchLaunch L) {
   // one CoopWave per wave of the block (dynamic LDS sized by the launch)
   extern __shared__ CoopWave waves[];
   CoopWave& wv = waves[threadIdx.x >> 6];

@@ -75,12 +75,17 @@ def test_batch_rejects_bad_ranges(gpu_ctx, small_batch):
         re.Batch(gpu_ctx, [], [(vols[0], 0, 0, 10)], ro, rd, hybrid=dict(max_iters=0))   # unbounded march
     with pytest.raises(re.ReError):
         re.Batch(gpu_ctx, [], [(vols[0], 1, 0, 10)], ro, rd, sdf=dict(max_steps=0))
+    for bad in (-0.1, 0.95, float("nan")):
+        with pytest.raises(re.ReError):
+            re.Batch(gpu_ctx, [(trees[0], 0, 10)], [], ro, rd, side_share=bad)
 
 
 @pytest.mark.timeout(120, method="thread")
-def test_batch_with_surrogate_tile_equals_single_calls(gpu_ctx, small_batch):
+@pytest.mark.parametrize("side_share", [0.0, 0.5])
+def test_batch_with_surrogate_tile_equals_single_calls(gpu_ctx, small_batch, side_share):
     """A surrogate tile runs on its own branch of the graph beside the
-    persistent kernel; every ray still equals the single-call result."""
+    persistent kernel (which may leave part of the device to it); every ray
+    still equals the single-call result."""
     trees, vols, ro, rd, sr, tr = small_batch
     dv = re.distance_volume(vols[1])
     net = re.Surrogate(gpu_ctx, seed=2)
@@ -92,7 +97,7 @@ def test_batch_with_surrogate_tile_equals_single_calls(gpu_ctx, small_batch):
     say("test_batch_with_surrogate_tile_equals_single_calls: create")
     b = re.Batch(gpu_ctx, [(t, s, c) for t, (s, c) in zip(trees, sr)],
                  [(v, m, s, c) + ((net,) if m == 2 else ()) for v, m, (s, c) in zip(vols, modes, tr)], ro, rd,
-                 item_rays=256, block=128, hybrid=hyb, march=mar)
+                 item_rays=256, block=128, hybrid=hyb, march=mar, side_share=side_share)
     refs = []
     for tree, (s, c) in zip(trees, sr):
         scene = gpu_ctx.upload_scene([re.SceneObject(tree)])

@@ -106,6 +106,36 @@ def test_tree_invariants(small):
     assert seen.all()
 
 
+def _check_descent_bounds(nodes, n_leaves):
+    """The device descent (device_paths.hpp:descend) follows, from an inner
+    entry e, e + 1 (below) or the entry in meta (above).  Both lie after e and
+    inside the array, and a leaf entry's id inside the leaf array, so every
+    descent ends at a leaf within (tree depth) steps."""
+    meta = nodes["meta"].astype(np.int64)
+    axis = meta >> 30
+    idx = meta & 0x3FFFFFFF
+    e = np.arange(len(nodes))
+    inner = axis != 3
+    n = len(nodes)
+    assert (e[inner] + 1 < n).all()
+    assert (idx[inner] > e[inner] + 1).all() and (idx[inner] < n).all()
+    assert (idx[~inner] < n_leaves).all()
+    # each leaf id appears at exactly one entry (the walk from the root reaches it once)
+    assert np.array_equal(np.sort(idx[~inner]), np.arange(n_leaves))
+
+
+def test_descent_indices_increase(small):
+    tri_v, tree, _, _ = small
+    nodes, leaves, _ = tree.export()
+    _check_descent_bounds(nodes, len(leaves))
+    soup = re.cluster_soup(3, 40000, 16, -2.0, 2.0, 0.3, -3.0, 0.5)
+    for kw in (dict(), dict(max_leaf_tris=1, max_depth=40), dict(max_depth=6)):
+        t = re.KDTree(soup, **kw)
+        nodes, leaves, _ = t.export()
+        _check_descent_bounds(nodes, len(leaves))
+        assert t.info["depth"] <= kw.get("max_depth", 24)
+
+
 FACE_AXIS = [0, 2, 0, 2, 1, 1]
 FACE_HIGH = [False, True, True, False, True, False]
 
