@@ -49,7 +49,7 @@ FP32_PEAK_TFLOPS = 157.3
 BF16_PEAK_TFLOPS = 2500.0
 METRIC = "Mrays/s closest-hit; terrain Mrays/s surrogate vs exact SDF at ≤0.5-voxel err"
 TRACE_KERNEL = {0: "trace_closest_kernel", 1: "trace_persistent_kernel", 2: "trace_closest_kernel",
-                3: "trace_closest_o6_kernel", 4: "trace_coop_kernel"}
+                3: "trace_closest_o6_kernel", 4: "trace_coop_kernel", 5: "trace_coop_refill_kernel"}
 DEFAULT_TUNE = {"max_leaf_tris": 8, "tile": 8, "block": 64, "variant": 4}
 COPY_PROBE_BYTES = 1 << 28
 # rays of the headline workload at the default tree (leaf 8): 2,073,600

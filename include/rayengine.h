@@ -207,7 +207,10 @@ typedef struct re_trace_params {
                                  the triangles of all lanes' leaves fetched as
                                  one coalesced list and tested one triangle
                                  per lane, or, when every visiting lane is at
-                                 the same leaf, staged once in LDS (default).
+                                 the same leaf, staged once in LDS (default);
+                              5: as 4 in persistent waves: a lane whose ray has
+                                 finished takes the next ray at the next leaf
+                                 round (rays taken from the end of the array).
                               Results are identical for every variant. */
 } re_trace_params;
 

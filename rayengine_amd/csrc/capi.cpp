@@ -316,7 +316,7 @@ re_status trace_common(re_ctx* ctx, int32_t scene_id, const float* ray_o, const 
   if (params) p = *params;
   if (!(p.block == 0 || p.block == 64 || p.block == 128 || p.block == 256))
     return set_err(ctx, RE_ERR_INVALID, "re_trace_closest: block must be 0, 64, 128 or 256");
-  if (p.variant < 0 || p.variant > 4) return set_err(ctx, RE_ERR_INVALID, "re_trace_closest: variant must be 0..4");
+  if (p.variant < 0 || p.variant > 5) return set_err(ctx, RE_ERR_INVALID, "re_trace_closest: variant must be 0..5");
   if (n == 0) return RE_OK;
   HIP_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
   TraceLaunch L;
@@ -332,8 +332,9 @@ re_status trace_common(re_ctx* ctx, int32_t scene_id, const float* ray_o, const 
   L.queue = (unsigned long long*)(header + 64);
   {
     const int blk = p.block > 0 ? p.block : 256;
-    int64_t& g = ctx->persistent_grid[blk == 64 ? 0 : (blk == 128 ? 1 : 2)];
-    if (g <= 0) g = trace_persistent_grid(blk);
+    const int slot = blk == 64 ? 0 : (blk == 128 ? 1 : 2);
+    int64_t& g = p.variant == 5 ? ctx->coop_refill_grid[slot] : ctx->persistent_grid[slot];
+    if (g <= 0) g = p.variant == 5 ? trace_coop_refill_grid(blk) : trace_persistent_grid(blk);
     L.grid_blocks = g;
   }
   const bool dev = (flags & RE_DEVICE_PTRS) != 0;
@@ -378,8 +379,12 @@ re_status trace_common(re_ctx* ctx, int32_t scene_id, const float* ray_o, const 
     // counters (bytes 0-31) and ray counter (64-71) of the fixed area start at zero
     HIP_TRY(ctx, launch_zero_words((uint32_t*)header, 18, ctx->stream), "scratch clear");
   } else {
-    if (counts) HIP_TRY(ctx, hipMemsetAsync(d_counts, 0, sizeof(TraceCounts), ctx->stream), "counts clear");
-    if (L.variant == 1) HIP_TRY(ctx, hipMemsetAsync(L.queue, 0, sizeof(unsigned long long), ctx->stream), "queue clear");
+    // cleared by a fill kernel, not a memset node, so that callers may
+    // capture the call in a graph
+    if (counts) HIP_TRY(ctx, launch_zero_words((uint32_t*)d_counts, (int32_t)(sizeof(TraceCounts) / 4), ctx->stream),
+                        "counts clear");
+    if (L.variant == 1 || L.variant == 5)
+      HIP_TRY(ctx, launch_zero_words((uint32_t*)L.queue, 2, ctx->stream), "queue clear");
   }
   HIP_TRY(ctx, ctx->launch_begin(), "event");
   HIP_TRY(ctx, launch_trace_closest(L, ctx->stream), "trace_closest launch");

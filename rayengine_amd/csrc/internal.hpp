@@ -83,6 +83,9 @@ struct TraceLaunch {
 // Blocks of `block` threads that fill the device at the persistent kernel's
 // occupancy.
 int64_t trace_persistent_grid(int block);
+// The same for the cooperative refill kernel (variant 5), whose blocks also
+// hold one CoopWave of LDS per wave.
+int64_t trace_coop_refill_grid(int block);
 
 hipError_t launch_trace_closest(const TraceLaunch& L, hipStream_t s);
 // Zeroes n 32-bit words with a kernel (no memset node when a caller captures).
@@ -354,6 +357,7 @@ struct re_ctx {
   void* counts_buf = nullptr;  // 256 bytes: work counters at 0, ray queue counter at 64
   void* noalloc_area = nullptr;  // this device's fixed area of re_trace_closest_noalloc (capi_noalloc.cpp)
   int64_t persistent_grid[3] = {0, 0, 0};  // per block size 64, 128, 256
+  int64_t coop_refill_grid[3] = {0, 0, 0};  // the same for variant 5
   void* stage = nullptr;
   size_t stage_bytes = 0;
   std::string err;

@@ -271,6 +271,181 @@ trace_coop_kernel(RE_CLOSEST_PARAMS) {
   }
 }
 
+// Cooperative variant with refill (variant 5): persistent waves; a lane whose
+// ray has finished takes the next one at the start of the following visit
+// round, from a wave-private chunk of 64 consecutive rays taken from a global
+// counter, so a wave is not held by its longest walk with the other lanes
+// idle (on incoherent bounce rays the longest walk of a wave is about four
+// times the mean).  The leaf work is coop_leaf_test's, shared by every lane
+// at a leaf whatever ray it holds; the rays are taken from the end of the
+// array, so appended bounce rays start first.  Per-ray results equal the
+// other variants' (each ray's walk is the same sequence of visits).
+template <bool COUNT, bool ONE_SIDED, bool ONE_OBJ>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6)))
+trace_coop_refill_kernel(RE_CLOSEST_PARAMS, unsigned long long* __restrict__ queue) {
+  extern __shared__ CoopWave waves[];
+  CoopWave& W = waves[threadIdx.x >> 6];
+  const int lane = (int)(threadIdx.x & 63);
+  const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const int32_t k_end = ONE_OBJ ? 1 : n_obj;
+  int64_t pool_next = 0, pool_end = 0;  // wave-uniform
+  bool drained = false;                 // wave-uniform
+  int64_t ray = -1;
+  f3 o{0, 0, 0}, d{0, 0, 0};  // kept only with several objects (the next object's transform)
+  f3 ro{0, 0, 0}, rd{0, 0, 0}, fr{0, 0, 0};
+  float t_entry = 0.0f, t_exit = 0.0f, hit_t = kInf, best_t = kInf;
+  int32_t k_obj = -1, cur = 0, visits = 0, hit_prim = -1, best_prim = -1, best_obj = -1;
+  bool live = false, open = false, found = false, have_window = false;
+  uint4 w = make_uint4(0u, 0u, 0u, 0u);
+  WalkMemo memo;
+  RayCounts rc;
+  unsigned c_hit = 0;
+  // the walk of object k from the ray (o, d): its first visit, or none
+  auto start_walk = [&](const ObjXform& ob, f3 oo, f3 dd) {
+    xform_ray(ob, oo, dd, secondary != 0, ro, rd);
+    fr = f3{1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z};
+    if (!ray_box_inv(ob.root_min, ob.root_max, ro, fr, t_entry, t_exit)) return;
+    cur = ob.root;
+    found = false;
+    hit_t = kInf;
+    hit_prim = -1;
+    visits = 0;
+    have_window = false;
+    memo = WalkMemo{};
+    live = t_entry < t_exit && visits < cap;
+    open = live;
+  };
+  // the next object whose root box the ray enters (trace_ray's object loop)
+  auto start_next = [&]() {
+    while (!live && k_obj + 1 < k_end) {
+      ++k_obj;
+      start_walk(objs[k_obj], o, d);
+    }
+  };
+  auto finish = [&]() {
+    prim_out[ray] = best_prim;
+    t_out[ray] = best_t;
+    if (obj_out) obj_out[ray] = best_obj;
+    if (COUNT && best_prim >= 0) ++c_hit;
+    ray = -1;
+  };
+#pragma unroll 1
+  for (;;) {
+    // close a finished walk: the next object, or the ray's result
+    if (ray >= 0 && !live) {
+      if (open && found && hit_t < best_t) {
+        best_t = hit_t;
+        best_prim = hit_prim;
+        best_obj = ONE_OBJ ? 0 : k_obj;
+      }
+      open = false;
+      if (!ONE_OBJ) start_next();
+      if (!live) finish();
+    }
+    // refill (wave-uniform): lanes without a ray take the next rays of the
+    // wave's chunk; the chunk start is taken with every lane in the atomic
+    // and read back with readfirstlane, so it is a scalar
+    const unsigned long long need = __ballot(ray < 0);
+    if (need != 0ull && !drained) {
+      if (pool_next >= pool_end) {
+        const unsigned long long v = atomicAdd(queue, lane == 0 ? 64ull : 0ull);
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
+        const int64_t b = (int64_t)(((unsigned long long)hi << 32) | lo);
+        pool_next = b;
+        pool_end = b + 64 < n ? b + 64 : n;
+        if (b >= n) drained = true;
+      }
+      if (!drained) {
+        const int64_t avail = pool_end - pool_next;
+        const int rank = __popcll(need & below);
+        if (ray < 0 && rank < avail) {
+          ray = n - 1 - (pool_next + rank);
+          const f3 no{ray_o[3 * ray], ray_o[3 * ray + 1], ray_o[3 * ray + 2]};
+          const f3 nd{ray_d[3 * ray], ray_d[3 * ray + 1], ray_d[3 * ray + 2]};
+          best_t = kInf;
+          best_prim = -1;
+          best_obj = -1;
+          live = false;
+          open = false;
+          if (ONE_OBJ) {
+            start_walk(objs[0], no, nd);
+          } else {
+            o = no;
+            d = nd;
+            k_obj = -1;
+            start_next();
+          }
+          if (!live) finish();  // enters no object: a miss
+        }
+        const int64_t taken = (int64_t)__popcll(need) < avail ? (int64_t)__popcll(need) : avail;
+        pool_next += taken;
+      }
+    }
+    if (__ballot(ray >= 0) == 0ull) {
+      if (drained) break;
+      continue;
+    }
+    // one visit round (as coop_trace_wave)
+    bool at_leaf = false, box_ok = false;
+    uint32_t first = 0, cnt = 0;
+    float tf = 0.0f;
+    int32_t r = kNoRope;
+    if (live) {
+      if (memo.repeats(cur, t_entry, t_exit)) {
+        live = false;
+      } else {
+        ++visits;
+        const f3 pe = add(ro, smul(t_entry, rd));
+        uint32_t leaf_id;
+        if (ref_is_leaf(cur)) {
+          leaf_id = ref_leaf_id(cur);
+        } else {
+          if (!have_window) w = load_window(nodes, cur);
+          leaf_id = descend<COUNT>(nodes, cur, w, pe, rc.ent);
+        }
+        const DevLeaf* lp = leaves + leaf_id;
+        const float4 h0 = *reinterpret_cast<const float4*>(lp->bmin);
+        const float4 h1 = *reinterpret_cast<const float4*>(lp->bmax + 1);
+        const float bmin[3] = {h0.x, h0.y, h0.z}, bmax[3] = {h0.w, h1.x, h1.y};
+        first = __float_as_uint(h1.z);
+        cnt = __float_as_uint(h1.w);
+        if (COUNT) ++rc.leaf;
+        float tn;
+        box_ok = ray_box_inv(bmin, bmax, ro, fr, tn, tf);
+        const int face = box_ok ? exit_face(bmin, bmax, add(ro, smul(tf, rd))) : -1;
+        r = face >= 0 ? lp->rope[face] : kNoRope;
+        have_window = r >= 0;
+        if (have_window) w = load_window(nodes, r);
+        at_leaf = true;
+      }
+    }
+    if (__ballot(at_leaf) != 0ull) {
+      coop_leaf_test<ONE_SIDED>(tris, at_leaf, first, cnt, ro, rd, t_exit, found, hit_t, hit_prim, W);
+      if (COUNT && at_leaf) rc.tri += cnt;
+    }
+    if (at_leaf) {
+      if (!box_ok) {
+        live = false;
+      } else {
+        t_entry = tf;
+        if (r == kNoRope) {
+          live = false;
+        } else {
+          cur = r;
+          live = t_entry < t_exit && visits < cap;
+        }
+      }
+    }
+  }
+  if (COUNT) {
+    atomicAdd(&counts->entries, (unsigned long long)rc.ent);
+    atomicAdd(&counts->leaves, (unsigned long long)rc.leaf);
+    atomicAdd(&counts->tris, (unsigned long long)rc.tri);
+    atomicAdd(&counts->rays_hit, (unsigned long long)c_hit);
+  }
+}
+
 // Hit attributes of traced rays: the barycentrics of the hit triangle (the
 // Moller-Trumbore test of the hit repeated on the same transformed ray and
 // record, so u, v are the traversal's own), the interpolated vertex normal
@@ -384,12 +559,45 @@ int64_t trace_persistent_grid(int block) {
   return (int64_t)cus * (per_cu > 0 ? per_cu : 1);
 }
 
+int64_t trace_coop_refill_grid(int block) {
+  int dev = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, trace_coop_refill_kernel<false, true, true>, block,
+                                                    (size_t)(block / 64) * sizeof(CoopWave)) != hipSuccess)
+    return 0;
+  return (int64_t)cus * (per_cu > 0 ? per_cu : 1);
+}
+
 hipError_t launch_trace_closest(const TraceLaunch& L, hipStream_t s) {
   if (L.n <= 0) return hipSuccess;
   // Default work-group: 256 threads for the persistent kernel, one wave for
   // the one-ray-per-thread kernels (finer dispatch granularity at the tail;
   // 64 and 128 measured alike and ahead of 256 on the cluster workload).
   const int block = L.block > 0 ? L.block : (L.variant == 1 ? 256 : 64);
+  if (L.variant == 5) {
+    // Persistent cooperative waves: enough blocks to fill every CU at the
+    // kernel's occupancy, never more than the rays need.
+    const int64_t want = (L.n + block - 1) / block;
+    int64_t grid = L.grid_blocks > 0 ? L.grid_blocks : want;
+    if (grid > want) grid = want;
+    const DevScene& S5 = *L.scene;
+    const int32_t cap5 = (L.cap > 0 && L.cap < kMaxLeafVisits) ? L.cap : kMaxLeafVisits;
+#define RE_LAUNCH_REFILL(C, O, ONE)                                                                            \
+  hipLaunchKernelGGL((trace_coop_refill_kernel<C, O, ONE>), dim3((unsigned)grid), dim3(block),                 \
+                     (block / 64) * sizeof(CoopWave), s, S5.nodes, S5.leaves, S5.tris, S5.objs, S5.n_obj,        \
+                     L.ray_o, L.ray_d, L.n, cap5, L.secondary, L.prim, L.t, L.obj, L.counts, L.queue)
+    const bool one = S5.n_obj == 1;
+    if (L.counts) {
+      if (L.one_sided) { if (one) RE_LAUNCH_REFILL(true, true, true); else RE_LAUNCH_REFILL(true, true, false); }
+      else { if (one) RE_LAUNCH_REFILL(true, false, true); else RE_LAUNCH_REFILL(true, false, false); }
+    } else {
+      if (L.one_sided) { if (one) RE_LAUNCH_REFILL(false, true, true); else RE_LAUNCH_REFILL(false, true, false); }
+      else { if (one) RE_LAUNCH_REFILL(false, false, true); else RE_LAUNCH_REFILL(false, false, false); }
+    }
+#undef RE_LAUNCH_REFILL
+    return hipGetLastError();
+  }
   const DevScene& S = *L.scene;
   // A ray can alternate between two leaves when its exit point lies within
   // the rope tolerance of two faces; the visit cap is what ends it, so

@@ -58,11 +58,13 @@ def test_batch_kernel_item_loop_is_wave_uniform(tmp_path):
         assert "ds_bpermute" not in between and "ds_permute" not in between, f"{name}: shuffle of the item index"
 
 
-def test_persistent_trace_kernel_chunk_fetch_is_wave_uniform(tmp_path):
-    """trace_persistent_kernel (variant 1) takes chunks of rays from a device
-    counter with the same every-lane atomic and readfirstlane read-back."""
+@pytest.mark.parametrize("kernel", ["trace_persistent_kernel", "trace_coop_refill_kernel"])
+def test_persistent_trace_kernel_chunk_fetch_is_wave_uniform(tmp_path, kernel):
+    """The persistent closest-hit kernels (variants 1 and 5) take chunks of
+    rays from a device counter with the same every-lane atomic and
+    readfirstlane read-back."""
     src = ROOT / "rayengine_amd" / "csrc" / "kernels" / "trace.hip"
-    for name, body in _kernel_asm(tmp_path, "trace_persistent_kernel", src).items():
+    for name, body in _kernel_asm(tmp_path, kernel, src).items():
         atom = body.index("global_atomic_add_x2")
         window = body[atom:atom + 600]
         assert window.count("v_readfirstlane_b32") >= 2, f"{name}: chunk start not read back uniformly"

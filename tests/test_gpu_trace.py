@@ -24,7 +24,7 @@ def _parity(p, t, po, to):
     assert np.array_equal(t.view(np.uint32), to.view(np.uint32)), f"{int((t.view(np.uint32) != to.view(np.uint32)).sum())} t differ"
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5])
 @pytest.mark.parametrize("cap", [50, 1, 4096])
 def test_small_scene_bit_exact(gpu_ctx, small, cap, variant):
     say(f"small scene, cap {cap}, variant {variant}")
@@ -52,13 +52,13 @@ def test_two_sided_and_block_shapes(gpu_ctx, small):
     scene = gpu_ctx.upload_scene([re.SceneObject(tree)])
     po, to = oracle.trace(*tree.export(), tree.identity_object(), o, d, two_sided=True)
     for block in (64, 128, 256):
-        for variant in (0, 1, 2, 3, 4):
+        for variant in (0, 1, 2, 3, 4, 5):
             say(f"two-sided, block {block}, variant {variant}")
             p, t = scene.trace_closest(o, d, one_sided=False, block=block, variant=variant)
             _parity(p, t, po, to)
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5])
 def test_counters_match_oracle(gpu_ctx, small, variant):
     tri_v, tree, o, d = small
     scene = gpu_ctx.upload_scene([re.SceneObject(tree)])
@@ -87,7 +87,7 @@ def test_multi_object_transforms(gpu_ctx):
     o, d = re.camera_rays(re.Camera(pos=(0.3, 0.5, 6.0), view=(0.0, -0.1, -1.0)), 160, 120)
     for secondary in (False, True):
         po, to = oracle.trace(nodes, leaves, tris, obs, o, d, secondary=secondary)
-        for variant in (0, 1, 2, 3, 4):
+        for variant in (0, 1, 2, 3, 4, 5):
             say(f"multi-object, secondary={secondary}, variant {variant}")
             p, t, ob = scene.trace_closest(o, d, with_obj=True, secondary=secondary, variant=variant)
             _parity(p, t, po, to)
@@ -112,7 +112,7 @@ def test_degenerate_rays(gpu_ctx, small):
     d[512:1024] = -0.0
     d[512:1024, 2] = -1.0
     po, to = oracle.trace(nodes, leaves, tris, tree.identity_object(), o, d)
-    for variant in (0, 1, 4):
+    for variant in (0, 1, 4, 5):
         p, t = scene.trace_closest(o, d, variant=variant)
         _parity(p, t, po, to)
 
@@ -123,8 +123,12 @@ def test_empty_batch_and_empty_scene(gpu_ctx):
     p, t = scene.trace_closest(np.zeros((0, 3), np.float32), np.zeros((0, 3), np.float32))
     assert p.size == 0
     o, d = re.camera_rays(W.SMALL_CAMERA, 8, 8)
-    p, t = scene.trace_closest(o, d)
-    assert (p == -1).all() and (t == np.float32(1e20)).all()
+    for variant in (4, 5):
+        p, t = scene.trace_closest(o, d, variant=variant)
+        assert (p == -1).all() and (t == np.float32(1e20)).all()
+    # a variant outside 0..5 is refused
+    with pytest.raises(re.ReError):
+        scene.trace_closest(o, d, variant=6)
 
 
 def test_bad_arguments_report_status(gpu_ctx, small):
@@ -169,13 +173,13 @@ def test_cluster_scene_every_ray_bit_exact(gpu_ctx, cluster):
     say("oracle over every ray")
     po, to = oracle.trace(nodes, leaves, tris, tree.identity_object(), ro, rd)
     _parity(pa, ta, po, to)
-    for variant in (0, 1, 2, 4):
+    for variant in (0, 1, 2, 4, 5):
         say(f"cluster all rays, variant {variant}")
         pv, tv = scene.trace_closest(ro, rd, variant=variant)
         _parity(pv, tv, po, to)
     # work counters of the whole launch against the oracle's shortened walks
     _, _, co = oracle.trace(nodes, leaves, tris, tree.identity_object(), ro, rd, counts=True, cut=True)
-    for variant in (3, 4):
+    for variant in (3, 4, 5):
         _, _, cnt = scene.trace_counts(ro, rd, variant=variant)
         assert cnt[:3] == [int(x) for x in co[:, :3].sum(0)], f"variant {variant} counters"
     # work counters of the whole launch against the oracle's shortened walks
