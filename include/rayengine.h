@@ -115,6 +115,11 @@ typedef struct re_kd_build_params {
                              clipped polygon reaches, with the clipped box ("perfect
                              splits"); 0: by its bounding box (reference rule,
                              KDTree.cpp:581-600).  Default 1. */
+  int32_t split_rule;     /* 0: SAH (default); 1: the reference's spatial median, the
+                             midpoint lo + (hi - lo) / 2 of the node box's longest side
+                             (KDTree.cpp:192-199,368-389), with exact split boxes for the
+                             children instead of the reference's boxes grown by +-1e-4.
+                             Host build only (re_kd_build_gpu refuses 1). */
 } re_kd_build_params;
 
 typedef struct re_kd_info {
@@ -224,6 +229,18 @@ RE_API void re_trace_default_params(re_trace_params* p);
 RE_API re_status re_trace_closest(re_ctx* ctx, int32_t scene_id, const float* ray_o,
                                   const float* ray_d, int64_t n, const re_trace_params* params,
                                   int32_t* prim_id, float* t, int32_t* obj_id, uint32_t flags);
+/* Closest hit with a status byte per ray, a debugging aid with no
+ * counterpart in the reference (whose walk ends silently,
+ * ray_functions.cuh:284-347): prim_id and t as re_trace_closest, and in
+ * status how the walk of the last object ended - 0 t_entry >= t_exit (the
+ * normal end), 1 the leaf-visit cap, 2 a leaf slab test missed, 3 no exit
+ * face within the 1e-4 tolerance, 4 the exit rope is -1 (left the root box),
+ * 5 the root box was missed, 6 the walk's state repeated (stopped, its
+ * result final) - plus 0x80 when the ray's origin or direction holds a NaN or
+ * an infinity or the direction is zero.  One ray per thread, no tuning. */
+RE_API re_status re_trace_status(re_ctx* ctx, int32_t scene_id, const float* ray_o, const float* ray_d,
+                                 int64_t n, const re_trace_params* params, int32_t* prim_id, float* t,
+                                 uint8_t* status, uint32_t flags);
 
 /* Per-triangle hit attributes of a scene, the per-vertex data of
  * copy_memory's normals and uvs (RayEngine/src/cuda-src/cuda_memory_functions.cuh:58-211;

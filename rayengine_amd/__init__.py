@@ -120,12 +120,13 @@ class KDTree:
 
     def __init__(self, tri_v, max_leaf_tris: int = 8, max_depth: int = 24, cost_traversal: float = 1.0,
                  cost_intersect: float = 1.5, empty_bonus: float = 0.8, n_bins: int = 32, clip: bool = True,
-                 ctx: "Context | None" = None):
+                 ctx: "Context | None" = None, split_rule: int = 0):
         """Host build (re_kd_build), or with ctx the device build
-        (re_kd_build_gpu, the same tree; its device time in build_ms)."""
+        (re_kd_build_gpu, the same tree; its device time in build_ms).
+        split_rule 1: the reference's spatial median (host build only)."""
         self.tri_v = _f32c(tri_v, 9).reshape(-1, 9)
         p = _lib.re_kd_build_params(max_leaf_tris, max_depth, cost_traversal, cost_intersect, empty_bonus, n_bins,
-                                    int(clip))
+                                    int(clip), int(split_rule))
         h = C.c_void_p()
         self.build_ms = None
         if ctx is None:
@@ -709,6 +710,21 @@ class Scene:
         check(lib().re_trace_counts(self.ctx.handle, self.id, ptr(o), ptr(d), n, C.byref(p), ptr(prim), ptr(t),
                                     cnt, 0), self.ctx.handle)
         return prim, t, list(cnt)
+
+    def trace_status(self, o, d, **params):
+        """(prim, t, status) with each ray's status byte (re_trace_status): the
+        end of its last walk (0 normal, 1 visit cap, 2 leaf slab miss, 3 no
+        exit face, 4 rope -1, 5 root box missed, 6 repeated state) | 0x80
+        for non-finite input."""
+        o, d = _f32c(o, 3), _f32c(d, 3)
+        n = o.size // 3
+        prim = np.empty(n, np.int32)
+        t = np.empty(n, np.float32)
+        st = np.empty(n, np.uint8)
+        p = _trace_params(**params)
+        check(lib().re_trace_status(self.ctx.handle, self.id, ptr(o), ptr(d), n, C.byref(p), ptr(prim), ptr(t),
+                                    ptr(st), 0), self.ctx.handle)
+        return prim, t, st
 
     def trace_counts_device(self, o_ptr: int, d_ptr: int, n: int, prim_ptr: int, t_ptr: int, **params):
         cnt = (C.c_uint64 * 4)()

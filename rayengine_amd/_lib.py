@@ -44,7 +44,8 @@ class re_camera(C.Structure):
 
 class re_kd_build_params(C.Structure):
     _fields_ = [("max_leaf_tris", C.c_int32), ("max_depth", C.c_int32), ("cost_traversal", C.c_float),
-                ("cost_intersect", C.c_float), ("empty_bonus", C.c_float), ("n_bins", C.c_int32), ("clip", C.c_int32)]
+                ("cost_intersect", C.c_float), ("empty_bonus", C.c_float), ("n_bins", C.c_int32), ("clip", C.c_int32),
+                ("split_rule", C.c_int32)]
 
 
 class re_kd_info(C.Structure):
@@ -145,6 +146,8 @@ SIGNATURES = {
     "re_trace_default_params": (None, [C.POINTER(re_trace_params)]),
     "re_trace_closest": (C.c_int, [_P, C.c_int32, _P, _P, C.c_int64, C.POINTER(re_trace_params), _P, _P, _P,
                                    C.c_uint32]),
+    "re_trace_status": (C.c_int, [_P, C.c_int32, _P, _P, C.c_int64, C.POINTER(re_trace_params), _P, _P, _P,
+                                  C.c_uint32]),
     "re_trace_counts": (C.c_int, [_P, C.c_int32, _P, _P, C.c_int64, C.POINTER(re_trace_params), _P, _P, _U64,
                                   C.c_uint32]),
     "re_trace_occluded": (C.c_int, [_P, C.c_int32, _P, _P, C.c_int64, C.POINTER(re_trace_params), _P, C.c_uint32]),

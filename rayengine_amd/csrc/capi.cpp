@@ -526,6 +526,59 @@ RE_API re_status re_scene_set_attributes(re_ctx* ctx, int32_t scene_id, const fl
   });
 }
 
+RE_API re_status re_trace_status(re_ctx* ctx, int32_t scene_id, const float* ray_o, const float* ray_d, int64_t n,
+                                 const re_trace_params* params, int32_t* prim_id, float* t, uint8_t* status,
+                                 uint32_t flags) {
+  if (!ctx) return fail(RE_ERR_INVALID, "re_trace_status: null ctx");
+  if (n < 0 || (n > 0 && (!ray_o || !ray_d || !prim_id || !t || !status)))
+    return set_err(ctx, RE_ERR_INVALID, "re_trace_status: bad arguments");
+  auto it = ctx->scenes.find(scene_id);
+  if (it == ctx->scenes.end()) return set_err(ctx, RE_ERR_NOT_FOUND, "re_trace_status: unknown scene");
+  re_trace_params p;
+  re_trace_default_params(&p);
+  if (params) p = *params;
+  if (n == 0) return RE_OK;
+  HIP_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+  StatusLaunch L{};
+  L.scene = &it->second;
+  L.n = n;
+  L.cap = p.max_leaf_visits;
+  L.one_sided = p.one_sided;
+  L.secondary = p.secondary;
+  const bool dev = (flags & RE_DEVICE_PTRS) != 0;
+  if (!dev) {
+    const size_t rb = (size_t)n * 12;
+    re_status st = ensure_stage(ctx, stage_size({rb, rb, (size_t)n * 4, (size_t)n * 4, (size_t)n}));
+    if (st) return st;
+    StageCursor c{(char*)ctx->stage};
+    float* so = c.take<float>(3 * n);
+    float* sd = c.take<float>(3 * n);
+    L.prim = c.take<int32_t>(n);
+    L.t = c.take<float>(n);
+    L.status = c.take<uint8_t>(n);
+    HIP_TRY(ctx, hipMemcpyAsync(so, ray_o, rb, hipMemcpyHostToDevice, ctx->stream), "copy ray_o");
+    HIP_TRY(ctx, hipMemcpyAsync(sd, ray_d, rb, hipMemcpyHostToDevice, ctx->stream), "copy ray_d");
+    L.ray_o = so;
+    L.ray_d = sd;
+  } else {
+    L.ray_o = ray_o;
+    L.ray_d = ray_d;
+    L.prim = prim_id;
+    L.t = t;
+    L.status = status;
+  }
+  HIP_TRY(ctx, ctx->launch_begin(), "event");
+  HIP_TRY(ctx, launch_trace_status(L, ctx->stream), "trace status");
+  HIP_TRY(ctx, ctx->launch_end(), "event");
+  if (!dev) {
+    HIP_TRY(ctx, hipMemcpyAsync(prim_id, L.prim, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream), "copy prim");
+    HIP_TRY(ctx, hipMemcpyAsync(t, L.t, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream), "copy t");
+    HIP_TRY(ctx, hipMemcpyAsync(status, L.status, (size_t)n, hipMemcpyDeviceToHost, ctx->stream), "copy status");
+  }
+  if (!(flags & RE_ASYNC) || !dev) HIP_TRY(ctx, hipStreamSynchronize(ctx->stream), "status sync");
+  return RE_OK;
+}
+
 RE_API re_status re_hit_attributes(re_ctx* ctx, int32_t scene_id, const float* ray_o, const float* ray_d, int64_t n,
                                    const re_trace_params* params, const int32_t* prim_id, const int32_t* obj_id,
                                    float* normal, float* uv, float* bary, float* hit_point, uint32_t flags) {

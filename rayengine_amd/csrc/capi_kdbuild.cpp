@@ -33,6 +33,8 @@ RE_API re_status re_kd_build_gpu(re_ctx* ctx, const float* tri_v, int64_t n_tris
   if (params) p = *params;
   if (p.max_leaf_tris < 1 || p.max_depth < 1 || p.max_depth > 64 || p.n_bins > kdb::kMaxBins)
     return set_err(ctx, RE_ERR_INVALID, "re_kd_build_gpu: max_leaf_tris >= 1, max_depth in [1,64], n_bins <= 64");
+  if (p.split_rule != 0)
+    return set_err(ctx, RE_ERR_UNSUPPORTED, "re_kd_build_gpu: the spatial-median rule is built on the host (re_kd_build)");
   if (device_ms) *device_ms = 0.0f;
   // Root box and input check on the host, as re_kd_build does them.
   float rmin[3] = {kInf, kInf, kInf}, rmax[3] = {-kInf, -kInf, -kInf};

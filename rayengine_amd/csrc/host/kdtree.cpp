@@ -115,7 +115,26 @@ class Builder {
     return c;
   }
 
+  // The reference's spatial median (KDTree.cpp:192-199,368-389): the longest
+  // side of the node box (x on a strict win over y and z, else y on a strict
+  // win over z, else z) split at lo + (hi - lo) / 2.
+  Split median_split(const float* lo, const float* hi, const std::vector<Ref>& refs) const {
+    Split s;
+    const float ex = hi[0] - lo[0], ey = hi[1] - lo[1], ez = hi[2] - lo[2];
+    const int a = (ex > ey && ex > ez) ? 0 : (ey > ez ? 1 : 2);
+    const float pos = lo[a] + ((hi[a] - lo[a]) / 2.0f);
+    if (!(pos > lo[a] && pos < hi[a])) return s;  // no room for a split
+    s.axis = a;
+    s.pos = pos;
+    for (const Ref& r : refs) {
+      if (r.lo[a] < pos) ++s.nl;
+      if (r.hi[a] >= pos) ++s.nr;
+    }
+    return s;
+  }
+
   Split find_split(const float* lo, const float* hi, const std::vector<Ref>& refs) const {
+    if (p_.split_rule == 1) return median_split(lo, hi, refs);
     Split best;
     best.cost = INFINITY;
     float sa = area(lo, hi);
@@ -333,6 +352,7 @@ RE_API void re_kd_default_params(re_kd_build_params* p) {
   p->empty_bonus = 0.8f;
   p->n_bins = 32;
   p->clip = 1;
+  p->split_rule = 0;
 }
 
 RE_API re_status re_kd_build(const float* tri_v, int64_t n_tris, const re_kd_build_params* params,
@@ -344,8 +364,8 @@ RE_API re_status re_kd_build(const float* tri_v, int64_t n_tris, const re_kd_bui
   re_kd_build_params p;
   re_kd_default_params(&p);
   if (params) p = *params;
-  if (p.max_leaf_tris < 1 || p.max_depth < 1 || p.max_depth > 64)
-    return fail(RE_ERR_INVALID, "re_kd_build: max_leaf_tris >= 1 and max_depth in [1,64]");
+  if (p.max_leaf_tris < 1 || p.max_depth < 1 || p.max_depth > 64 || p.split_rule < 0 || p.split_rule > 1)
+    return fail(RE_ERR_INVALID, "re_kd_build: max_leaf_tris >= 1, max_depth in [1,64], split_rule 0 or 1");
   re_kdtree* tree = new (std::nothrow) re_kdtree();
   if (!tree) return fail(RE_ERR_OOM, "re_kd_build: out of host memory");
   try {

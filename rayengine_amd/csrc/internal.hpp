@@ -55,6 +55,23 @@ struct AttrLaunch {
 };
 hipError_t launch_hit_attributes(const AttrLaunch& L, hipStream_t s);
 
+// Closest hit with each ray's status (a debugging aid): how the walk of the
+// last object ended, as the oracle numbers it (0 t_entry >= t_exit, 1 visit
+// cap, 2 leaf slab miss, 3 no exit face within the tolerance, 4 rope -1,
+// 5 root box missed, 6 walk state repeated), plus 0x80 when the ray's origin
+// or direction holds a NaN or an infinity, or the direction is zero.
+struct StatusLaunch {
+  const DevScene* scene;
+  const float* ray_o;
+  const float* ray_d;
+  int64_t n;
+  int32_t cap, one_sided, secondary;
+  int32_t* prim;
+  float* t;
+  uint8_t* status;
+};
+hipError_t launch_trace_status(const StatusLaunch& L, hipStream_t s);
+
 // Per-ray work counters of a traversal: node entries read, leaf records read,
 // triangle records tested.
 struct TraceCounts {

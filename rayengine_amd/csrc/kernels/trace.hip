@@ -446,6 +446,81 @@ trace_coop_refill_kernel(RE_CLOSEST_PARAMS, unsigned long long* __restrict__ que
   }
 }
 
+// Closest hit with each ray's status: trace_ray's walk (same visits, same
+// cap and repeated-state stop, so the same prim and t) with the way the walk
+// of the last object ended recorded as the oracle numbers it, and a flag for
+// non-finite input.  A debugging aid: one ray per thread, no specialisation.
+template <bool ONE_SIDED>
+__global__ void __launch_bounds__(256) trace_status_kernel(const NodeEntry* __restrict__ nodes,
+                                                           const DevLeaf* __restrict__ leaves,
+                                                           const TriRec* __restrict__ tris,
+                                                           const ObjXform* __restrict__ objs, int32_t n_obj,
+                                                           const float* __restrict__ ray_o,
+                                                           const float* __restrict__ ray_d, int64_t n, int32_t cap,
+                                                           int32_t secondary, int32_t* __restrict__ prim_out,
+                                                           float* __restrict__ t_out, uint8_t* __restrict__ status) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const f3 o{ray_o[3 * i], ray_o[3 * i + 1], ray_o[3 * i + 2]};
+  const f3 d{ray_d[3 * i], ray_d[3 * i + 1], ray_d[3 * i + 2]};
+  const bool finite = isfinite(o.x) && isfinite(o.y) && isfinite(o.z) && isfinite(d.x) && isfinite(d.y) &&
+                      isfinite(d.z) && (d.x != 0.0f || d.y != 0.0f || d.z != 0.0f);
+  float best_t = kInf;
+  int32_t best_prim = -1;
+  uint8_t reason = 5;
+  unsigned c_tri = 0;
+  for (int32_t k = 0; k < n_obj; ++k) {
+    const ObjXform ob = objs[k];
+    f3 ro, rd;
+    xform_ray(ob, o, d, secondary != 0, ro, rd);
+    const f3 fr{1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z};
+    float t_entry, t_exit;
+    if (!ray_box_inv(ob.root_min, ob.root_max, ro, fr, t_entry, t_exit)) {
+      reason = 5;
+      continue;
+    }
+    int32_t cur = ob.root, hit_prim = -1, visits = 0;
+    bool found = false;
+    float hit_t = kInf;
+    WalkMemo memo;
+    uint8_t why = 255;
+    while (t_entry < t_exit && visits < cap) {
+      if (memo.repeats(cur, t_entry, t_exit)) {
+        why = 6;
+        break;
+      }
+      ++visits;
+      const f3 pe = add(ro, smul(t_entry, rd));
+      unsigned c_ent = 0;
+      const uint32_t leaf_id =
+          ref_is_leaf(cur) ? ref_leaf_id(cur) : descend<false>(nodes, cur, load_window(nodes, cur), pe, c_ent);
+      const DevLeaf L = leaves[leaf_id];
+      test_leaf<ONE_SIDED, false>(tris + L.first, L.count, ro, rd, t_exit, found, hit_t, hit_prim, c_tri);
+      float tn, tf;
+      if (!ray_box_inv(L.bmin, L.bmax, ro, fr, tn, tf)) {
+        why = 2;
+        break;
+      }
+      t_entry = tf;
+      const int face = exit_face(L.bmin, L.bmax, add(ro, smul(t_entry, rd)));
+      const int32_t r = face >= 0 ? L.rope[face] : kNoRope;
+      if (r == kNoRope) {
+        why = face < 0 ? 3 : 4;
+        break;
+      }
+      cur = r;
+    }
+    reason = why != 255 ? why : (t_entry < t_exit ? 1 : 0);
+    if (found && hit_t < best_t) {
+      best_t = hit_t;
+      best_prim = hit_prim;
+    }
+  }
+  prim_out[i] = best_prim;
+  t_out[i] = best_t;
+  status[i] = (uint8_t)(reason | (finite ? 0u : 0x80u));
+}
+
 // Hit attributes of traced rays: the barycentrics of the hit triangle (the
 // Moller-Trumbore test of the hit repeated on the same transformed ray and
 // record, so u, v are the traversal's own), the interpolated vertex normal
@@ -540,6 +615,20 @@ hipError_t launch_hit_attributes(const AttrLaunch& L, hipStream_t s) {
   hipLaunchKernelGGL(hit_attr_kernel, dim3((unsigned)((L.n + 255) / 256)), dim3(256), 0, s, S.objs, S.attr_geom,
                      S.attr_nrm, S.attr_uv, L.ray_o, L.ray_d, L.prim, L.obj, L.n, S.n_attr, S.n_obj, L.one_sided,
                      L.secondary, L.normal, L.uv, L.bary, L.hit_point);
+  return hipGetLastError();
+}
+
+hipError_t launch_trace_status(const StatusLaunch& L, hipStream_t s) {
+  if (L.n <= 0) return hipSuccess;
+  const DevScene& S = *L.scene;
+  const int32_t cap = (L.cap > 0 && L.cap < kMaxLeafVisits) ? L.cap : kMaxLeafVisits;
+  const dim3 grid((unsigned)((L.n + 255) / 256));
+  if (L.one_sided)
+    hipLaunchKernelGGL(trace_status_kernel<true>, grid, dim3(256), 0, s, S.nodes, S.leaves, S.tris, S.objs, S.n_obj,
+                       L.ray_o, L.ray_d, L.n, cap, L.secondary, L.prim, L.t, L.status);
+  else
+    hipLaunchKernelGGL(trace_status_kernel<false>, grid, dim3(256), 0, s, S.nodes, S.leaves, S.tris, S.objs,
+                       S.n_obj, L.ray_o, L.ray_d, L.n, cap, L.secondary, L.prim, L.t, L.status);
   return hipGetLastError();
 }
 

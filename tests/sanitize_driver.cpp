@@ -159,6 +159,12 @@ int main(int argc, char** argv) {
   trace_tree(clu, 2000, &p);
   p.max_depth = 2;
   trace_tree(clu, 2000, &p);
+  // the reference's spatial-median rule with bounding-box assignment
+  re_kd_default_params(&p);
+  p.split_rule = 1;
+  p.clip = 0;
+  p.max_leaf_tris = 20;
+  trace_tree(rnd, 500, &p);
   // degenerate triangles (zero area, repeated vertices) next to good ones
   std::vector<float> degen(ico);
   for (int k = 0; k < 9; ++k) degen[k] = 0.25f;

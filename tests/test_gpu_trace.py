@@ -264,3 +264,49 @@ def test_hit_attributes_multi_object(gpu_ctx):
         _attr_parity(got, want)
         assert not np.isnan(got[0][p >= 0]).any()
     scene.release()
+
+
+def test_median_tree_bit_exact(gpu_ctx, small):
+    """The reference's spatial-median tree (split_rule 1, bounding-box
+    assignment) traced on the device equals the oracle's walk of it."""
+    tri_v, _, o, d = small
+    tree = re.KDTree(tri_v, split_rule=1, clip=False)
+    scene = gpu_ctx.upload_scene([re.SceneObject(tree)])
+    po, to = oracle.trace(*tree.export(), tree.identity_object(), o, d)
+    for variant in (0, 3, 4, 5):
+        say(f"median tree, variant {variant}")
+        p, t = scene.trace_closest(o, d, variant=variant)
+        _parity(p, t, po, to)
+    assert (p >= 0).sum() > 8000
+    scene.release()
+    with pytest.raises(re.ReError):
+        re.KDTree(tri_v, split_rule=1, ctx=gpu_ctx)   # the device build has the SAH rule only
+
+
+def test_trace_status_matches_oracle_end_reasons(gpu_ctx, small):
+    """re_trace_status: prim and t as the closest-hit query, and each ray's
+    end reason equal to the oracle's (the walk that stops on a repeated
+    state, cut=True); non-finite or zero directions flagged with 0x80."""
+    tri_v, tree, o, d = small
+    scene = gpu_ctx.upload_scene([re.SceneObject(tree)])
+    nodes, leaves, tris = tree.export()
+    rng = np.random.default_rng(12)
+    extra_o = rng.uniform(-3, 3, (512, 3)).astype(np.float32)
+    extra_d = np.zeros((512, 3), np.float32)
+    ax = rng.integers(0, 3, 512)
+    extra_d[np.arange(512), ax] = rng.choice([-1.0, 1.0], 512)
+    extra_d[:32] = 0.0
+    extra_d[32:64, 1] = np.nan
+    extra_o[64:96, 2] = np.inf
+    oo, dd = np.concatenate([o, extra_o]), np.concatenate([d, extra_d])
+    for cap in (50, 3):
+        say(f"trace status, cap {cap}")
+        po, to, co = oracle.trace(nodes, leaves, tris, tree.identity_object(), oo, dd, cap=cap, counts=True, cut=True)
+        p, t, st = scene.trace_status(oo, dd, max_leaf_visits=cap)
+        _parity(p, t, po, to)
+        assert np.array_equal(st & 0x7F, co[:, 3].astype(np.uint8))
+        bad = ~(np.isfinite(oo).all(1) & np.isfinite(dd).all(1) & (dd != 0).any(1))
+        assert np.array_equal((st & 0x80) != 0, bad) and bad.sum() == 96
+        assert set(np.unique(st & 0x7F)) >= {0, 5}
+    assert (st & 0x7F == 1).sum() > 0          # some walks reach a cap of 3
+    scene.release()

@@ -248,3 +248,51 @@ def test_small_scene_golden(small):
     p, t = oracle.trace(nodes, leaves, tris, tree.identity_object(), o, d, cap=50)
     assert np.array_equal(p, g["prim"])
     assert np.array_equal(t.view(np.uint32), g["t_bits"])
+
+
+# ------------------------------------------------------- reference median rule
+
+def test_median_rule_splits_at_box_midpoints(small):
+    """split_rule 1 is the reference's spatial median (KDTree.cpp:192-199,
+    368-389): every inner node splits its box's longest side (x on a strict
+    win, then y on a strict win over z, else z) at lo + (hi - lo) / 2 in
+    FP32; triangles go below when their minimum is < the split and above when
+    their maximum is >= it (KDTree.cpp:581-600, clip off)."""
+    tri_v = small[0]
+    t = re.KDTree(tri_v, split_rule=1, clip=False, max_leaf_tris=20, max_depth=12)
+    nodes, leaves, tris = t.export()
+    _check_descent_bounds(nodes, len(leaves))
+    boxes = _node_boxes(nodes, t.info["root_min"], t.info["root_max"])
+    n_inner = 0
+    for e, (lo, hi) in boxes.items():
+        ax = int(nodes[e]["meta"] >> 30)
+        if ax == 3:
+            continue
+        n_inner += 1
+        ext = hi - lo
+        want = 0 if (ext[0] > ext[1] and ext[0] > ext[2]) else (1 if ext[1] > ext[2] else 2)
+        assert ax == want
+        assert nodes[e]["split"] == np.float32(lo[ax] + np.float32((hi[ax] - lo[ax]) / np.float32(2)))
+    assert n_inner > 100
+    # bounding-box assignment: every triangle whose box reaches into a leaf is in it
+    tv = tri_v.reshape(-1, 3, 3)
+    tmin, tmax = tv.min(1), tv.max(1)
+    for l in leaves[:200]:
+        inside = set(tris[l["first"]:l["first"] + l["count"]]["prim"].tolist())
+        for p in inside:
+            assert (tmin[p] <= l["bmax"]).all() and (tmax[p] >= l["bmin"]).all()
+    # the SAH tree of the same soup is a different tree
+    assert t.info["n_leaves"] != re.KDTree(tri_v).info["n_leaves"]
+
+
+def test_median_tree_traversal_equals_brute_force(small):
+    tri_v, _, o, d = small
+    t = re.KDTree(tri_v, split_rule=1, clip=False)
+    nodes, leaves, tris = t.export()
+    p, tt, cnt = oracle.trace(nodes, leaves, tris, t.identity_object(), o, d, cap=1 << 12, counts=True)
+    pb, tb = oracle.brute(tri_v, t.info["root_min"], t.info["root_max"], o, d)
+    ok = np.isin(cnt[:, 3], [0, 4, 5])
+    assert ok.mean() > 0.99
+    assert np.array_equal(tt[ok].view(np.uint32), tb[ok].view(np.uint32))
+    with pytest.raises(re.ReError):
+        re.KDTree(tri_v, split_rule=2)
