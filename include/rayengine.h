@@ -229,6 +229,15 @@ RE_API void re_trace_default_params(re_trace_params* p);
 RE_API re_status re_trace_closest(re_ctx* ctx, int32_t scene_id, const float* ray_o,
                                   const float* ray_d, int64_t n, const re_trace_params* params,
                                   int32_t* prim_id, float* t, int32_t* obj_id, uint32_t flags);
+/* Bounds-checked debug build (librayengine_debug.so, built with
+ * RE_DEBUG_BOUNDS): the closest-hit, any-hit and status kernels check every
+ * node entry, leaf record and triangle run index against the scene's arrays
+ * before reading it; an index outside sets a bit (1 node entry, 2 leaf
+ * record, 4 triangle run) and the read goes to index 0 instead.  This call
+ * returns the bits set since the last call (after the context's stream has
+ * finished) and clears them; RE_ERR_UNSUPPORTED in the normal build.  No
+ * counterpart in the reference. */
+RE_API re_status re_debug_bounds_errors(re_ctx* ctx, uint32_t* mask);
 /* Closest hit with a status byte per ray, a debugging aid with no
  * counterpart in the reference (whose walk ends silently,
  * ray_functions.cuh:284-347): prim_id and t as re_trace_closest, and in

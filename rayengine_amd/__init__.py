@@ -204,6 +204,14 @@ class Context:
         check(lib().re_last_kernel_ms(self._h, C.byref(v)), self._h)
         return v.value
 
+    def debug_bounds_errors(self) -> int:
+        """Bounds-check bits set by the trace kernels since the last call (1
+        node entry, 2 leaf record, 4 triangle run), after the stream finishes;
+        only in the debug library (RE_LIBRARY=debug), ReError otherwise."""
+        v = C.c_uint32()
+        check(lib().re_debug_bounds_errors(self._h, C.byref(v)), self._h)
+        return int(v.value)
+
     def kernel_times(self, max_n: int = 256) -> np.ndarray:
         """Durations (ms) of the last launches, oldest first (HIP events)."""
         buf = np.zeros(max_n, np.float32)

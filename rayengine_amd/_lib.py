@@ -6,11 +6,14 @@ loudly when it is missing: there is no fallback implementation.
 from __future__ import annotations
 
 import ctypes as C
+import os
 from pathlib import Path
 
 import numpy as np
 
-LIB_PATH = Path(__file__).resolve().parent / "librayengine.so"
+# RE_LIBRARY=debug selects the bounds-checked debug build (librayengine_debug.so)
+LIB_PATH = Path(__file__).resolve().parent / ("librayengine_debug.so" if os.environ.get("RE_LIBRARY") == "debug"
+                                              else "librayengine.so")
 
 RE_OK = 0
 RE_DEVICE_PTRS = 0x1
@@ -146,6 +149,7 @@ SIGNATURES = {
     "re_trace_default_params": (None, [C.POINTER(re_trace_params)]),
     "re_trace_closest": (C.c_int, [_P, C.c_int32, _P, _P, C.c_int64, C.POINTER(re_trace_params), _P, _P, _P,
                                    C.c_uint32]),
+    "re_debug_bounds_errors": (C.c_int, [_P, C.POINTER(C.c_uint32)]),
     "re_trace_status": (C.c_int, [_P, C.c_int32, _P, _P, C.c_int64, C.POINTER(re_trace_params), _P, _P, _P,
                                   C.c_uint32]),
     "re_trace_counts": (C.c_int, [_P, C.c_int32, _P, _P, C.c_int64, C.POINTER(re_trace_params), _P, _P, _U64,

@@ -526,6 +526,17 @@ RE_API re_status re_scene_set_attributes(re_ctx* ctx, int32_t scene_id, const fl
   });
 }
 
+RE_API re_status re_debug_bounds_errors(re_ctx* ctx, uint32_t* mask) {
+  if (!ctx || !mask) return fail(RE_ERR_INVALID, "re_debug_bounds_errors: null argument");
+  HIP_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream), "debug bounds sync");
+  const hipError_t e = debug_bounds_take(mask);
+  if (e == hipErrorNotSupported)
+    return set_err(ctx, RE_ERR_UNSUPPORTED, "re_debug_bounds_errors: library built without RE_DEBUG_BOUNDS");
+  if (e != hipSuccess) return hip_err(ctx, e, "re_debug_bounds_errors");
+  return RE_OK;
+}
+
 RE_API re_status re_trace_status(re_ctx* ctx, int32_t scene_id, const float* ray_o, const float* ray_d, int64_t n,
                                  const re_trace_params* params, int32_t* prim_id, float* t, uint8_t* status,
                                  uint32_t flags) {

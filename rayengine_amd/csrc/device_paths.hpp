@@ -7,6 +7,39 @@
 
 namespace re {
 
+// Bounds-checked debug build (compiled with RE_DEBUG_BOUNDS, in the
+// translation unit that defines RE_DEBUG_BOUNDS_HERE: the single-scene trace
+// kernels): every node window, leaf record and triangle run a walk reads is
+// checked against the scene's arrays first.  An index outside them sets a bit
+// in g_dbg_errors (1 node entry, 2 leaf record, 4 triangle run) and is
+// replaced by 0 (an empty run), so the walk goes on without reading outside
+// the arrays; its result is then meaningless and the error word says so.
+#if defined(RE_DEBUG_BOUNDS) && defined(RE_DEBUG_BOUNDS_HERE)
+struct DebugBounds {
+  uint32_t n_entries, n_leaves, n_tris, pad;
+};
+static __constant__ DebugBounds g_dbg_bounds;
+static __device__ unsigned int g_dbg_errors;
+__device__ __forceinline__ uint32_t dbg_index(uint32_t i, uint32_t n, unsigned bit) {
+  if (i < n) return i;
+  atomicOr(&g_dbg_errors, bit);
+  return 0u;
+}
+__device__ __forceinline__ void dbg_run(uint32_t& first, uint32_t& count) {
+  if ((uint64_t)first + count <= g_dbg_bounds.n_tris) return;
+  atomicOr(&g_dbg_errors, 4u);
+  first = 0u;
+  count = 0u;
+}
+#define RE_DBG_ENTRY(c) ((int32_t)dbg_index((uint32_t)(c), g_dbg_bounds.n_entries, 1u))
+#define RE_DBG_LEAF(l) dbg_index((uint32_t)(l), g_dbg_bounds.n_leaves, 2u)
+#define RE_DBG_RUN(f, n) dbg_run((f), (n))
+#else
+#define RE_DBG_ENTRY(c) (c)
+#define RE_DBG_LEAF(l) (l)
+#define RE_DBG_RUN(f, n) ((void)0)
+#endif
+
 struct TriGeom {
   f3 v0, e1, e2;
 };
@@ -63,7 +96,7 @@ struct RayCounts {
 static_assert(kNodeWindow == 2, "the descent reads two-entry windows");
 __device__ __forceinline__ uint4 load_window(const NodeEntry* __restrict__ nodes, int32_t c) {
   uint4 q;
-  __builtin_memcpy(&q, nodes + c, sizeof(q));
+  __builtin_memcpy(&q, nodes + RE_DBG_ENTRY(c), sizeof(q));
   return q;
 }
 
@@ -166,11 +199,12 @@ __device__ __forceinline__ void trace_ray(const NodeEntry* __restrict__ nodes, c
       // fetched while the triangles are.  Results are unchanged: the
       // reference's order (triangles, then exit, :295-347) only matters for
       // what the loop does next, which still waits for the tests.
-      const DevLeaf* lp = leaves + leaf_id;
+      const DevLeaf* lp = leaves + RE_DBG_LEAF(leaf_id);
       const float4 h0 = *reinterpret_cast<const float4*>(lp->bmin);
       const float4 h1 = *reinterpret_cast<const float4*>(lp->bmax + 1);
       const float bmin[3] = {h0.x, h0.y, h0.z}, bmax[3] = {h0.w, h1.x, h1.y};
-      const uint32_t first = __float_as_uint(h1.z), count = __float_as_uint(h1.w);
+      uint32_t first = __float_as_uint(h1.z), count = __float_as_uint(h1.w);
+      RE_DBG_RUN(first, count);
       if (COUNT) ++rc.leaf;
       float tn, tf;
       const bool box_ok = ray_box_inv(bmin, bmax, ro, fr, tn, tf);
@@ -413,12 +447,13 @@ __device__ __forceinline__ void coop_trace_wave(const NodeEntry* __restrict__ no
           if (!have_window) w = load_window(nodes, cur);
           leaf_id = descend<COUNT>(nodes, cur, w, pe, rc.ent);
         }
-        const DevLeaf* lp = leaves + leaf_id;
+        const DevLeaf* lp = leaves + RE_DBG_LEAF(leaf_id);
         const float4 h0 = *reinterpret_cast<const float4*>(lp->bmin);
         const float4 h1 = *reinterpret_cast<const float4*>(lp->bmax + 1);
         const float bmin[3] = {h0.x, h0.y, h0.z}, bmax[3] = {h0.w, h1.x, h1.y};
         first = __float_as_uint(h1.z);
         cnt = __float_as_uint(h1.w);
+        RE_DBG_RUN(first, cnt);
         if (COUNT) ++rc.leaf;
         float tn;
         box_ok = ray_box_inv(bmin, bmax, ro, fr, tn, tf);

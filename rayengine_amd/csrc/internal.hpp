@@ -71,6 +71,9 @@ struct StatusLaunch {
   uint8_t* status;
 };
 hipError_t launch_trace_status(const StatusLaunch& L, hipStream_t s);
+// The bounds-check error mask of the debug build since the last call (read
+// and cleared); hipErrorNotSupported in a build without RE_DEBUG_BOUNDS.
+hipError_t debug_bounds_take(uint32_t* mask);
 
 // Per-ray work counters of a traversal: node entries read, leaf records read,
 // triangle records tested.

@@ -11,6 +11,8 @@
 // One ray per lane; the node/leaf/triangle records are laid out for single
 // wide loads (8-byte entries, 64-byte leaves, 40-byte triangles in leaf
 // order), so each step of the pointer chase is one or a few cache lines.
+#define RE_DEBUG_BOUNDS_HERE 1
+#include <cstdlib>
 #include "../device_paths.hpp"
 
 namespace re {
@@ -196,10 +198,12 @@ __global__ void __launch_bounds__(256) trace_persistent_kernel(
         } else {
           leaf_id = descend<COUNT>(nodes, cur, load_window(nodes, cur), pe, c_ent);
         }
-        const DevLeaf* lp = leaves + leaf_id;
+        const DevLeaf* lp = leaves + RE_DBG_LEAF(leaf_id);
         const DevLeaf L = *lp;
         if (COUNT) ++c_leaf;
-        test_leaf<ONE_SIDED, COUNT>(tris + L.first, L.count, ro, rd, t_exit, found, hit_t, hit_prim, c_tri);
+        uint32_t lf = L.first, lc = L.count;
+        RE_DBG_RUN(lf, lc);
+        test_leaf<ONE_SIDED, COUNT>(tris + lf, lc, ro, rd, t_exit, found, hit_t, hit_prim, c_tri);
         float tn, tf;
         if (!ray_box_inv(L.bmin, L.bmax, ro, fr, tn, tf)) {
           live = false;
@@ -404,12 +408,13 @@ trace_coop_refill_kernel(RE_CLOSEST_PARAMS, unsigned long long* __restrict__ que
           if (!have_window) w = load_window(nodes, cur);
           leaf_id = descend<COUNT>(nodes, cur, w, pe, rc.ent);
         }
-        const DevLeaf* lp = leaves + leaf_id;
+        const DevLeaf* lp = leaves + RE_DBG_LEAF(leaf_id);
         const float4 h0 = *reinterpret_cast<const float4*>(lp->bmin);
         const float4 h1 = *reinterpret_cast<const float4*>(lp->bmax + 1);
         const float bmin[3] = {h0.x, h0.y, h0.z}, bmax[3] = {h0.w, h1.x, h1.y};
         first = __float_as_uint(h1.z);
         cnt = __float_as_uint(h1.w);
+        RE_DBG_RUN(first, cnt);
         if (COUNT) ++rc.leaf;
         float tn;
         box_ok = ray_box_inv(bmin, bmax, ro, fr, tn, tf);
@@ -494,8 +499,10 @@ __global__ void __launch_bounds__(256) trace_status_kernel(const NodeEntry* __re
       unsigned c_ent = 0;
       const uint32_t leaf_id =
           ref_is_leaf(cur) ? ref_leaf_id(cur) : descend<false>(nodes, cur, load_window(nodes, cur), pe, c_ent);
-      const DevLeaf L = leaves[leaf_id];
-      test_leaf<ONE_SIDED, false>(tris + L.first, L.count, ro, rd, t_exit, found, hit_t, hit_prim, c_tri);
+      const DevLeaf L = leaves[RE_DBG_LEAF(leaf_id)];
+      uint32_t lf = L.first, lc = L.count;
+      RE_DBG_RUN(lf, lc);
+      test_leaf<ONE_SIDED, false>(tris + lf, lc, ro, rd, t_exit, found, hit_t, hit_prim, c_tri);
       float tn, tf;
       if (!ray_box_inv(L.bmin, L.bmax, ro, fr, tn, tf)) {
         why = 2;
@@ -609,6 +616,29 @@ __global__ void zero_words_kernel(uint32_t* __restrict__ p, int32_t n) {
 
 }  // namespace
 
+// The scene's array sizes for the bounds checks of the debug build (a no-op
+// otherwise); the copy is synchronous, so launches on other streams of the
+// process see one scene's sizes at a time.
+static hipError_t set_debug_bounds(const DevScene& S, hipStream_t s) {
+#ifdef RE_DEBUG_BOUNDS
+  hipError_t e = hipStreamSynchronize(s);
+  DebugBounds b{(uint32_t)S.n_nodes, (uint32_t)S.n_leaves, (uint32_t)S.n_trirecs, 0u};
+  // RE_DEBUG_BOUNDS_LEAF_LIMIT=k checks leaf ids against min(k, leaves) instead:
+  // ids at or above k are then flagged and read as leaf 0 (still inside the
+  // array; walks end at their leaf-visit cap), which shows the check firing.
+  if (const char* lim = std::getenv("RE_DEBUG_BOUNDS_LEAF_LIMIT")) {
+    const long k = std::strtol(lim, nullptr, 10);
+    if (k > 0 && (uint64_t)k < b.n_leaves) b.n_leaves = (uint32_t)k;
+  }
+  if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(g_dbg_bounds), &b, sizeof(b), 0, hipMemcpyHostToDevice);
+  return e;
+#else
+  (void)S;
+  (void)s;
+  return hipSuccess;
+#endif
+}
+
 hipError_t launch_hit_attributes(const AttrLaunch& L, hipStream_t s) {
   if (L.n <= 0) return hipSuccess;
   const DevScene& S = *L.scene;
@@ -620,6 +650,10 @@ hipError_t launch_hit_attributes(const AttrLaunch& L, hipStream_t s) {
 
 hipError_t launch_trace_status(const StatusLaunch& L, hipStream_t s) {
   if (L.n <= 0) return hipSuccess;
+  {
+    const hipError_t e = set_debug_bounds(*L.scene, s);
+    if (e != hipSuccess) return e;
+  }
   const DevScene& S = *L.scene;
   const int32_t cap = (L.cap > 0 && L.cap < kMaxLeafVisits) ? L.cap : kMaxLeafVisits;
   const dim3 grid((unsigned)((L.n + 255) / 256));
@@ -658,8 +692,25 @@ int64_t trace_coop_refill_grid(int block) {
   return (int64_t)cus * (per_cu > 0 ? per_cu : 1);
 }
 
+hipError_t debug_bounds_take(uint32_t* mask) {
+#ifdef RE_DEBUG_BOUNDS
+  unsigned int v = 0, z = 0;
+  hipError_t e = hipMemcpyFromSymbol(&v, HIP_SYMBOL(g_dbg_errors), sizeof(v), 0, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(g_dbg_errors), &z, sizeof(z), 0, hipMemcpyHostToDevice);
+  *mask = v;
+  return e;
+#else
+  *mask = 0;
+  return hipErrorNotSupported;
+#endif
+}
+
 hipError_t launch_trace_closest(const TraceLaunch& L, hipStream_t s) {
   if (L.n <= 0) return hipSuccess;
+  {
+    const hipError_t e = set_debug_bounds(*L.scene, s);
+    if (e != hipSuccess) return e;
+  }
   // Default work-group: 256 threads for the persistent kernel, one wave for
   // the one-ray-per-thread kernels (finer dispatch granularity at the tail;
   // 64 and 128 measured alike and ahead of 256 on the cluster workload).
@@ -750,6 +801,10 @@ hipError_t launch_trace_closest(const TraceLaunch& L, hipStream_t s) {
 
 hipError_t launch_trace_occluded(const TraceLaunch& L, hipStream_t s) {
   if (L.n <= 0) return hipSuccess;
+  {
+    const hipError_t e = set_debug_bounds(*L.scene, s);
+    if (e != hipSuccess) return e;
+  }
   const int block = L.block > 0 ? L.block : 64;
   const DevScene& S = *L.scene;
   const int32_t cap = (L.cap > 0 && L.cap < kMaxLeafVisits) ? L.cap : kMaxLeafVisits;

@@ -29,6 +29,14 @@ def test_library_exports_every_declared_symbol():
     assert not missing, f"declared but not exported: {missing}"
 
 
+def test_debug_library_exports_every_declared_symbol():
+    dbg = LIB.with_name("librayengine_debug.so")
+    assert dbg.exists(), "build with python -m rayengine_amd.build --debug-bounds"
+    lib = ctypes.CDLL(str(dbg))
+    missing = [s for s in declared_symbols() if not hasattr(lib, s)]
+    assert not missing, f"declared but not exported: {missing}"
+
+
 def test_python_binding_covers_header():
     from rayengine_amd import _lib
     missing = [s for s in declared_symbols() if s not in _lib.SIGNATURES]
