@@ -12,6 +12,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <cstring>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -184,9 +185,34 @@ class RKDTreeCPU {
   re_kdtree* tree_ = nullptr;
 };
 
-// RKDThreeGPU (RayEngine/src/Primitives/KDThreeGPU.h:9) flattened the CPU
-// tree for upload; re_kd_build already produces the flattened arrays.
-using RKDThreeGPU = RKDTreeCPU;
+// RKDThreeGPU, RayEngine/src/Primitives/KDThreeGPU.h:9: the reference
+// flattened the CPU tree into upload arrays here (KDThreeGPU.cpp:155-194).
+// re_kd_build already produced them, so this view keeps the CPU tree (which
+// must outlive it) and answers the reference's getters from its arrays.
+class RKDThreeGPU {
+ public:
+  explicit RKDThreeGPU(RKDTreeCPU* cpu) : cpu_(cpu) {
+    if (!cpu) throw Error(RE_ERR_INVALID, "RKDThreeGPU: null tree");
+  }
+  const RKDTreeCPU* cpu() const { return cpu_; }
+  int num_faces() const { return cpu_->num_faces(); }
+  size_t get_num_faces() const { return (size_t)cpu_->num_faces(); }
+  size_t GetNumNodes() const { return (size_t)cpu_->info().n_nodes; }
+  size_t get_root_index() const { return 0; }  // the root is entry 0 of the preorder array
+  // The triangle ids in leaf order (the reference's obj_index_list): one per
+  // triangle record, straddler copies included.
+  std::vector<int> GetIndexList() const {
+    const re_kd_info i = cpu_->info();
+    std::vector<unsigned char> recs((size_t)i.n_trirecs * 40);
+    check(re_kd_export(cpu_->handle(), nullptr, nullptr, recs.data()));
+    std::vector<int> ids((size_t)i.n_trirecs);
+    for (size_t k = 0; k < ids.size(); ++k) std::memcpy(&ids[k], recs.data() + 40 * k + 36, sizeof(int));
+    return ids;
+  }
+
+ private:
+  RKDTreeCPU* cpu_;
+};
 
 namespace detail {
 struct State {
@@ -319,7 +345,8 @@ inline void copy_memory(const std::vector<RKDThreeGPU*>& trees, const RCamera& c
   if (s.scene >= 0) check(re_scene_release(c, s.scene), c);
   s.scene = -1;
   s.attributes = false;
-  s.trees.assign(trees.begin(), trees.end());
+  s.trees.clear();
+  for (const RKDThreeGPU* t : trees) s.trees.push_back(t->cpu());
   s.prim_offsets.clear();
   int32_t off = 0;
   for (const RKDThreeGPU* t : trees) {

@@ -8,6 +8,7 @@
 // points in the order SDFScene uses them (initialize_volume_render,
 // cuda_update_chunk, cuda_update_chunk_gen, generate_noise,
 // update_render_settings, cuda_render_frame) and dumps each volume.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -125,12 +126,16 @@ int main(int argc, char** argv) {
   for (int64_t f = 0; f < n_tris; ++f) faces[f] = uint3{(uint32_t)(3 * f), (uint32_t)(3 * f + 1), (uint32_t)(3 * f + 2)};
   RKDTreeCPU tree(verts.data(), faces.data(), nullptr, nullptr, (int)verts.size(), (int)faces.size(), 0, 0);
   re_kd_info info = tree.info();
-  std::printf("{\"n_tris\": %lld, \"n_leaves\": %lld, \"n_trirecs\": %lld, \"depth\": %d}\n", (long long)info.n_tris,
-              (long long)info.n_leaves, (long long)info.n_trirecs, info.depth);
+  RKDThreeGPU gpu_tree(&tree);  // RTScene::build_scene, RTScene.cpp:48
+  const std::vector<int> index_list = gpu_tree.GetIndexList();
+  std::printf("{\"n_tris\": %lld, \"n_leaves\": %lld, \"n_trirecs\": %lld, \"depth\": %d, \"index_list\": %zu, "
+              "\"index_max\": %d, \"n_nodes\": %zu}\n",
+              (long long)info.n_tris, (long long)info.n_leaves, (long long)info.n_trirecs, info.depth, index_list.size(),
+              index_list.empty() ? -1 : *std::max_element(index_list.begin(), index_list.end()), gpu_tree.GetNumNodes());
   if (!gpu) return 0;
   RCamera cam;
   cam.campos = float3{0, 0, 5};
-  std::vector<RKDThreeGPU*> trees = {&tree};
+  std::vector<RKDThreeGPU*> trees = {&gpu_tree};
   std::vector<GPUSceneObject> objs(1);
   objs[0].num_prims = (size_t)n_tris;
   // RTScene::init_triangles packing: v0, v1 - v0, v2 - v0 per triangle; the

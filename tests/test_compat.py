@@ -32,6 +32,10 @@ def test_compat_builds_tree_on_cpu(demo):
     tree = re.KDTree(re.icosphere(2, 1.0))
     assert info["n_tris"] == 320 and info["n_leaves"] == tree.info["n_leaves"]
     assert info["n_trirecs"] == tree.info["n_trirecs"]
+    # RKDThreeGPU's getters: one index per triangle record, the prim ids of the
+    # records in leaf order, and the tree's node-entry count
+    assert info["index_list"] == tree.info["n_trirecs"] and info["index_max"] == 319
+    assert info["n_nodes"] == tree.info["n_nodes"]
 
 
 @pytest.mark.gpu
