@@ -620,6 +620,45 @@ RE_API re_status re_batch_results(re_ctx* ctx, int32_t batch_id, int32_t* prim, 
 RE_API re_status re_batch_completion_ms(re_ctx* ctx, int32_t batch_id, float* ms);
 RE_API re_status re_batch_release(re_ctx* ctx, int32_t batch_id);
 
+/* ------------------------------------------------------ autotuner */
+
+/* Candidates of re_autotune_trace; a NULL list (or count 0) takes the default
+ * in brackets. */
+typedef struct re_tune_space {
+  const int32_t* leaf_sizes; /* kd-tree max_leaf_tris, the tree rebuilt on the device for each [4, 8, 12, 16] */
+  int32_t n_leaf_sizes;
+  const int32_t* blocks;     /* work-group sizes, 64, 128 or 256 [64, 128, 256] */
+  int32_t n_blocks;
+  const int32_t* variants;   /* re_trace_params.variant values [3, 4, 5] */
+  int32_t n_variants;
+  int32_t reps;              /* timed launches per candidate, at most 64 [10] */
+} re_tune_space;
+
+typedef struct re_tune_result {
+  int32_t max_leaf_tris;
+  int32_t block;
+  int32_t variant;
+  float ms; /* mean kernel time of the winning candidate (HIP events) */
+} re_tune_result;
+
+/* Picks the leaf size, work-group size and kernel variant of the closest-hit
+ * query for one scene (the soup tri_v, 9 floats per triangle, host memory)
+ * and a ray set, by timing on the context's device: for every leaf size the
+ * tree is built with re_kd_build_gpu (other fields from build, NULL =
+ * defaults) and uploaded; every (block, variant) pair is launched once
+ * untimed and then reps times, and the lowest mean kernel time wins (the
+ * first such candidate on a tie).  base supplies the other trace parameters
+ * (cap, one_sided, secondary; NULL = defaults).  Rays are host memory, or
+ * device memory with RE_DEVICE_PTRS in flags.  ms_all (nullable) receives
+ * every candidate's mean, leaf sizes outermost, variants innermost.  The
+ * tuner allocates its own device buffers for the results (and for host rays)
+ * and frees them, with the trees, before it returns.  No counterpart in the
+ * reference, whose launch shape was fixed (renderer.cu:389-410); the batched
+ * launch has its own tuner in the Python host mirror. */
+RE_API re_status re_autotune_trace(re_ctx* ctx, const float* tri_v, int64_t n_tris, const re_kd_build_params* build,
+                                   const float* ray_o, const float* ray_d, int64_t n, const re_trace_params* base,
+                                   const re_tune_space* space, uint32_t flags, re_tune_result* best, float* ms_all);
+
 /* Elapsed milliseconds of the last kernel launch of the context, measured
  * with HIP events recorded around the launch on its stream (waits for it).
  * Replaces the reference's FPS counter (RayEngine/src/Engine/Window.cpp:46-56). */

@@ -116,6 +116,16 @@ class re_batch_config(C.Structure):
                 ("side_share", C.c_float)]
 
 
+class re_tune_space(C.Structure):
+    _fields_ = [("leaf_sizes", C.POINTER(C.c_int32)), ("n_leaf_sizes", C.c_int32), ("blocks", C.POINTER(C.c_int32)),
+                ("n_blocks", C.c_int32), ("variants", C.POINTER(C.c_int32)), ("n_variants", C.c_int32),
+                ("reps", C.c_int32)]
+
+
+class re_tune_result(C.Structure):
+    _fields_ = [("max_leaf_tris", C.c_int32), ("block", C.c_int32), ("variant", C.c_int32), ("ms", C.c_float)]
+
+
 _P = C.c_void_p
 _F = C.POINTER(C.c_float)
 _I32 = C.POINTER(C.c_int32)
@@ -137,6 +147,8 @@ SIGNATURES = {
     "re_kd_default_params": (None, [C.POINTER(re_kd_build_params)]),
     "re_kd_build": (C.c_int, [_P, C.c_int64, C.POINTER(re_kd_build_params), C.POINTER(C.c_void_p)]),
     "re_kd_build_gpu": (C.c_int, [_P, _P, C.c_int64, _P, _P, _P]),
+    "re_autotune_trace": (C.c_int, [_P, _P, C.c_int64, _P, _P, _P, C.c_int64, _P, C.POINTER(re_tune_space), C.c_uint32,
+                                    C.POINTER(re_tune_result), _P]),
     "re_kd_free": (None, [_P]),
     "re_kd_get_info": (C.c_int, [_P, C.POINTER(re_kd_info)]),
     "re_kd_export": (C.c_int, [_P, _P, _P, _P]),

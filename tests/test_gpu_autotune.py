@@ -38,3 +38,27 @@ def test_tune_trace_small_scene(gpu_ctx):
     po, to = oracle.trace(*tree.export(), tree.identity_object(), o, d)
     assert np.array_equal(p, po) and np.array_equal(t.view(np.uint32), to.view(np.uint32))
     scene.release()
+
+
+def test_native_tuner_picks_first_minimum(gpu_ctx):
+    """re_autotune_trace: every leaf size x block x variant timed, the first
+    minimum of the recorded means returned, and that configuration traces
+    bit-exactly; bad candidate lists are refused."""
+    tri_v = W.small_soup()
+    o, d = re.camera_rays(W.SMALL_CAMERA, 128, 128)
+    best, meas = AT.tune_trace_native(gpu_ctx, tri_v, o, d, leaf_sizes=(4, 8), blocks=(64, 256), variants=(3, 4, 5),
+                                      reps=3)
+    print(best, flush=True)
+    assert len(meas) == 12 and all(m["ms"] > 0 for m in meas)
+    ms = [m["ms"] for m in meas]
+    first = int(np.argmin(ms))
+    assert best == {k: meas[first][k] for k in ("max_leaf_tris", "block", "variant", "ms")}
+    tree = re.KDTree(tri_v, max_leaf_tris=best["max_leaf_tris"])
+    scene = gpu_ctx.upload_scene([re.SceneObject(tree)])
+    p, t = scene.trace_closest(o, d, block=best["block"], variant=best["variant"])
+    po, to = oracle.trace(*tree.export(), tree.identity_object(), o, d)
+    assert np.array_equal(p, po) and np.array_equal(t.view(np.uint32), to.view(np.uint32))
+    scene.release()
+    for bad in (dict(blocks=(96,)), dict(variants=(6,)), dict(leaf_sizes=(0,)), dict(reps=65)):
+        with pytest.raises(re.ReError):
+            AT.tune_trace_native(gpu_ctx, tri_v, o[:256], d[:256], **bad)
