@@ -610,6 +610,20 @@ RE_API re_status re_batch_create(re_ctx* ctx, const re_batch_scene* scenes, int3
                                  const re_batch_tile* tiles, int32_t n_tiles, const float* ray_o,
                                  const float* ray_d, int64_t n_rays, const re_batch_config* config,
                                  int32_t* batch_id);
+/* The work items re_batch_create cuts for these targets, with no context or
+ * device: targets (slots: scenes 0..n_scenes-1, then tiles) in ascending ray
+ * count, ties in slot order, each target's rays cut into items of item_rays
+ * rays.  items receives 24-byte records {int32 kind (0 triangles, 1 exact
+ * march, 2 reference sphere trace), int32 target (scene or tile index),
+ * int64 first ray, int32 count, int32 slot}; surrogate tiles (mode 2) get no
+ * items (their launches run beside the persistent kernel) and count one
+ * completion.  remaining (nullable, n_scenes + n_tiles entries) receives each
+ * slot's item count: the counter the wave finishing a slot's last item takes
+ * to zero.  items may be NULL to query *n_items.  The tree pointers of scenes
+ * are not read.  No counterpart in the reference. */
+RE_API re_status re_batch_plan(const re_batch_scene* scenes, int32_t n_scenes, const re_batch_tile* tiles,
+                               int32_t n_tiles, int64_t n_rays, int32_t item_rays, void* items, int64_t max_items,
+                               int64_t* n_items, int32_t* remaining);
 /* Replays the captured graph (RE_ASYNC: do not wait). */
 RE_API re_status re_batch_run(re_ctx* ctx, int32_t batch_id, uint32_t flags);
 RE_API re_status re_batch_info(re_ctx* ctx, int32_t batch_id, int64_t* n_rays, int64_t* n_items,

@@ -555,6 +555,30 @@ def sample_points(dist_vol: "Volume", n: int, seed: int = 1, near_band: float = 
     return out
 
 
+BATCH_ITEM_DT = np.dtype([("kind", "<i4"), ("target", "<i4"), ("begin", "<i8"), ("count", "<i4"), ("slot", "<i4")])
+
+
+def batch_plan(scene_ranges, tiles, n_rays: int, item_rays: int = 256):
+    """re_batch_plan: the work items a batch would be cut into, on the host.
+    scene_ranges: (ray_begin, ray_count) per scene; tiles: (mode, ray_begin,
+    ray_count) per tile.  Returns (items as BATCH_ITEM_DT, remaining item
+    count per slot)."""
+    sc = (_lib.re_batch_scene * max(len(scene_ranges), 1))()
+    for i, (b, c) in enumerate(scene_ranges):
+        sc[i].tree, sc[i].ray_begin, sc[i].ray_count = None, int(b), int(c)
+    tl = (_lib.re_batch_tile * max(len(tiles), 1))()
+    for i, (mode, b, c) in enumerate(tiles):
+        tl[i].vol_id, tl[i].mode, tl[i].ray_begin, tl[i].ray_count, tl[i].mlp_id = -1, int(mode), int(b), int(c), -1
+    n = C.c_int64()
+    rem = np.zeros(max(len(scene_ranges) + len(tiles), 1), np.int32)
+    check(lib().re_batch_plan(sc, len(scene_ranges), tl, len(tiles), int(n_rays), int(item_rays), None, 0, C.byref(n),
+                              ptr(rem)))
+    items = np.zeros(n.value, BATCH_ITEM_DT)
+    check(lib().re_batch_plan(sc, len(scene_ranges), tl, len(tiles), int(n_rays), int(item_rays), ptr(items),
+                              n.value, C.byref(n), None))
+    return items, rem[:len(scene_ranges) + len(tiles)]
+
+
 class Batch:
     """A batched mixed workload captured in a hipGraph (re_batch_*).
 

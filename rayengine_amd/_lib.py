@@ -210,6 +210,8 @@ SIGNATURES = {
     "re_hybrid_trace": (C.c_int, [_P, C.c_int32, C.c_int32, C.POINTER(re_hybrid_params), _P, _P, C.c_int64, _P, _P,
                                   _P, _P, C.c_uint32]),
     "re_batch_default_config": (None, [C.POINTER(re_batch_config)]),
+    "re_batch_plan": (C.c_int, [C.POINTER(re_batch_scene), C.c_int32, C.POINTER(re_batch_tile), C.c_int32, C.c_int64,
+                                C.c_int32, _P, C.c_int64, _I64, _P]),
     "re_batch_create": (C.c_int, [_P, C.POINTER(re_batch_scene), C.c_int32, C.POINTER(re_batch_tile), C.c_int32, _P,
                                   _P, C.c_int64, C.POINTER(re_batch_config), _I32]),
     "re_batch_run": (C.c_int, [_P, C.c_int32, C.c_uint32]),

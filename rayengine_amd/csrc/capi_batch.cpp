@@ -135,44 +135,22 @@ RE_API re_status re_batch_create(re_ctx* ctx, const re_batch_scene* scenes, int3
     b.tile_mode.push_back(tiles[k].mode);
   }
   // Work items, targets in ascending ray count (shortest first).
-  std::vector<int32_t> order(b.n_slots);
-  std::iota(order.begin(), order.end(), 0);
-  auto count_of = [&](int32_t s) { return s < n_scenes ? scenes[s].ray_count : tiles[s - n_scenes].ray_count; };
-  std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t c) { return count_of(a) < count_of(c); });
-  std::vector<BatchItem> items;
-  std::vector<int32_t> rem(b.n_slots, 0);
-  for (int32_t s : order) {
-    int32_t kind;
-    int64_t begin, count;
-    int32_t target;
-    if (s < n_scenes) {
-      kind = kItemTri;
-      target = s;
-      begin = scenes[s].ray_begin;
-      count = scenes[s].ray_count;
-    } else {
-      const re_batch_tile& tl = tiles[s - n_scenes];
-      target = s - n_scenes;
-      begin = tl.ray_begin;
-      count = tl.ray_count;
-      if (tl.mode == 2) {
-        b.surrogate.push_back(SurrogateTile{target, s, begin, count, tl.mlp_id >= 0 ? tl.mlp_id : cfg.mlp_id});
-        rem[s] = 1;
-        continue;
-      }
-      kind = tl.mode == 0 ? kItemExact : kItemPlain;
-    }
-    for (int64_t off = 0; off < count; off += cfg.item_rays) {
-      const int32_t c = (int32_t)std::min<int64_t>(cfg.item_rays, count - off);
-      items.push_back(BatchItem{kind, target, begin + off, c, s});
-      ++rem[s];
+  BatchPlan plan;
+  {
+    const re_status st = plan_batch(scenes, n_scenes, tiles, n_tiles, n_rays, cfg.item_rays, plan);
+    if (st) {
+      free_batch(b);
+      const std::string msg = std::string("re_batch_create: ") + g_last_error;  // copied before set_err rewrites it
+      return set_err(ctx, st, msg);
     }
   }
+  for (int32_t s : plan.surrogate_slots) {
+    const re_batch_tile& tl = tiles[s - n_scenes];
+    b.surrogate.push_back(SurrogateTile{s - n_scenes, s, tl.ray_begin, tl.ray_count, tl.mlp_id >= 0 ? tl.mlp_id : cfg.mlp_id});
+  }
+  const std::vector<BatchItem>& items = plan.items;
+  const std::vector<int32_t>& rem = plan.remaining;
   b.n_items = (int64_t)items.size();
-  if (b.n_items > kMaxBatchItems) {
-    free_batch(b);
-    return set_err(ctx, RE_ERR_INVALID, "re_batch_create: more than 2^30 work items (raise item_rays)");
-  }
   hipError_t e = hipSuccess;
   auto alloc = [&](void** p, size_t bytes) {
     if (e == hipSuccess) e = hipMalloc(p, bytes > 0 ? bytes : 64);

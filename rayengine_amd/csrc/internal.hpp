@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "../../include/rayengine.h"
+#include "host/batch_plan.hpp"
 #include "mlp_layout.hpp"
 #include "re_math.hpp"
 #include "terrain_math.hpp"
@@ -247,16 +248,7 @@ hipError_t launch_hybrid_trace(const DevVolume& field, const void* pack, const f
                                const float* ray_d, int64_t n, const HybridParams& hp, bool use_mlp, float* t,
                                uint8_t* hit, int32_t* evals, int32_t* samples, hipStream_t s);
 
-// ---- batched work queue (kernels/batch.hip) ----
-enum BatchKind : int32_t { kItemTri = 0, kItemExact = 1, kItemPlain = 2 };
-
-struct BatchItem {
-  int32_t kind;    // BatchKind
-  int32_t target;  // scene object index (triangles) or tile index (terrain)
-  int64_t begin;   // first ray
-  int32_t count;   // rays in the item
-  int32_t slot;    // completion slot (scenes first, then tiles)
-};
+// ---- batched work queue (kernels/batch.hip); items: host/batch_plan.hpp ----
 
 // A terrain tile of a batch: its volume, box and exact-march parameters in
 // world units (scaled by that tile's own voxel size, as re_hybrid_trace
@@ -266,10 +258,6 @@ struct BatchTile {
   float box_min[3], box_max[3];
   HybridParams hp;
 };
-
-// Work items of one batch are counted by a 32-bit device counter, which each
-// wave also advances once past the end.
-constexpr int64_t kMaxBatchItems = (int64_t)1 << 30;
 
 struct BatchSdf {
   int32_t max_steps;

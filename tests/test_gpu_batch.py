@@ -39,6 +39,8 @@ def test_batch_equals_single_calls(gpu_ctx, small_batch, item_rays, block):
     b = re.Batch(gpu_ctx, [(t, s, c) for t, (s, c) in zip(trees, sr)],
                  [(v, m, s, c) for v, m, (s, c) in zip(vols, modes, tr)], ro, rd, item_rays=item_rays, block=block)
     say(f"batch items={b.items()} item_rays={item_rays} block={block}")
+    plan, _ = re.batch_plan(sr, [(m, s, c) for m, (s, c) in zip(modes, tr)], len(ro), item_rays)
+    assert b.items() == len(plan)            # the device queue holds the host plan's items
     refs = []
     for tree, (s, c) in zip(trees, sr):
         scene = gpu_ctx.upload_scene([re.SceneObject(tree)])
