@@ -25,6 +25,33 @@ struct LdsTable {
   __device__ uint32_t operator[](uint32_t i) const { return t[i]; }
 };
 
+__device__ __forceinline__ uint32_t wave_uniform(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
+}
+
+// One octave of noise_eval for the active lanes of a wave.  When they all lie
+// in one lattice cell, the cell's corner hashes are looked up once and kept in
+// scalar registers, so the eight gradient selections are scalar branches
+// taken by the whole wave; otherwise every lane looks up its own.  Either way
+// each lane's arithmetic is noise_eval's.  A wave covers a 4x4x4 voxel block;
+// on the 256^3 unit-cube terrain the finest octave's cells are four voxels
+// wide and aligned with those blocks, so every wave of it takes the first
+// path.
+__device__ __forceinline__ float noise_eval_wave(const LdsTable& P, f3 p) {
+  const int fx = (int)floorf(p.x), fy = (int)floorf(p.y), fz = (int)floorf(p.z);
+  const int sx = __builtin_amdgcn_readfirstlane(fx), sy = __builtin_amdgcn_readfirstlane(fy),
+            sz = __builtin_amdgcn_readfirstlane(fz);
+  const NoiseCell c = noise_cell(p, fx, fy, fz);
+  uint32_t h[8];
+  if (__builtin_amdgcn_ballot_w64(fx != sx || fy != sy || fz != sz) == 0ull) {
+    noise_hashes(P, sx, sy, sz, h);
+    for (int k = 0; k < 8; ++k) h[k] = wave_uniform(h[k]);
+    return noise_blend(h, c);
+  }
+  noise_hashes(P, fx, fy, fz, h);
+  return noise_blend(h, c);
+}
+
 __global__ void __launch_bounds__(256) noise_fill_kernel(float* __restrict__ out, const uint32_t* __restrict__ perm,
                                                          int32_t dx, int32_t dy, int32_t dz, int32_t bx, int32_t by,
                                                          float sx, float sy, float sz, float ox, float oy,
@@ -42,7 +69,7 @@ __global__ void __launch_bounds__(256) noise_fill_kernel(float* __restrict__ out
     float v = 0.0f;
     if (x < dx && y < dy && z < dz) {
       const f3 p{ox + (float)x * sx, oy + (float)y * sy, oz + (float)z * sz};
-      v = terrain_field(T, fp, p);
+      v = fbm_field(fp, p, [&](f3 q) { return noise_eval_wave(T, q); });
     }
     out[brick * kBrickVox + m] = v;
   }

@@ -61,34 +61,74 @@ RE_HD float grad_dot(uint32_t h, float x, float y, float z) {
 
 RE_HD float quintic(float t) { return t * t * t * (t * (t * 6 - 15) + 10); }
 
+// One octave of noise_eval split in two: the cell-relative coordinates and
+// fades of p in its lattice cell (fx, fy, fz = floor of p), and the blend of
+// the eight corner gradients given the corners' hashes h[0..7] (corner order
+// 000, 100, 010, 110, 001, 101, 011, 111 in x, y, z).
+struct NoiseCell {
+  float x0, x1, y0, y1, z0, z1, u, v, w;
+};
+
+RE_HD NoiseCell noise_cell(f3 p, int fx, int fy, int fz) {
+  NoiseCell c;
+  const float tx = p.x - (float)fx, ty = p.y - (float)fy, tz = p.z - (float)fz;
+  c.u = quintic(tx);
+  c.v = quintic(ty);
+  c.w = quintic(tz);
+  c.x0 = tx;
+  c.x1 = tx - 1;
+  c.y0 = ty;
+  c.y1 = ty - 1;
+  c.z0 = tz;
+  c.z1 = tz - 1;
+  return c;
+}
+
+RE_HD float noise_blend(const uint32_t* h, const NoiseCell& c) {
+  float a = grad_dot(h[0], c.x0, c.y0, c.z0);
+  float b = grad_dot(h[1], c.x1, c.y0, c.z0);
+  float cc = grad_dot(h[2], c.x0, c.y1, c.z0);
+  float d = grad_dot(h[3], c.x1, c.y1, c.z0);
+  float e = grad_dot(h[4], c.x0, c.y0, c.z1);
+  float f = grad_dot(h[5], c.x1, c.y0, c.z1);
+  float g = grad_dot(h[6], c.x0, c.y1, c.z1);
+  float hh = grad_dot(h[7], c.x1, c.y1, c.z1);
+  float k0 = a;
+  float k1 = (b - a);
+  float k2 = (cc - a);
+  float k3 = (e - a);
+  float k4 = (a + d - b - cc);
+  float k5 = (a + f - b - e);
+  float k6 = (a + g - cc - e);
+  float k7 = (b + cc + e + hh - a - d - f - g);
+  const float u = c.u, v = c.v, w = c.w;
+  return k0 + k1 * u + k2 * v + k3 * w + k4 * u * v + k5 * u * w + k6 * v * w + k7 * u * v * w;
+}
+
+// The corners' hashes of lattice cell (fx, fy, fz).
+template <typename Table>
+RE_HD void noise_hashes(const Table& P, int fx, int fy, int fz, uint32_t* h) {
+  const int xi0 = fx & 255, yi0 = fy & 255, zi0 = fz & 255;
+  const int xi1 = (xi0 + 1) & 255, yi1 = (yi0 + 1) & 255, zi1 = (zi0 + 1) & 255;
+  const uint32_t px0 = P[xi0], px1 = P[xi1];
+  const uint32_t p00 = P[px0 + yi0], p10 = P[px1 + yi0], p01 = P[px0 + yi1], p11 = P[px1 + yi1];
+  h[0] = P[p00 + zi0];
+  h[1] = P[p10 + zi0];
+  h[2] = P[p01 + zi0];
+  h[3] = P[p11 + zi0];
+  h[4] = P[p00 + zi1];
+  h[5] = P[p10 + zi1];
+  h[6] = P[p01 + zi1];
+  h[7] = P[p11 + zi1];
+}
+
 // P is the doubled table (512 entries, values 0..255).
 template <typename Table>
 RE_HD float noise_eval(const Table& P, f3 p) {
-  int fx = (int)floorf(p.x), fy = (int)floorf(p.y), fz = (int)floorf(p.z);
-  int xi0 = fx & 255, yi0 = fy & 255, zi0 = fz & 255;
-  int xi1 = (xi0 + 1) & 255, yi1 = (yi0 + 1) & 255, zi1 = (zi0 + 1) & 255;
-  float tx = p.x - (float)fx, ty = p.y - (float)fy, tz = p.z - (float)fz;
-  float u = quintic(tx), v = quintic(ty), w = quintic(tz);
-  float x0 = tx, x1 = tx - 1, y0 = ty, y1 = ty - 1, z0 = tz, z1 = tz - 1;
-  uint32_t px0 = P[xi0], px1 = P[xi1];
-  uint32_t p00 = P[px0 + yi0], p10 = P[px1 + yi0], p01 = P[px0 + yi1], p11 = P[px1 + yi1];
-  float a = grad_dot(P[p00 + zi0], x0, y0, z0);
-  float b = grad_dot(P[p10 + zi0], x1, y0, z0);
-  float c = grad_dot(P[p01 + zi0], x0, y1, z0);
-  float d = grad_dot(P[p11 + zi0], x1, y1, z0);
-  float e = grad_dot(P[p00 + zi1], x0, y0, z1);
-  float f = grad_dot(P[p10 + zi1], x1, y0, z1);
-  float g = grad_dot(P[p01 + zi1], x0, y1, z1);
-  float h = grad_dot(P[p11 + zi1], x1, y1, z1);
-  float k0 = a;
-  float k1 = (b - a);
-  float k2 = (c - a);
-  float k3 = (e - a);
-  float k4 = (a + d - b - c);
-  float k5 = (a + f - b - e);
-  float k6 = (a + g - c - e);
-  float k7 = (b + c + e + h - a - d - f - g);
-  return k0 + k1 * u + k2 * v + k3 * w + k4 * u * v + k5 * u * w + k6 * v * w + k7 * u * v * w;
+  const int fx = (int)floorf(p.x), fy = (int)floorf(p.y), fz = (int)floorf(p.z);
+  uint32_t h[8];
+  noise_hashes(P, fx, fy, fz, h);
+  return noise_blend(h, noise_cell(p, fx, fy, fz));
 }
 
 struct FbmParams {
@@ -100,15 +140,22 @@ struct FbmParams {
 // lacunarity^o), i.e. minus the density fBm(p) - (p.y - y_offset).  Positive
 // above the surface, like the reference's terrain distance
 // (RayEngine/src/cuda-src/sdf_functions.cuh:476-503).
-template <typename Table>
-RE_HD float terrain_field(const Table& P, const FbmParams& fp, f3 p) {
+// noise(q) is one octave of the noise at q (noise_eval, or an equivalent
+// evaluation of it).
+template <typename Noise>
+RE_HD float fbm_field(const FbmParams& fp, f3 p, const Noise& noise) {
   float sum = 0.0f, amp = 1.0f, freq = fp.base_freq;
   for (int o = 0; o < fp.octaves; ++o) {
-    sum = sum + amp * noise_eval(P, f3{p.x * freq, p.y * freq, p.z * freq});
+    sum = sum + amp * noise(f3{p.x * freq, p.y * freq, p.z * freq});
     freq = freq * fp.lacunarity;
     amp = amp * fp.gain;
   }
   return (p.y - fp.y_offset) - sum;
+}
+
+template <typename Table>
+RE_HD float terrain_field(const Table& P, const FbmParams& fp, f3 p) {
+  return fbm_field(fp, p, [&](f3 q) { return noise_eval(P, q); });
 }
 
 // Trilinear sample at world point p: grid coordinate c = p / spacing, cell
