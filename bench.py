@@ -216,6 +216,17 @@ def traffic_probe(args):
     print(json.dumps({"probe": "ok", "rays": n}), flush=True)
 
 
+def noise_probe(args):
+    """The program the committed noise-fill counter profile runs: the 256^3
+    terrain volume of the terrain section generated 5 times."""
+    import torch  # noqa: F401  (one HIP runtime: torch's)
+    import rayengine_amd as re
+    ctx = re.Context(0)
+    for _ in range(5):
+        ctx.noise_volume(dim=(256, 256, 256), spacing=(1 / 256,) * 3, seed=2016).release()
+    print(json.dumps({"probe": "ok", "noise_ms": round(ctx.last_kernel_ms(), 4)}), flush=True)
+
+
 # --------------------------------------------------------------------------- sections
 
 def launch_stats(ctx, fn, steps):
@@ -734,6 +745,7 @@ def main():
     ap.add_argument("--autotune", action="store_true",
                     help="pick leaf size, ray tile, block and variant by measurement before timing")
     ap.add_argument("--traffic-probe", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--noise-probe", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--probe-leaf", type=int, default=8, help=argparse.SUPPRESS)
     ap.add_argument("--probe-tile", type=int, default=8, help=argparse.SUPPRESS)
     ap.add_argument("--probe-block", type=int, default=64, help=argparse.SUPPRESS)
@@ -741,6 +753,9 @@ def main():
     args = ap.parse_args()
     if args.traffic_probe:
         traffic_probe(args)
+        return
+    if args.noise_probe:
+        noise_probe(args)
         return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
