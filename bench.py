@@ -18,7 +18,9 @@ steps.  `roofline.traffic` is measured by this run: two rocprofv3 counter
 passes (FETCH_SIZE, then WRITE_SIZE) over a child process that launches the
 same kernel on the same workload beside a device copy of known size, which
 calibrates the counters.  `parts` times the primary rays and the bounce rays
-(the incoherent part) as launches of their own.  `cpu_baseline` times the CPU
+(the incoherent part) as launches of their own, each also with kernel
+variants 4 and 5, and the two back to back with their faster variants
+(`parts.split_launches`, a secondary figure beside the single-launch value).  `cpu_baseline` times the CPU
 oracle (the reference's traversal restated in C++) on the box's host cores.
 Further sections: the small icosphere scene (`small_scene`), any-hit queries
 (`shadow`), the device kd build (`kd_build`), the many-scene mixed workload
@@ -831,13 +833,46 @@ def main():
 
     # the primary rays and the bounce rays as launches of their own
     parts = {}
+    part_args = {}
     for name, (b, c) in (("primary", (0, n_primary)), ("bounce", (n_primary, n - n_primary))):
         args_ = (o_ptr + 12 * b, d_ptr + 12 * b, c, p_ptr + 4 * b, t_ptr + 4 * b)
+        part_args[name] = args_
         pc = scene.trace_counts_device(*args_, **tp)
         pms = launch_stats(ctx, lambda s: scene.trace_closest_device(*args_, sync=s, **tp), 10)
         parts[name] = part_record(c, pc, pms)
+        # the cooperative kernel without and with lane refill (variants 4 and 5)
+        by_var = {}
+        for v in (4, 5):
+            tpv = dict(tp, variant=v)
+            by_var[v] = launch_stats(ctx, lambda s: scene.trace_closest_device(*args_, sync=s, **tpv), 10)
+        best_v = min(by_var, key=by_var.get)
+        parts[name]["kernel_ms_by_variant"] = {str(v): round(ms, 4) for v, ms in by_var.items()}
+        best = part_record(c, pc, by_var[best_v])
+        parts[name]["best_variant"] = {"variant": int(best_v), "kernel_ms": best["kernel_ms"],
+                                       "mrays_s": best["mrays_s"], "achieved_gbs": best["achieved_gbs"],
+                                       "frac_of_hbm": best["frac_of_hbm"]}
     parts["bounce"]["note"] = ("the incoherent part: one cosine-weighted diffuse bounce per primary hit, from the "
                                "hit point into the hemisphere about the facing geometric normal")
+    # the step as a renderer issues it: the primaries, then their bounces, each
+    # launch with its faster variant, back to back on the context's stream
+    vp, vb = parts["primary"]["best_variant"]["variant"], parts["bounce"]["best_variant"]["variant"]
+
+    def split_step(sync):
+        scene.trace_closest_device(*part_args["primary"], sync=False, **dict(tp, variant=vp))
+        scene.trace_closest_device(*part_args["bounce"], sync=sync, **dict(tp, variant=vb))
+    split_step(True)
+    torch.cuda.synchronize()
+    t_split = time.perf_counter()
+    for _ in range(args.steps):
+        split_step(False)
+    torch.cuda.synchronize()
+    split_wall = (time.perf_counter() - t_split) / args.steps
+    kt = ctx.kernel_times(2 * args.steps)
+    parts["split_launches"] = {"variants": {"primary": int(vp), "bounce": int(vb)},
+                               "ms_per_step": round(split_wall * 1e3, 4),
+                               "kernel_ms": round(float(np.sum(kt)) / args.steps, 4),
+                               "mrays_s": round(n / split_wall / 1e6, 1),
+                               "note": "the same rays as two launches (primaries, then bounces), not the headline"}
     same = bool(np.array_equal(d_p.cpu().numpy(), res_p) and
                 np.array_equal(d_t.cpu().numpy().view(np.uint32), res_t.view(np.uint32)))
     parts["results_equal_whole_launch"] = same

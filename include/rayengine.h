@@ -213,9 +213,11 @@ typedef struct re_trace_params {
                                  one coalesced list and tested one triangle
                                  per lane, or, when every visiting lane is at
                                  the same leaf, staged once in LDS (default);
-                              5: as 4 in persistent waves: a lane whose ray has
-                                 finished takes the next ray at the next leaf
-                                 round (rays taken from the end of the array).
+                              5: as 4 in persistent waves: once half a wave's
+                                 lanes have finished their rays, they take the
+                                 next rays at the next leaf round (rays taken
+                                 from the end of the array); faster than 4 on
+                                 incoherent rays.
                               Results are identical for every variant. */
 } re_trace_params;
 

@@ -275,15 +275,20 @@ trace_coop_kernel(RE_CLOSEST_PARAMS) {
   }
 }
 
-// Cooperative variant with refill (variant 5): persistent waves; a lane whose
-// ray has finished takes the next one at the start of the following visit
-// round, from a wave-private chunk of 64 consecutive rays taken from a global
-// counter, so a wave is not held by its longest walk with the other lanes
-// idle (on incoherent bounce rays the longest walk of a wave is about four
-// times the mean).  The leaf work is coop_leaf_test's, shared by every lane
-// at a leaf whatever ray it holds; the rays are taken from the end of the
-// array, so appended bounce rays start first.  Per-ray results equal the
-// other variants' (each ray's walk is the same sequence of visits).
+// Cooperative variant with refill (variant 5): persistent waves; once at
+// least half of a wave's lanes have finished their rays, those lanes take the
+// next rays at the start of the following visit round, from a wave-private
+// chunk of 64 consecutive rays taken from a global counter, so a wave is not
+// held by its longest walk with most lanes idle (on incoherent bounce rays
+// the longest walk of a wave is about four times the mean).  Waiting for half
+// a wave keeps the rays taken together consecutive, so they stay as coherent
+// as their neighbours in the array are.  The leaf work is coop_leaf_test's,
+// shared by every lane at a leaf whatever ray it holds; the rays are taken
+// from the end of the array, so appended bounce rays start first.  Per-ray
+// results equal the other variants' (each ray's walk is the same sequence of
+// visits).
+constexpr int kRefillLanes = 32;  // refill threshold: half a wave
+
 template <bool COUNT, bool ONE_SIDED, bool ONE_OBJ>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6)))
 trace_coop_refill_kernel(RE_CLOSEST_PARAMS, unsigned long long* __restrict__ queue) {
@@ -346,11 +351,13 @@ trace_coop_refill_kernel(RE_CLOSEST_PARAMS, unsigned long long* __restrict__ que
       if (!ONE_OBJ) start_next();
       if (!live) finish();
     }
-    // refill (wave-uniform): lanes without a ray take the next rays of the
-    // wave's chunk; the chunk start is taken with every lane in the atomic
-    // and read back with readfirstlane, so it is a scalar
+    // refill (wave-uniform), once half the lanes or more have no ray: they
+    // take the next rays of the wave's chunk; the chunk start is taken with
+    // every lane in the atomic and read back with readfirstlane, so it is a
+    // scalar.  While fewer lanes are idle, the others hold live walks, so the
+    // wave keeps making progress, and a wave with no ray always refills.
     const unsigned long long need = __ballot(ray < 0);
-    if (need != 0ull && !drained) {
+    if (__popcll(need) >= kRefillLanes && !drained) {
       if (pool_next >= pool_end) {
         const unsigned long long v = atomicAdd(queue, lane == 0 ? 64ull : 0ull);
         const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
