@@ -46,7 +46,7 @@ RE_API re_status re_create(int device, re_ctx** out) {
   ctx->device = device;
   if ((e = hipSetDevice(device)) == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
   for (int i = 0; e == hipSuccess && i < 2 * re_ctx::kRing; ++i) e = hipEventCreate(&ctx->ring[i]);
-  if (e == hipSuccess) e = hipMalloc(&ctx->counts_buf, 256);
+  if (e == hipSuccess) e = hipMalloc(&ctx->counts_buf, kCountsBytes);
   if (e != hipSuccess) {
     re_status st = fail(RE_ERR_HIP, "re_create: %s", hipGetErrorString(e));
     re_destroy(ctx);
@@ -376,15 +376,16 @@ re_status trace_common(re_ctx* ctx, int32_t scene_id, const float* ray_o, const 
     L.counts = d_counts;
   }
   if (sa) {
-    // counters (bytes 0-31) and ray counter (64-71) of the fixed area start at zero
-    HIP_TRY(ctx, launch_zero_words((uint32_t*)header, 18, ctx->stream), "scratch clear");
+    // work counters (bytes 0-31) and ray counters (from byte 64) of the fixed
+    // area start at zero
+    HIP_TRY(ctx, launch_zero_words((uint32_t*)header, 16 + kQueueWords, ctx->stream), "scratch clear");
   } else {
     // cleared by a fill kernel, not a memset node, so that callers may
     // capture the call in a graph
     if (counts) HIP_TRY(ctx, launch_zero_words((uint32_t*)d_counts, (int32_t)(sizeof(TraceCounts) / 4), ctx->stream),
                         "counts clear");
     if (L.variant == 1 || L.variant == 5)
-      HIP_TRY(ctx, launch_zero_words((uint32_t*)L.queue, 2, ctx->stream), "queue clear");
+      HIP_TRY(ctx, launch_zero_words((uint32_t*)L.queue, kQueueWords, ctx->stream), "queue clear");
   }
   HIP_TRY(ctx, ctx->launch_begin(), "event");
   HIP_TRY(ctx, launch_trace_closest(L, ctx->stream), "trace_closest launch");

@@ -112,10 +112,21 @@ hipError_t launch_trace_closest(const TraceLaunch& L, hipStream_t s);
 // Zeroes n 32-bit words with a kernel (no memset node when a caller captures).
 hipError_t launch_zero_words(uint32_t* p, int32_t n, hipStream_t s);
 
-// A fixed device area standing in for the allocated buffers of a call: work
-// counters at byte 0, the persistent ray counter at byte 64, host-pointer
-// staging from kScratchHeader on.
-constexpr size_t kScratchHeader = 256;
+// Ray counters of the persistent kernels, from byte 64 of the counter
+// header: kQueueSlots of them (one per XCD for the refill variant; the
+// persistent per-lane variant uses the first), kQueueStride 64-bit words apart
+// so that each has a 128-byte line of its own.
+constexpr int kQueueSlots = 8;
+constexpr int kQueueStride = 16;
+constexpr int32_t kQueueWords = kQueueSlots * kQueueStride * 2;  // 32-bit words to clear
+static_assert((kQueueSlots & (kQueueSlots - 1)) == 0, "a power of two: the XCD index is masked with it");
+// The counter header (ctx->counts_buf, and the start of the fixed area):
+// work counters at byte 0, the ray counters at byte 64.
+constexpr size_t kCountsBytes = 2048;
+static_assert(64 + 4 * (size_t)kQueueWords <= kCountsBytes, "ray counters inside the header");
+// A fixed device area standing in for the allocated buffers of a call: the
+// counter header, then host-pointer staging from kScratchHeader on.
+constexpr size_t kScratchHeader = kCountsBytes;
 struct ScratchArea {
   void* base;
   size_t bytes;
@@ -362,7 +373,7 @@ struct re_ctx {
   std::map<int32_t, re::DevMlp> mlps;
   std::map<int32_t, re::DevBatch> batches;
   // Staging for host-pointer calls, grown on demand.
-  void* counts_buf = nullptr;  // 256 bytes: work counters at 0, ray queue counter at 64
+  void* counts_buf = nullptr;  // kCountsBytes: work counters at 0, ray counters at 64
   void* noalloc_area = nullptr;  // this device's fixed area of re_trace_closest_noalloc (capi_noalloc.cpp)
   int64_t persistent_grid[3] = {0, 0, 0};  // per block size 64, 128, 256
   int64_t coop_refill_grid[3] = {0, 0, 0};  // the same for variant 5

@@ -278,7 +278,8 @@ trace_coop_kernel(RE_CLOSEST_PARAMS) {
 // Cooperative variant with refill (variant 5): persistent waves; once at
 // least half of a wave's lanes have finished their rays, those lanes take the
 // next rays at the start of the following visit round, from a wave-private
-// chunk of 64 consecutive rays taken from a global counter, so a wave is not
+// chunk of 64 consecutive rays taken from a counter of its XCD's part of the
+// ray array, so a wave is not
 // held by its longest walk with most lanes idle (on incoherent bounce rays
 // the longest walk of a wave is about four times the mean).  Waiting for half
 // a wave keeps the rays taken together consecutive, so they stay as coherent
@@ -299,6 +300,14 @@ trace_coop_refill_kernel(RE_CLOSEST_PARAMS, unsigned long long* __restrict__ que
   const int32_t k_end = ONE_OBJ ? 1 : n_obj;
   int64_t pool_next = 0, pool_end = 0;  // wave-uniform
   bool drained = false;                 // wave-uniform
+  // The rays are split into kQueueSlots equal parts, each with its own
+  // counter; a wave takes chunks from the part of the XCD it runs on (its L2
+  // then holds the part's tree lines) and, once that part is taken, from the
+  // others in turn.
+  const int xcd = (int)(__builtin_amdgcn_s_getreg((20 /* HW_REG_XCC_ID */) | (0 << 6) | ((4 - 1) << 11)) &
+                        (unsigned)(kQueueSlots - 1));
+  int tried = 0;  // wave-uniform: parts found taken
+  const int64_t part = ((n + kQueueSlots - 1) / kQueueSlots + 63) / 64 * 64;
   int64_t ray = -1;
   f3 o{0, 0, 0}, d{0, 0, 0};  // kept only with several objects (the next object's transform)
   f3 ro{0, 0, 0}, rd{0, 0, 0}, fr{0, 0, 0};
@@ -359,15 +368,24 @@ trace_coop_refill_kernel(RE_CLOSEST_PARAMS, unsigned long long* __restrict__ que
     const unsigned long long need = __ballot(ray < 0);
     if (__popcll(need) >= kRefillLanes && !drained) {
       if (pool_next >= pool_end) {
-        const unsigned long long v = atomicAdd(queue, lane == 0 ? 64ull : 0ull);
+        // at most one chunk request per round: a part found taken moves the
+        // wave on to the next part at the next round (every wave tries all
+        // parts before it stops, so every chunk is taken once)
+        const int x = (xcd + tried) & (kQueueSlots - 1);
+        const int64_t lo_x = part * x < n ? part * x : n;
+        const int64_t hi_x = part * (x + 1) < n ? part * (x + 1) : n;
+        const unsigned long long v = atomicAdd(queue + kQueueStride * x, lane == 0 ? 64ull : 0ull);
         const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
         const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
-        const int64_t b = (int64_t)(((unsigned long long)hi << 32) | lo);
-        pool_next = b;
-        pool_end = b + 64 < n ? b + 64 : n;
-        if (b >= n) drained = true;
+        const int64_t b = lo_x + (int64_t)(((unsigned long long)hi << 32) | lo);
+        if (b < hi_x) {
+          pool_next = b;
+          pool_end = b + 64 < hi_x ? b + 64 : hi_x;
+        } else if (++tried == kQueueSlots) {
+          drained = true;
+        }
       }
-      if (!drained) {
+      if (!drained && pool_next < pool_end) {
         const int64_t avail = pool_end - pool_next;
         const int rank = __popcll(need & below);
         if (ray < 0 && rank < avail) {

@@ -66,6 +66,6 @@ def test_persistent_trace_kernel_chunk_fetch_is_wave_uniform(tmp_path, kernel):
     src = ROOT / "rayengine_amd" / "csrc" / "kernels" / "trace.hip"
     for name, body in _kernel_asm(tmp_path, kernel, src).items():
         atom = body.index("global_atomic_add_x2")
-        window = body[atom:atom + 600]
+        window = body[atom:atom + 1500]   # the part bounds' scalar arithmetic may be scheduled in between
         assert window.count("v_readfirstlane_b32") >= 2, f"{name}: chunk start not read back uniformly"
         assert "ds_bpermute" not in window, f"{name}: shuffle of the chunk start"
