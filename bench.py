@@ -6,21 +6,23 @@ seed 1), SAH kd-tree with ropes (leaf <= 8, depth <= 24); 1920x1080 primary
 rays from a camera 0.8 units outside the cluster nearest the origin, looking
 at its centre (FOV 60; 95 % of the primaries hit), issued in 8x8 pixel tiles,
 plus one cosine-weighted diffuse bounce ray per primary hit, in the order of
-their primaries: 4,038,016 rays, 1,964,416 of them bounces.  One step = one
-closest-hit launch over every ray, with rays and results resident in HBM.
+their primaries: 4,038,016 rays, 1,964,416 of them bounces.  One step = the
+closest-hit launch over the primaries (kernel variant 4, the cooperative
+walk), then the launch over their bounce rays (variant 5, the cooperative walk
+with lane refill), back to back on one stream, as a renderer issues them; rays
+and results resident in HBM.
 
 Prints ONE JSON line.  `value` is rays traced per second (Mrays/s) over the
-timed steps.  `roofline` uses the algorithmic bytes of a launch, from the
-kernel's own work counters: per ray 24 B in + 8 B out, 8 B per node entry,
-56 B per leaf record, 40 B per triangle record; divided by the mean kernel
-duration from HIP events recorded on the launch stream during the timed
-steps.  `roofline.traffic` is measured by this run: two rocprofv3 counter
-passes (FETCH_SIZE, then WRITE_SIZE) over a child process that launches the
-same kernel on the same workload beside a device copy of known size, which
-calibrates the counters.  `parts` times the primary rays and the bounce rays
-(the incoherent part) as launches of their own, each also with kernel
-variants 4 and 5, and the two back to back with their faster variants
-(`parts.split_launches`, a secondary figure beside the single-launch value).  `cpu_baseline` times the CPU
+timed steps.  `roofline` is the dominant kernel's, the bounce launch: its
+algorithmic bytes, from the kernel's own work counters (per ray 24 B in + 8 B
+out, 8 B per node entry, 56 B per leaf record, 40 B per triangle record),
+divided by its mean duration from HIP events recorded on the launch stream
+during the timed steps (`roofline.step` has the same for the whole step).
+`roofline.traffic` is measured by this run: two rocprofv3 counter passes
+(FETCH_SIZE, then WRITE_SIZE) over a child process that launches the same
+kernel on the bounce rays beside a device copy of known size, which
+calibrates the counters.  `single_launch` is every ray in one launch of
+variant 4; `parts` times each ray set with variants 4 and 5.  `cpu_baseline` times the CPU
 oracle (the reference's traversal restated in C++) on the box's host cores.
 Further sections: the small icosphere scene (`small_scene`), any-hit queries
 (`shadow`), the device kd build (`kd_build`), the many-scene mixed workload
@@ -52,7 +54,7 @@ BF16_PEAK_TFLOPS = 2500.0
 METRIC = "Mrays/s closest-hit; terrain Mrays/s surrogate vs exact SDF at ≤0.5-voxel err"
 TRACE_KERNEL = {0: "trace_closest_kernel", 1: "trace_persistent_kernel", 2: "trace_closest_kernel",
                 3: "trace_closest_o6_kernel", 4: "trace_coop_kernel", 5: "trace_coop_refill_kernel"}
-DEFAULT_TUNE = {"max_leaf_tris": 8, "tile": 8, "block": 64, "variant": 4}
+DEFAULT_TUNE = {"max_leaf_tris": 8, "tile": 8, "block": 64, "variant": 4, "bounce_variant": 5}
 COPY_PROBE_BYTES = 1 << 28
 # rays of the headline workload at the default tree (leaf 8): 2,073,600
 # primaries + 1,964,416 bounces; checked against the built workload below
@@ -139,17 +141,18 @@ def _dispatch_counters(csv_dir: Path, counter: str):
     return [(k, rows[k][0], rows[k][1], rows[k][2]) for k in sorted(rows)]
 
 
-def measure_traffic(tune, n_rays, timeout_s=150):
-    """HBM-side bytes per launch of the closest-hit kernel on the headline
-    workload, from two rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE) over
-    `bench.py --traffic-probe`, each calibrated by a device copy of known size
-    in the same process.  Runs before this process touches the GPU (the
-    probes are child processes).  Returns a dict, or one with "error"."""
+def measure_traffic(tune, timeout_s=150):
+    """HBM-side bytes per launch of the step's dominant kernel (the bounce
+    rays' launch, with the bounce variant) on the headline workload, from two
+    rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE) over `bench.py
+    --traffic-probe --probe-set bounce`, each calibrated by a device copy of
+    known size in the same process.  Runs before this process touches the GPU
+    (the probes are child processes).  Returns a dict, or one with "error"."""
     exe = subprocess.run(["which", "rocprofv3"], capture_output=True, text=True).stdout.strip()
     if not exe:
         return {"error": "rocprofv3 not found"}
-    kname = TRACE_KERNEL[tune["variant"]]
-    out = {"kernel": kname, "counters": {}}
+    kname = TRACE_KERNEL[tune["bounce_variant"]]
+    out = {"kernel": kname, "launch": "the bounce rays", "counters": {}}
     env = dict(os.environ, TMPDIR="/tmp")
     total = 0.0
     for counter, bytes_per_copy in (("FETCH_SIZE", COPY_PROBE_BYTES), ("WRITE_SIZE", COPY_PROBE_BYTES)):
@@ -157,15 +160,16 @@ def measure_traffic(tune, n_rays, timeout_s=150):
         cmd = [exe, "--pmc", counter, "--output-format", "csv", "-d", str(d), "-o", "run", "--",
                sys.executable, str(ROOT / "bench.py"), "--traffic-probe", "--probe-leaf", str(tune["max_leaf_tris"]),
                "--probe-tile", str(tune["tile"]), "--probe-block", str(tune["block"]),
-               "--probe-variant", str(tune["variant"])]
+               "--probe-variant", str(tune["bounce_variant"]), "--probe-set", "bounce"]
         t0 = time.time()
         try:
             r = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout_s, cwd="/tmp", env=env)
         except subprocess.TimeoutExpired:
             return {"error": f"{counter} pass timed out after {timeout_s} s"}
         rows = _dispatch_counters(d, counter)
-        whole = -(-n_rays // tune["block"]) * tune["block"]     # grid of the launch over every ray
-        trace = [v for _, n, g, v in rows if kname in n and g == whole]
+        # the probe's only launches of this kernel are the bounce launches (its
+        # workload set-up traces the primaries with the default kernel)
+        trace = [v for _, n, g, v in rows if kname in n]
         copy = [v for _, n, g, v in rows if "copyBuffer" in n or ("copy" in n.lower() and kname not in n)]
         if len(trace) < 3 or len(copy) < 3:
             return {"error": f"{counter} pass: rc {r.returncode}, {len(trace)} trace and {len(copy)} copy dispatches "
@@ -188,8 +192,9 @@ def traffic_probe(args):
     """Child process of measure_traffic (and the program the committed counter
     profiles run): the headline workload; the closest-hit kernel launched 3
     times over every ray, 3 times over the primary rays alone and 3 times over
-    the bounce rays alone (the three are told apart by their grid sizes); then
-    3 device copies of COPY_PROBE_BYTES."""
+    the bounce rays alone, in that order (--probe-set each), or 3 times over one
+    of those sets (all, primary, bounce); then 3 device copies of
+    COPY_PROBE_BYTES."""
     import torch
     import rayengine_amd as re
     from rayengine_amd import workloads as W
@@ -205,7 +210,9 @@ def traffic_probe(args):
     d_t = torch.empty(n, dtype=torch.float32, device=dev)
     torch.cuda.synchronize()
     n_primary = 1920 * 1080
-    for b, c in ((0, n), (0, n_primary), (n_primary, n - n_primary)):
+    sets = {"all": (0, n), "primary": (0, n_primary), "bounce": (n_primary, n - n_primary)}
+    chosen = list(sets.values()) if args.probe_set == "each" else [sets[args.probe_set]]
+    for b, c in chosen:
         for _ in range(3):
             scene.trace_closest_device(d_o.data_ptr() + 12 * b, d_d.data_ptr() + 12 * b, c, d_p.data_ptr() + 4 * b,
                                        d_t.data_ptr() + 4 * b, block=args.probe_block, variant=args.probe_variant)
@@ -752,6 +759,7 @@ def main():
     ap.add_argument("--probe-tile", type=int, default=8, help=argparse.SUPPRESS)
     ap.add_argument("--probe-block", type=int, default=64, help=argparse.SUPPRESS)
     ap.add_argument("--probe-variant", type=int, default=3, help=argparse.SUPPRESS)
+    ap.add_argument("--probe-set", default="each", choices=("each", "all", "primary", "bounce"), help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.traffic_probe:
         traffic_probe(args)
@@ -769,7 +777,7 @@ def main():
     traffic = None
     if not args.no_traffic and not args.autotune:
         log("traffic: two rocprofv3 counter passes over the headline launch")
-        traffic = measure_traffic(tune, CLUSTER_RAYS)
+        traffic = measure_traffic(tune)
         log(f"traffic: {traffic.get('bytes_per_launch', traffic.get('error'))}")
 
     import torch
@@ -788,7 +796,7 @@ def main():
         from rayengine_amd.autotune import tune_trace
         tuned, _ = tune_trace(ctx, tri_v, lambda sc, tile: W.cluster_workload_rays(sc, tri_v, tile), 1920, 1080,
                               device_name=torch.cuda.get_device_name(0), cache=None, log=log)
-        tune = {k: tuned[k] for k in tune}
+        tune = {k: tuned.get(k, tune[k]) for k in tune}
     t0 = time.time()
     tree = re.KDTree(tri_v, max_leaf_tris=tune["max_leaf_tris"])
     host_build_s = time.time() - t0
@@ -814,68 +822,64 @@ def main():
     d_t = torch.empty(n, dtype=torch.float32, device=dev)
     torch.cuda.synchronize()
     o_ptr, d_ptr, p_ptr, t_ptr = d_o.data_ptr(), d_d.data_ptr(), d_p.data_ptr(), d_t.data_ptr()
-    counts = scene.trace_counts_device(o_ptr, d_ptr, n, p_ptr, t_ptr, **tp)
-    alg_bytes = 32 * n + 8 * counts[0] + 56 * counts[1] + 40 * counts[2]
+    n_bounce = n - n_primary
+    prim_args = (o_ptr, d_ptr, n_primary, p_ptr, t_ptr)
+    bnc_args = (o_ptr + 12 * n_primary, d_ptr + 12 * n_primary, n_bounce, p_ptr + 4 * n_primary, t_ptr + 4 * n_primary)
+    tpb = {"block": tune["block"], "variant": tune["bounce_variant"]}
+    counts_p = scene.trace_counts_device(*prim_args, **tp)
+    counts_b = scene.trace_counts_device(*bnc_args, **tpb)
+    counts = [a + b for a, b in zip(counts_p, counts_b)]
+    alg_p = 32 * n_primary + 8 * counts_p[0] + 56 * counts_p[1] + 40 * counts_p[2]
+    alg_b = 32 * n_bounce + 8 * counts_b[0] + 56 * counts_b[1] + 40 * counts_b[2]
+    alg_bytes = alg_p + alg_b
     ref_layout_bytes = 32 * n + 128 * (counts[0] + counts[1]) + 52 * counts[2]
 
+    # One step: the primaries, then their bounce rays, as a renderer issues
+    # them (the bounces are made from the primaries' hits), each launch with
+    # its kernel variant, back to back on the context's stream.
+    def step():
+        scene.trace_closest_device(*prim_args, sync=False, **tp)
+        scene.trace_closest_device(*bnc_args, sync=False, **tpb)
+
     for _ in range(args.warmup):
-        scene.trace_closest_device(o_ptr, d_ptr, n, p_ptr, t_ptr, sync=False, **tp)
+        step()
     torch.cuda.synchronize()
     t_start = time.perf_counter()
     for _ in range(args.steps):
-        scene.trace_closest_device(o_ptr, d_ptr, n, p_ptr, t_ptr, sync=False, **tp)
+        step()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t_start
-    ktimes = ctx.kernel_times(args.steps)
-    kms = float(np.mean(ktimes))
-    achieved = alg_bytes / (kms * 1e-3) / 1e9
+    ktimes = np.asarray(ctx.kernel_times(2 * args.steps))
+    kms_p, kms_b = float(np.mean(ktimes[0::2])), float(np.mean(ktimes[1::2]))
+    kms = kms_b                                      # the dominant kernel: the bounce launch
+    achieved = alg_b / (kms * 1e-3) / 1e9
     res_p, res_t = d_p.cpu().numpy(), d_t.cpu().numpy()
 
-    # the primary rays and the bounce rays as launches of their own
+    # the same rays in one launch over every ray, and each ray set with both
+    # cooperative variants (4 and 5)
+    single_ms = launch_stats(ctx, lambda s: scene.trace_closest_device(o_ptr, d_ptr, n, p_ptr, t_ptr, sync=s, **tp),
+                             10)
+    single = {"variant": int(tune["variant"]), "kernel_ms": round(single_ms, 4),
+              "mrays_s": round(n / (single_ms * 1e-3) / 1e6, 1), "algorithmic_bytes": int(alg_bytes),
+              "achieved_gbs": round(alg_bytes / (single_ms * 1e-3) / 1e9, 1),
+              "frac_of_hbm": round(alg_bytes / (single_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    same_single = bool(np.array_equal(d_p.cpu().numpy(), res_p) and
+                       np.array_equal(d_t.cpu().numpy().view(np.uint32), res_t.view(np.uint32)))
     parts = {}
-    part_args = {}
-    for name, (b, c) in (("primary", (0, n_primary)), ("bounce", (n_primary, n - n_primary))):
-        args_ = (o_ptr + 12 * b, d_ptr + 12 * b, c, p_ptr + 4 * b, t_ptr + 4 * b)
-        part_args[name] = args_
-        pc = scene.trace_counts_device(*args_, **tp)
-        pms = launch_stats(ctx, lambda s: scene.trace_closest_device(*args_, sync=s, **tp), 10)
-        parts[name] = part_record(c, pc, pms)
-        # the cooperative kernel without and with lane refill (variants 4 and 5)
+    for name, args_, pc in (("primary", prim_args, counts_p), ("bounce", bnc_args, counts_b)):
         by_var = {}
         for v in (4, 5):
             tpv = dict(tp, variant=v)
             by_var[v] = launch_stats(ctx, lambda s: scene.trace_closest_device(*args_, sync=s, **tpv), 10)
-        best_v = min(by_var, key=by_var.get)
+        used = tune["variant"] if name == "primary" else tune["bounce_variant"]
+        parts[name] = part_record(args_[2], pc, by_var[used])
+        parts[name]["variant"] = int(used)
         parts[name]["kernel_ms_by_variant"] = {str(v): round(ms, 4) for v, ms in by_var.items()}
-        best = part_record(c, pc, by_var[best_v])
-        parts[name]["best_variant"] = {"variant": int(best_v), "kernel_ms": best["kernel_ms"],
-                                       "mrays_s": best["mrays_s"], "achieved_gbs": best["achieved_gbs"],
-                                       "frac_of_hbm": best["frac_of_hbm"]}
+    parts["primary"]["kernel_ms_in_step"] = round(kms_p, 4)
+    parts["bounce"]["kernel_ms_in_step"] = round(kms_b, 4)
     parts["bounce"]["note"] = ("the incoherent part: one cosine-weighted diffuse bounce per primary hit, from the "
                                "hit point into the hemisphere about the facing geometric normal")
-    # the step as a renderer issues it: the primaries, then their bounces, each
-    # launch with its faster variant, back to back on the context's stream
-    vp, vb = parts["primary"]["best_variant"]["variant"], parts["bounce"]["best_variant"]["variant"]
-
-    def split_step(sync):
-        scene.trace_closest_device(*part_args["primary"], sync=False, **dict(tp, variant=vp))
-        scene.trace_closest_device(*part_args["bounce"], sync=sync, **dict(tp, variant=vb))
-    split_step(True)
-    torch.cuda.synchronize()
-    t_split = time.perf_counter()
-    for _ in range(args.steps):
-        split_step(False)
-    torch.cuda.synchronize()
-    split_wall = (time.perf_counter() - t_split) / args.steps
-    kt = ctx.kernel_times(2 * args.steps)
-    parts["split_launches"] = {"variants": {"primary": int(vp), "bounce": int(vb)},
-                               "ms_per_step": round(split_wall * 1e3, 4),
-                               "kernel_ms": round(float(np.sum(kt)) / args.steps, 4),
-                               "mrays_s": round(n / split_wall / 1e6, 1),
-                               "note": "the same rays as two launches (primaries, then bounces), not the headline"}
-    same = bool(np.array_equal(d_p.cpu().numpy(), res_p) and
-                np.array_equal(d_t.cpu().numpy().view(np.uint32), res_t.view(np.uint32)))
-    parts["results_equal_whole_launch"] = same
+    parts["results_equal_single_launch"] = same_single
 
     value = n * args.steps / wall / 1e6
     res = {
@@ -894,21 +898,28 @@ def main():
         "config": {"workload": f"cluster scene: 2^20-triangle Gaussian-cluster soup (64 clusters, seed 1), "
                                f"SAH rope kd-tree leaf<={tune['max_leaf_tris']} depth<=24, 1920x1080 primary rays "
                                f"({tune['tile']}x{tune['tile']} pixel tiles) from 0.8 units outside the cluster "
-                               f"nearest the origin + 1 diffuse bounce per hit, closest hit, leaf-visit cap 50",
+                               f"nearest the origin + 1 diffuse bounce per hit, closest hit, leaf-visit cap 50; "
+                               f"one step = the primaries' launch, then the bounces' launch",
                    "camera": {"pos": list(W.CLUSTER_CAMERA.pos), "view": list(W.CLUSTER_CAMERA.view), "fov": [60, 60]},
-                   "launch": {"block": tune["block"], "variant": tune["variant"], "autotuned": tuned is not None},
-                   "rays": n, "primary_rays": n_primary, "bounce_rays": n - n_primary,
+                   "launch": {"block": tune["block"], "variant": tune["variant"],
+                              "bounce_variant": tune["bounce_variant"], "autotuned": tuned is not None},
+                   "rays": n, "primary_rays": n_primary, "bounce_rays": n_bounce,
                    "triangles": int(tri_v.shape[0]), "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic.get("bytes_per_launch") if traffic else None,
-                     "kernel": TRACE_KERNEL[tune["variant"]],
+                     "kernel": TRACE_KERNEL[tune["bounce_variant"]],
+                     "launch": "the bounce rays (the step's dominant kernel)",
                      "kernel_ms": round(kms, 4),
-                     "algorithmic_bytes": int(alg_bytes),
-                     "reference_layout_bytes": int(ref_layout_bytes),
-                     "reference_layout_gbs": round(ref_layout_bytes / (kms * 1e-3) / 1e9, 1),
-                     "work": {"node_entries": counts[0], "leaves": counts[1], "tri_tests": counts[2],
-                              "rays_hit": counts[3]}},
+                     "algorithmic_bytes": int(alg_b),
+                     "work": {"node_entries": counts_b[0], "leaves": counts_b[1], "tri_tests": counts_b[2],
+                              "rays_hit": counts_b[3]},
+                     "step": {"kernel_ms": round(kms_p + kms_b, 4), "algorithmic_bytes": int(alg_bytes),
+                              "achieved_gbs": round(alg_bytes / ((kms_p + kms_b) * 1e-3) / 1e9, 1),
+                              "reference_layout_bytes": int(ref_layout_bytes),
+                              "work": {"node_entries": counts[0], "leaves": counts[1], "tri_tests": counts[2],
+                                       "rays_hit": counts[3]}}},
+        "single_launch": single,
         "parts": parts,
     }
     if traffic is not None:
