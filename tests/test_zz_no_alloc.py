@@ -44,7 +44,7 @@ def test_noalloc_matches_trace_closest(gpu_ctx):
     sub = np.sort(rng.choice(len(o), 5000, replace=False))
     # two calls in a row through host pointers, the second smaller
     for k, (so, sd) in enumerate(((o, d), (np.ascontiguousarray(o[sub]), np.ascontiguousarray(d[sub])))):
-        for variant in (1, 3, 4):
+        for variant in (1, 3, 4, 5):   # 1 and 5 take rays from the area's ray counters
             say(f"host pointers, call {k}, {len(so)} rays, variant {variant}")
             p, t, cnt = _host_call(gpu_ctx, scene, so, sd, variant=variant)
             pr, tr, cr = scene.trace_counts(so, sd, variant=variant)
