@@ -62,3 +62,30 @@ def test_native_tuner_picks_first_minimum(gpu_ctx):
     for bad in (dict(blocks=(96,)), dict(variants=(6,)), dict(leaf_sizes=(0,)), dict(reps=65)):
         with pytest.raises(re.ReError):
             AT.tune_trace_native(gpu_ctx, tri_v, o[:256], d[:256], **bad)
+
+
+def test_native_tuner_with_device_rays(gpu_ctx):
+    """re_autotune_trace with RE_DEVICE_PTRS: the rays are read where they
+    lie, and the choice and its time are those of a host-pointer call's
+    candidates (same candidate set, each mean positive)."""
+    import ctypes as C
+    import torch
+    from rayengine_amd import _lib
+    tri_v = np.ascontiguousarray(W.small_soup(), np.float32)
+    o, d = re.camera_rays(W.SMALL_CAMERA, 96, 96)
+    dev = torch.device("cuda:0")
+    to, td = torch.from_numpy(o).to(dev), torch.from_numpy(d).to(dev)
+    torch.cuda.synchronize()
+    leaves, blocks, variants = (C.c_int32 * 2)(4, 8), (C.c_int32 * 1)(64), (C.c_int32 * 2)(4, 5)
+    space = _lib.re_tune_space(leaves, 2, blocks, 1, variants, 2, 2)
+    best = _lib.re_tune_result()
+    all_ms = np.zeros(4, np.float32)
+    p = re._trace_params()
+    st = _lib.lib().re_autotune_trace(gpu_ctx.handle, _lib.ptr(tri_v), len(tri_v), None, to.data_ptr(), td.data_ptr(),
+                                      len(o), C.byref(p), C.byref(space), re.RE_DEVICE_PTRS, C.byref(best),
+                                      _lib.ptr(all_ms))
+    assert st == 0, _lib.lib().re_last_error(gpu_ctx.handle)
+    assert (all_ms > 0).all() and best.ms == all_ms.min()
+    assert (best.max_leaf_tris, best.variant) in {(4, 4), (4, 5), (8, 4), (8, 5)} and best.block == 64
+    k = int(np.argmin(all_ms))
+    assert (best.max_leaf_tris, best.variant) == ((4, 8)[k // 2], (4, 5)[k % 2])
