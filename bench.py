@@ -687,12 +687,13 @@ def mixed_workload_section(re, W, torch, ctx, cpu=None, reps=3, n_streams=4):
     d_t = torch.empty(n, dtype=torch.float32, device=dev)
     d_h = torch.empty(n, dtype=torch.uint8, device=dev)
     ptrs = (d_o.data_ptr(), d_d.data_ptr(), d_p.data_ptr(), d_t.data_ptr(), d_h.data_ptr())
-    cfg, meas = tune_mixed(ctxs, soups, tiles, sr, tr, ptrs, torch.cuda.synchronize, log=log)
+    cfg, meas = tune_mixed(ctxs, soups, tiles, sr, tr, ptrs, torch.cuda.synchronize, variants=(4, 5), log=log)
     utrees = [re.KDTree(s, max_leaf_tris=cfg["max_leaf_tris"], ctx=ctx) for s in soups]
-    out["unbatched_autotune"] = {"chosen": {"max_leaf_tris": cfg["max_leaf_tris"], "block": cfg["block"]},
+    out["unbatched_autotune"] = {"chosen": {"max_leaf_tris": cfg["max_leaf_tris"], "block": cfg["block"],
+                                            "variant": cfg["variant"]},
                                  "issue": f"{n_streams} streams",
-                                 "wall_ms": {f"leaf{m['max_leaf_tris']}_block{m['block']}": m["wall_ms"]
-                                             for m in meas}}
+                                 "wall_ms": {f"leaf{m['max_leaf_tris']}_block{m['block']}_variant{m['variant']}":
+                                             m["wall_ms"] for m in meas}}
     first = None
     for streams in (1, n_streams):
         run = MixedRun(ctxs[:streams], utrees, tiles, sr, tr, *ptrs)

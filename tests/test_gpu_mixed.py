@@ -122,9 +122,11 @@ def test_tune_mixed_picks_a_measured_configuration(gpu_ctx):
     d_h = torch.empty(n, dtype=torch.uint8, device=dev)
     ptrs = (d_o.data_ptr(), d_d.data_ptr(), d_p.data_ptr(), d_t.data_ptr(), d_h.data_ptr())
     cfg, meas = tune_mixed([gpu_ctx], soups, list(zip(dims, seeds)), sr, tr, ptrs, torch.cuda.synchronize,
-                           leaf_sizes=(4, 8), blocks=(64, 128), reps=2, log=say)
-    assert len(meas) == 4
-    assert {(m["max_leaf_tris"], m["block"]) for m in meas} == {(4, 64), (4, 128), (8, 64), (8, 128)}
+                           leaf_sizes=(4, 8), blocks=(64, 128), variants=(4, 5), reps=2, log=say)
+    assert len(meas) == 8
+    assert {(m["max_leaf_tris"], m["block"], m["variant"]) for m in meas} == {
+        (lf, b, v) for lf in (4, 8) for b in (64, 128) for v in (4, 5)}
     best = min(meas, key=lambda m: m["wall_ms"])
-    assert (cfg["max_leaf_tris"], cfg["block"]) == (best["max_leaf_tris"], best["block"])
+    assert (cfg["max_leaf_tris"], cfg["block"], cfg["variant"]) == (best["max_leaf_tris"], best["block"],
+                                                                     best["variant"])
     assert cfg["rays"] == n and all(m["wall_ms"] > 0 for m in meas)
