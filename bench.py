@@ -826,6 +826,13 @@ def main():
     n_bounce = n - n_primary
     prim_args = (o_ptr, d_ptr, n_primary, p_ptr, t_ptr)
     bnc_args = (o_ptr + 12 * n_primary, d_ptr + 12 * n_primary, n_bounce, p_ptr + 4 * n_primary, t_ptr + 4 * n_primary)
+    if tuned is not None:
+        # the kernel variant of each launch, measured on its own ray set
+        for key, args_ in (("variant", prim_args), ("bounce_variant", bnc_args)):
+            by_v = {v: launch_stats(ctx, lambda s: scene.trace_closest_device(*args_, sync=s, block=tune["block"],
+                                                                              variant=v), 10) for v in (4, 5)}
+            tune[key] = min(by_v, key=by_v.get)
+        tp = {"block": tune["block"], "variant": tune["variant"]}
     tpb = {"block": tune["block"], "variant": tune["bounce_variant"]}
     counts_p = scene.trace_counts_device(*prim_args, **tp)
     counts_b = scene.trace_counts_device(*bnc_args, **tpb)
