@@ -236,6 +236,27 @@ def noise_probe(args):
     print(json.dumps({"probe": "ok", "noise_ms": round(ctx.last_kernel_ms(), 4)}), flush=True)
 
 
+def train_probe(args):
+    """The program the committed surrogate counter profile runs: the terrain
+    section's network trained for 50 Adam steps at batch 65,536 on 2^22
+    samples of the 256^3 terrain's distance field, then 3 forward passes over
+    2^22 device-resident points."""
+    import torch
+    import rayengine_amd as re
+    ctx = re.Context(0)
+    vol = ctx.noise_volume()
+    dv = re.distance_volume(vol)
+    net = re.Surrogate(ctx, seed=1)
+    net.train(dv, n_samples=1 << 22, batch=65536, steps=50, lr=1e-3, seed=2)
+    n = 1 << 22
+    pts = torch.rand((n, 3), dtype=torch.float32, device="cuda:0", generator=torch.Generator(device="cuda:0").manual_seed(3))
+    out = torch.empty(n, dtype=torch.float32, device="cuda:0")
+    torch.cuda.synchronize()
+    for _ in range(3):
+        net.forward_device(pts.data_ptr(), n, out.data_ptr())
+    print(json.dumps({"probe": "ok", "forward_ms": round(ctx.last_kernel_ms(), 4)}), flush=True)
+
+
 # --------------------------------------------------------------------------- sections
 
 def launch_stats(ctx, fn, steps):
@@ -756,6 +777,7 @@ def main():
                     help="pick leaf size, ray tile, block and variant by measurement before timing")
     ap.add_argument("--traffic-probe", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--noise-probe", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--train-probe", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--probe-leaf", type=int, default=8, help=argparse.SUPPRESS)
     ap.add_argument("--probe-tile", type=int, default=8, help=argparse.SUPPRESS)
     ap.add_argument("--probe-block", type=int, default=64, help=argparse.SUPPRESS)
@@ -767,6 +789,9 @@ def main():
         return
     if args.noise_probe:
         noise_probe(args)
+        return
+    if args.train_probe:
+        train_probe(args)
         return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
