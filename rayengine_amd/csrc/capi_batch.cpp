@@ -68,12 +68,12 @@ RE_API re_status re_batch_create(re_ctx* ctx, const re_batch_scene* scenes, int3
                    "re_batch_create: item_rays in [1,65536], block in {64,128,256}, side_share in [0,0.9]");
   for (int32_t k = 0; k < n_scenes; ++k)
     if (!scenes[k].tree || scenes[k].ray_begin < 0 || scenes[k].ray_count < 0 ||
-        scenes[k].ray_begin + scenes[k].ray_count > n_rays)
+        scenes[k].ray_count > n_rays - scenes[k].ray_begin)
       return set_err(ctx, RE_ERR_INVALID, "re_batch_create: scene ray range outside the ray arrays");
   bool any_surrogate = false;
   for (int32_t k = 0; k < n_tiles; ++k) {
     const re_batch_tile& tl = tiles[k];
-    if (tl.ray_begin < 0 || tl.ray_count < 0 || tl.ray_begin + tl.ray_count > n_rays || tl.mode < 0 || tl.mode > 2)
+    if (tl.ray_begin < 0 || tl.ray_count < 0 || tl.ray_count > n_rays - tl.ray_begin || tl.mode < 0 || tl.mode > 2)
       return set_err(ctx, RE_ERR_INVALID, "re_batch_create: tile ray range or mode invalid");
     if (ctx->volumes.find(tl.vol_id) == ctx->volumes.end())
       return set_err(ctx, RE_ERR_NOT_FOUND, "re_batch_create: unknown tile volume");

@@ -18,10 +18,10 @@ re_status plan_batch(const re_batch_scene* scenes, int32_t n_scenes, const re_ba
       (n_tiles > 0 && !tiles) || item_rays < 1 || item_rays > (1 << 16))
     return fail(RE_ERR_INVALID, "batch plan: bad arguments");
   for (int32_t k = 0; k < n_scenes; ++k)
-    if (scenes[k].ray_begin < 0 || scenes[k].ray_count < 0 || scenes[k].ray_begin + scenes[k].ray_count > n_rays)
+    if (scenes[k].ray_begin < 0 || scenes[k].ray_count < 0 || scenes[k].ray_count > n_rays - scenes[k].ray_begin)
       return fail(RE_ERR_INVALID, "batch plan: scene ray range outside the ray arrays");
   for (int32_t k = 0; k < n_tiles; ++k)
-    if (tiles[k].ray_begin < 0 || tiles[k].ray_count < 0 || tiles[k].ray_begin + tiles[k].ray_count > n_rays ||
+    if (tiles[k].ray_begin < 0 || tiles[k].ray_count < 0 || tiles[k].ray_count > n_rays - tiles[k].ray_begin ||
         tiles[k].mode < 0 || tiles[k].mode > 2)
       return fail(RE_ERR_INVALID, "batch plan: tile ray range or mode invalid");
   const int32_t n_slots = n_scenes + n_tiles;

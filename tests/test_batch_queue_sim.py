@@ -196,6 +196,10 @@ def test_checker_catches_a_broken_plan(workload, whole_range, mistake):
 def test_plan_refuses_bad_ranges():
     with pytest.raises(re.ReError):
         re.batch_plan([(10, 100)], [], 50)
+    with pytest.raises(re.ReError):   # begin + count overflows int64
+        re.batch_plan([(1 << 62, 1 << 62)], [], 50)
+    with pytest.raises(re.ReError):
+        re.batch_plan([], [(0, 1 << 62, 1 << 62)], 50)
     with pytest.raises(re.ReError):
         re.batch_plan([], [(3, 0, 10)], 50)
     with pytest.raises(re.ReError):
