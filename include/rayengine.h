@@ -231,6 +231,32 @@ RE_API void re_trace_default_params(re_trace_params* p);
 RE_API re_status re_trace_closest(re_ctx* ctx, int32_t scene_id, const float* ray_o,
                                   const float* ray_d, int64_t n, const re_trace_params* params,
                                   int32_t* prim_id, float* t, int32_t* obj_id, uint32_t flags);
+/* Closest hit with walk-start hints and hit leaves, for device-resident
+ * pipelines that trace secondary rays from the hits of an earlier launch.
+ * No counterpart in the reference, whose walks always start at the root
+ * (RayEngine/src/cuda-src/ray_functions.cuh:273-291).  As re_trace_closest,
+ * and:
+ *   start_leaf (may be NULL): per ray, a leaf id of the scene (as hit_leaf
+ *     returns them) or -1.  A walk of an object starts at that leaf instead
+ *     of descending from the root when the leaf belongs to the object and
+ *     holds the walk's first point on the same side of every split as the
+ *     descent would take it (bmin <= p < bmax on each axis); otherwise, and
+ *     for ids out of range, at the root.  Results are those of
+ *     re_trace_closest for any hints; only the work differs.
+ *   hit_leaf (may be NULL): per ray, the last leaf visited by the walk
+ *     that found the closest hit, -1 for a miss.  The walk goes on past a
+ *     taken hit until its entry point passes it, so that leaf's cell holds
+ *     the hit point (unless the walk ended at the visit cap or a missing
+ *     rope): a secondary ray leaving the hit point passes it as its
+ *     start_leaf.
+ *   counts (may be NULL): host array of 4, the summed work counters as
+ *     re_trace_counts returns them.
+ * Variants 4 and 5 only (RE_ERR_INVALID otherwise); all arrays are device
+ * pointers (RE_DEVICE_PTRS is required). */
+RE_API re_status re_trace_closest_hinted(re_ctx* ctx, int32_t scene_id, const float* ray_o, const float* ray_d,
+                                         int64_t n, const re_trace_params* params, const int32_t* start_leaf,
+                                         int32_t* prim_id, float* t, int32_t* obj_id, int32_t* hit_leaf,
+                                         uint64_t counts[4], uint32_t flags);
 /* Bounds-checked debug build (librayengine_debug.so, built with
  * RE_DEBUG_BOUNDS): the closest-hit, any-hit and status kernels check every
  * node entry, leaf record and triangle run index against the scene's arrays

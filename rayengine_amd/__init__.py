@@ -708,6 +708,21 @@ class Scene:
         check(lib().re_trace_closest(self.ctx.handle, self.id, o_ptr, d_ptr, n, C.byref(p), prim_ptr, t_ptr, None,
                                      flags), self.ctx.handle)
 
+    def trace_closest_hinted_device(self, o_ptr: int, d_ptr: int, n: int, prim_ptr: int, t_ptr: int,
+                                    start_leaf_ptr: int = 0, hit_leaf_ptr: int = 0, obj_ptr: int = 0, counts=False,
+                                    sync=True, **params):
+        """re_trace_closest_hinted on device buffers (raw pointers, 0 for
+        none): walks may start at start_leaf's leaves, hit_leaf receives
+        each ray's hit leaf.  Variants 4 and 5.  Returns the work counters
+        [entries, leaves, tris, rays_hit] when counts is true."""
+        p = _trace_params(**params)
+        cnt = (C.c_uint64 * 4)() if counts else None
+        flags = RE_DEVICE_PTRS | (0 if sync else RE_ASYNC)
+        check(lib().re_trace_closest_hinted(self.ctx.handle, self.id, o_ptr, d_ptr, n, C.byref(p),
+                                            start_leaf_ptr or None, prim_ptr, t_ptr, obj_ptr or None,
+                                            hit_leaf_ptr or None, cnt, flags), self.ctx.handle)
+        return list(cnt) if counts else None
+
     def set_attributes(self, normals, uvs=None):
         """Per-prim vertex normals ((n_prims, 9): n0, n1, n2) and optional
         vertex uvs ((n_prims, 6): uv0, uv1, uv2) for hit_attributes."""

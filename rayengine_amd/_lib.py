@@ -166,6 +166,8 @@ SIGNATURES = {
                                   C.c_uint32]),
     "re_trace_counts": (C.c_int, [_P, C.c_int32, _P, _P, C.c_int64, C.POINTER(re_trace_params), _P, _P, _U64,
                                   C.c_uint32]),
+    "re_trace_closest_hinted": (C.c_int, [_P, C.c_int32, _P, _P, C.c_int64, C.POINTER(re_trace_params), _P, _P, _P,
+                                          _P, _P, _U64, C.c_uint32]),
     "re_trace_occluded": (C.c_int, [_P, C.c_int32, _P, _P, C.c_int64, C.POINTER(re_trace_params), _P, C.c_uint32]),
     "re_trace_closest_noalloc": (C.c_int, [_P, C.c_int32, _P, _P, C.c_int64, C.POINTER(re_trace_params), _P, _P, _P,
                                            _U64, C.c_uint32]),

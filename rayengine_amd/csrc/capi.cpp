@@ -100,6 +100,7 @@ RE_API re_status re_scene_upload(re_ctx* ctx, const re_object* objs, int32_t n_o
   std::vector<LeafRec> leaves;
   std::vector<TriRec> tris;
   std::vector<ObjXform> xf(n_obj);
+  std::vector<size_t> leaf_end(n_obj);  // one past each object's last leaf
   for (int32_t k = 0; k < n_obj; ++k) {
     const re_object& ob = objs[k];
     const auto& tn = kd_nodes(ob.tree);
@@ -124,6 +125,7 @@ RE_API re_status re_scene_upload(re_ctx* ctx, const re_object* objs, int32_t n_o
       l.first += tb;
       leaves.push_back(l);
     }
+    leaf_end[k] = leaves.size();
     for (TriRec t : tt) {
       t.prim += ob.prim_offset;
       tris.push_back(t);
@@ -147,7 +149,13 @@ RE_API re_status re_scene_upload(re_ctx* ctx, const re_object* objs, int32_t n_o
   // Trailing pad entries (leaf entries of leaf 0) for window reads at the end.
   nodes.resize(nodes.size() + kNodePad, NodeEntry{0.0f, kLeafMeta});
   std::vector<DevLeaf> dleaves(leaves.size());
-  for (size_t i = 0; i < leaves.size(); ++i) dleaves[i] = dev_leaf(leaves[i]);
+  // pad[0] of a device leaf record holds its object's index (the walk-start
+  // hint check reads it)
+  for (int32_t k = 0, i = 0; k < n_obj; ++k)
+    for (; (size_t)i < leaf_end[k]; ++i) {
+      dleaves[i] = dev_leaf(leaves[i]);
+      dleaves[i].pad[0] = (uint32_t)k;
+    }
   auto up =[&](void** dst, const void* src, size_t bytes, const char* what) -> re_status {
     if (bytes == 0) bytes = 64;  // keep every pointer valid
     hipError_t e = hipMalloc(dst, bytes);
@@ -305,7 +313,8 @@ namespace re {
 // grows on demand).
 re_status trace_common(re_ctx* ctx, int32_t scene_id, const float* ray_o, const float* ray_d, int64_t n,
                        const re_trace_params* params, int32_t* prim_id, float* t, int32_t* obj_id,
-                       uint32_t flags, TraceCounts* counts, const ScratchArea* sa) {
+                       uint32_t flags, TraceCounts* counts, const ScratchArea* sa, const int32_t* start_leaf,
+                       int32_t* hit_leaf) {
   if (!ctx) return fail(RE_ERR_INVALID, "re_trace_closest: null ctx");
   if (n < 0 || (n > 0 && (!ray_o || !ray_d || !prim_id || !t)))
     return set_err(ctx, RE_ERR_INVALID, "re_trace_closest: bad arguments");
@@ -317,6 +326,8 @@ re_status trace_common(re_ctx* ctx, int32_t scene_id, const float* ray_o, const 
   if (!(p.block == 0 || p.block == 64 || p.block == 128 || p.block == 256))
     return set_err(ctx, RE_ERR_INVALID, "re_trace_closest: block must be 0, 64, 128 or 256");
   if (p.variant < 0 || p.variant > 5) return set_err(ctx, RE_ERR_INVALID, "re_trace_closest: variant must be 0..5");
+  if ((start_leaf || hit_leaf) && ((p.variant != 4 && p.variant != 5) || !(flags & RE_DEVICE_PTRS)))
+    return set_err(ctx, RE_ERR_INVALID, "re_trace_closest_hinted: variants 4 and 5 with RE_DEVICE_PTRS only");
   if (n == 0) return RE_OK;
   HIP_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
   TraceLaunch L;
@@ -328,10 +339,14 @@ re_status trace_common(re_ctx* ctx, int32_t scene_id, const float* ray_o, const 
   L.block = p.block;
   L.counts = nullptr;
   L.variant = p.variant;
+  L.start_leaf = start_leaf;
+  L.hit_leaf = hit_leaf;
   char* const header = sa ? (char*)sa->base : (char*)ctx->counts_buf;
   L.queue = (unsigned long long*)(header + 64);
   {
-    const int blk = p.block > 0 ? p.block : 256;
+    // the block size launch_trace_closest launches with (its default: 256
+    // threads for variant 1, one wave otherwise)
+    const int blk = trace_launch_block(p.block, p.variant);
     const int slot = blk == 64 ? 0 : (blk == 128 ? 1 : 2);
     int64_t& g = p.variant == 5 ? ctx->coop_refill_grid[slot] : ctx->persistent_grid[slot];
     if (g <= 0) g = p.variant == 5 ? trace_coop_refill_grid(blk) : trace_persistent_grid(blk);
@@ -422,6 +437,24 @@ RE_API re_status re_trace_counts(re_ctx* ctx, int32_t scene_id, const float* ray
   counts[1] = c.leaves;
   counts[2] = c.tris;
   counts[3] = c.rays_hit;
+  return st;
+}
+
+RE_API re_status re_trace_closest_hinted(re_ctx* ctx, int32_t scene_id, const float* ray_o, const float* ray_d,
+                                         int64_t n, const re_trace_params* params, const int32_t* start_leaf,
+                                         int32_t* prim_id, float* t, int32_t* obj_id, int32_t* hit_leaf,
+                                         uint64_t counts[4], uint32_t flags) {
+  if (!ctx) return fail(RE_ERR_INVALID, "re_trace_closest_hinted: null ctx");
+  if (!(flags & RE_DEVICE_PTRS)) return set_err(ctx, RE_ERR_INVALID, "re_trace_closest_hinted: needs RE_DEVICE_PTRS");
+  TraceCounts c{0, 0, 0, 0};
+  re_status st = trace_common(ctx, scene_id, ray_o, ray_d, n, params, prim_id, t, obj_id, flags,
+                              counts ? &c : nullptr, nullptr, start_leaf, hit_leaf);
+  if (counts) {
+    counts[0] = c.entries;
+    counts[1] = c.leaves;
+    counts[2] = c.tris;
+    counts[3] = c.rays_hit;
+  }
   return st;
 }
 

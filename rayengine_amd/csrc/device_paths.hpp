@@ -367,6 +367,31 @@ __device__ __forceinline__ void coop_leaf_test(const TriRec* __restrict__ tris, 
   }
 }
 
+// Walk-start hint (no counterpart in the reference, whose walks all start at
+// the root, ray_functions.cuh:273-291).  The first visit of a walk of object k
+// descends from the root to the leaf holding pe, the point at the walk's
+// t_entry, going below where pe < split and above otherwise.  A leaf's cell
+// box is built from its ancestors' splits (the below child's upper bound and
+// the above child's lower bound are the split itself, and every split lies in
+// its node's cell), so its lower bound on an axis is the largest split the
+// path went above at and its upper bound the smallest split it went below at.
+// Hence when bmin <= pe < bmax holds on every axis, every comparison on the
+// path from the root to that leaf goes the way the path does, and the descent
+// ends there: the walk may start at the leaf, and its visits (and so its
+// result) are those of the walk from the root.  Otherwise, for a NaN pe, a
+// leaf of another object (the object index is kept in the device record's
+// pad[0]) or an id outside [0, n_leaves), the walk starts at the root.
+__device__ __forceinline__ int32_t hinted_start(const DevLeaf* __restrict__ leaves, int64_t n_leaves, int32_t hint,
+                                                int32_t k, f3 pe, int32_t root) {
+  if (hint < 0 || (int64_t)hint >= n_leaves) return root;
+  const DevLeaf* lp = leaves + hint;
+  const float4 h0 = *reinterpret_cast<const float4*>(lp->bmin);
+  const float2 h1 = *reinterpret_cast<const float2*>(lp->bmax + 1);
+  const bool inside = lp->pad[0] == (uint32_t)k && pe.x >= h0.x && pe.y >= h0.y && pe.z >= h0.z && pe.x < h0.w &&
+                      pe.y < h1.x && pe.z < h1.y;
+  return inside ? leaf_ref((uint32_t)hint) : root;
+}
+
 // The closest hit of every lane's ray over objects [k0, k1) (k0 + 1 with
 // ONE_OBJ), the wave walking together: each lane makes its own descent, leaf
 // record read, exit and rope step (trace_ray's walk, same visit cap and
@@ -375,16 +400,20 @@ __device__ __forceinline__ void coop_leaf_test(const TriRec* __restrict__ tris, 
 // part in the wave's work without a ray of their own.  Results are
 // bit-identical to trace_ray's.  Every lane of the wave must call it (it
 // holds wave-wide operations); the loop ends on a wave-uniform branch.
-template <bool COUNT, bool ONE_SIDED, bool ONE_OBJ>
+template <bool COUNT, bool ONE_SIDED, bool ONE_OBJ, int HINT = 0>
 __device__ __forceinline__ void coop_trace_wave(const NodeEntry* __restrict__ nodes,
                                                 const DevLeaf* __restrict__ leaves, const TriRec* __restrict__ tris,
                                                 const ObjXform* __restrict__ objs, int32_t k0, int32_t k1,
                                                 bool has_ray, f3 o, f3 d, int32_t cap, bool secondary, float& best_t,
-                                                int32_t& best_prim, int32_t& best_obj, RayCounts& rc, CoopWave& W) {
+                                                int32_t& best_prim, int32_t& best_obj, RayCounts& rc, CoopWave& W,
+                                                int32_t hint = -1, int64_t n_leaves = 0,
+                                                int32_t* best_leaf = nullptr) {
   const int32_t k_end = ONE_OBJ ? k0 + 1 : k1;
   best_t = kInf;
   best_prim = -1;
   best_obj = -1;
+  int32_t last_leaf = -1;  // HINT == 2: the walk's last visited leaf
+  if (HINT == 2) *best_leaf = -1;
   int32_t k_obj = k0 - 1;
   bool live = false, open = false, done = !has_ray;
   f3 ro{0, 0, 0}, rd{0, 0, 0}, fr{0, 0, 0};
@@ -403,6 +432,7 @@ __device__ __forceinline__ void coop_trace_wave(const NodeEntry* __restrict__ no
       fr = f3{1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z};
       if (!ray_box_inv(ob.root_min, ob.root_max, ro, fr, t_entry, t_exit)) continue;
       cur = ob.root;
+      if (HINT != 0) cur = hinted_start(leaves, n_leaves, hint, k_obj, add(ro, smul(t_entry, rd)), cur);
       found = false;
       hit_t = kInf;
       hit_prim = -1;
@@ -424,6 +454,7 @@ __device__ __forceinline__ void coop_trace_wave(const NodeEntry* __restrict__ no
         best_t = hit_t;
         best_prim = hit_prim;
         best_obj = k_obj;
+        if (HINT == 2) *best_leaf = last_leaf;
       }
       open = false;
       if (ONE_OBJ) done = true;
@@ -447,6 +478,7 @@ __device__ __forceinline__ void coop_trace_wave(const NodeEntry* __restrict__ no
           if (!have_window) w = load_window(nodes, cur);
           leaf_id = descend<COUNT>(nodes, cur, w, pe, rc.ent);
         }
+        if (HINT == 2) last_leaf = (int32_t)leaf_id;
         const DevLeaf* lp = leaves + RE_DBG_LEAF(leaf_id);
         const float4 h0 = *reinterpret_cast<const float4*>(lp->bmin);
         const float4 h1 = *reinterpret_cast<const float4*>(lp->bmax + 1);

@@ -99,6 +99,12 @@ struct TraceLaunch {
   int32_t variant;      // re_trace_params.variant
   int64_t grid_blocks;  // persistent grid size (blocks)
   unsigned long long* queue;  // persistent ray counter, zeroed before launch
+  // Walk-start hints and hit leaves (variants 4 and 5 only; device pointers
+  // or null): start_leaf[i] a leaf the walk of ray i may start at,
+  // hit_leaf[i] receives the last leaf visited by the walk that found ray
+  // i's closest hit (-1 for a miss).
+  const int32_t* start_leaf = nullptr;
+  int32_t* hit_leaf = nullptr;
 };
 
 // Blocks of `block` threads that fill the device at the persistent kernel's
@@ -107,6 +113,10 @@ int64_t trace_persistent_grid(int block);
 // The same for the cooperative refill kernel (variant 5), whose blocks also
 // hold one CoopWave of LDS per wave.
 int64_t trace_coop_refill_grid(int block);
+
+// Threads per block of a closest-hit launch: the requested size, or the
+// default (256 for the persistent per-lane variant 1, one wave otherwise).
+inline int trace_launch_block(int block, int variant) { return block > 0 ? block : (variant == 1 ? 256 : 64); }
 
 hipError_t launch_trace_closest(const TraceLaunch& L, hipStream_t s);
 // Zeroes n 32-bit words with a kernel (no memset node when a caller captures).
@@ -397,5 +407,6 @@ re_status noalloc_setup(re_ctx* ctx);
 // The closest-hit entry points' shared body (capi.cpp).
 re_status trace_common(re_ctx* ctx, int32_t scene_id, const float* ray_o, const float* ray_d, int64_t n,
                        const re_trace_params* params, int32_t* prim_id, float* t, int32_t* obj_id,
-                       uint32_t flags, TraceCounts* counts, const ScratchArea* sa);
+                       uint32_t flags, TraceCounts* counts, const ScratchArea* sa,
+                       const int32_t* start_leaf = nullptr, int32_t* hit_leaf = nullptr);
 }  // namespace re

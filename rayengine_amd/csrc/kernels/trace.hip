@@ -245,9 +245,12 @@ __global__ void __launch_bounds__(256) trace_persistent_kernel(
 
 // Cooperative variant (see coop_trace_wave): one ray per lane, last block of
 // rays first, 6-wave register budget.
-template <bool COUNT, bool ONE_SIDED, bool ONE_OBJ>
+// HINT (as in the refill kernel below): 1 takes each ray's walk-start leaf
+// from start_leaf (may be null), 2 also writes each ray's hit leaf.
+template <bool COUNT, bool ONE_SIDED, bool ONE_OBJ, int HINT = 0>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6)))
-trace_coop_kernel(RE_CLOSEST_PARAMS) {
+trace_coop_kernel(RE_CLOSEST_PARAMS, const int32_t* __restrict__ start_leaf, int32_t* __restrict__ hit_leaf,
+                  int64_t n_leaves) {
   // one CoopWave per wave of the block (dynamic LDS sized by the launch)
   extern __shared__ CoopWave waves[];
   CoopWave& wv = waves[threadIdx.x >> 6];
@@ -260,12 +263,16 @@ trace_coop_kernel(RE_CLOSEST_PARAMS) {
   float best_t;
   int32_t best_prim, best_obj;
   RayCounts rc;
-  coop_trace_wave<COUNT, ONE_SIDED, ONE_OBJ>(nodes, leaves, tris, objs, 0, ONE_OBJ ? 1 : n_obj, has_ray, o, d, cap,
-                                             secondary != 0, best_t, best_prim, best_obj, rc, wv);
+  const int32_t hint = (HINT != 0 && start_leaf && has_ray) ? start_leaf[i] : -1;
+  int32_t best_leaf = -1;
+  coop_trace_wave<COUNT, ONE_SIDED, ONE_OBJ, HINT>(nodes, leaves, tris, objs, 0, ONE_OBJ ? 1 : n_obj, has_ray, o, d,
+                                                   cap, secondary != 0, best_t, best_prim, best_obj, rc, wv, hint,
+                                                   n_leaves, &best_leaf);
   if (has_ray) {
     prim_out[i] = best_prim;
     t_out[i] = best_t;
     if (obj_out) obj_out[i] = best_obj;
+    if (HINT == 2) hit_leaf[i] = best_leaf;
     if (COUNT) {
       atomicAdd(&counts->entries, (unsigned long long)rc.ent);
       atomicAdd(&counts->leaves, (unsigned long long)rc.leaf);
@@ -290,9 +297,18 @@ trace_coop_kernel(RE_CLOSEST_PARAMS) {
 // visits).
 constexpr int kRefillLanes = 32;  // refill threshold: half a wave
 
-template <bool COUNT, bool ONE_SIDED, bool ONE_OBJ>
+// HINT: 0 none; 1 each ray's walk may start at its start_leaf entry (may be
+// null; see hinted_start); 2 as 1, and each ray's hit leaf is written to
+// hit_leaf: the last leaf visited by the walk that found the closest hit, -1
+// for a miss.  A hit is taken at any t below the walk's t_exit, possibly
+// beyond the cell of the leaf whose triangle gave it; the walk then goes on
+// until its t_entry passes the hit, so its last leaf is the one whose cell
+// holds the hit point (when the walk ends there, not at the visit cap or a
+// missing rope), the leaf a ray leaving that point starts in.
+template <bool COUNT, bool ONE_SIDED, bool ONE_OBJ, int HINT = 0>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6)))
-trace_coop_refill_kernel(RE_CLOSEST_PARAMS, unsigned long long* __restrict__ queue) {
+trace_coop_refill_kernel(RE_CLOSEST_PARAMS, unsigned long long* __restrict__ queue,
+                         const int32_t* __restrict__ start_leaf, int32_t* __restrict__ hit_leaf, int64_t n_leaves) {
   extern __shared__ CoopWave waves[];
   CoopWave& W = waves[threadIdx.x >> 6];
   const int lane = (int)(threadIdx.x & 63);
@@ -313,17 +329,19 @@ trace_coop_refill_kernel(RE_CLOSEST_PARAMS, unsigned long long* __restrict__ que
   f3 ro{0, 0, 0}, rd{0, 0, 0}, fr{0, 0, 0};
   float t_entry = 0.0f, t_exit = 0.0f, hit_t = kInf, best_t = kInf;
   int32_t k_obj = -1, cur = 0, visits = 0, hit_prim = -1, best_prim = -1, best_obj = -1;
+  int32_t hint = -1, walk_leaf = -1, best_leaf = -1;  // HINT only
   bool live = false, open = false, found = false, have_window = false;
   uint4 w = make_uint4(0u, 0u, 0u, 0u);
   WalkMemo memo;
   RayCounts rc;
   unsigned c_hit = 0;
   // the walk of object k from the ray (o, d): its first visit, or none
-  auto start_walk = [&](const ObjXform& ob, f3 oo, f3 dd) {
+  auto start_walk = [&](const ObjXform& ob, int32_t k, f3 oo, f3 dd) {
     xform_ray(ob, oo, dd, secondary != 0, ro, rd);
     fr = f3{1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z};
     if (!ray_box_inv(ob.root_min, ob.root_max, ro, fr, t_entry, t_exit)) return;
     cur = ob.root;
+    if (HINT != 0) cur = hinted_start(leaves, n_leaves, hint, k, add(ro, smul(t_entry, rd)), cur);
     found = false;
     hit_t = kInf;
     hit_prim = -1;
@@ -337,13 +355,14 @@ trace_coop_refill_kernel(RE_CLOSEST_PARAMS, unsigned long long* __restrict__ que
   auto start_next = [&]() {
     while (!live && k_obj + 1 < k_end) {
       ++k_obj;
-      start_walk(objs[k_obj], o, d);
+      start_walk(objs[k_obj], k_obj, o, d);
     }
   };
   auto finish = [&]() {
     prim_out[ray] = best_prim;
     t_out[ray] = best_t;
     if (obj_out) obj_out[ray] = best_obj;
+    if (HINT == 2) hit_leaf[ray] = best_leaf;
     if (COUNT && best_prim >= 0) ++c_hit;
     ray = -1;
   };
@@ -355,6 +374,7 @@ trace_coop_refill_kernel(RE_CLOSEST_PARAMS, unsigned long long* __restrict__ que
         best_t = hit_t;
         best_prim = hit_prim;
         best_obj = ONE_OBJ ? 0 : k_obj;
+        if (HINT == 2) best_leaf = walk_leaf;
       }
       open = false;
       if (!ONE_OBJ) start_next();
@@ -395,10 +415,12 @@ trace_coop_refill_kernel(RE_CLOSEST_PARAMS, unsigned long long* __restrict__ que
           best_t = kInf;
           best_prim = -1;
           best_obj = -1;
+          if (HINT == 2) best_leaf = -1;
+          if (HINT != 0) hint = start_leaf ? start_leaf[ray] : -1;
           live = false;
           open = false;
           if (ONE_OBJ) {
-            start_walk(objs[0], no, nd);
+            start_walk(objs[0], 0, no, nd);
           } else {
             o = no;
             d = nd;
@@ -433,6 +455,7 @@ trace_coop_refill_kernel(RE_CLOSEST_PARAMS, unsigned long long* __restrict__ que
           if (!have_window) w = load_window(nodes, cur);
           leaf_id = descend<COUNT>(nodes, cur, w, pe, rc.ent);
         }
+        if (HINT == 2) walk_leaf = (int32_t)leaf_id;
         const DevLeaf* lp = leaves + RE_DBG_LEAF(leaf_id);
         const float4 h0 = *reinterpret_cast<const float4*>(lp->bmin);
         const float4 h1 = *reinterpret_cast<const float4*>(lp->bmax + 1);
@@ -739,7 +762,7 @@ hipError_t launch_trace_closest(const TraceLaunch& L, hipStream_t s) {
   // Default work-group: 256 threads for the persistent kernel, one wave for
   // the one-ray-per-thread kernels (finer dispatch granularity at the tail;
   // 64 and 128 measured alike and ahead of 256 on the cluster workload).
-  const int block = L.block > 0 ? L.block : (L.variant == 1 ? 256 : 64);
+  const int block = trace_launch_block(L.block, L.variant);
   if (L.variant == 5) {
     // Persistent cooperative waves: enough blocks to fill every CU at the
     // kernel's occupancy, never more than the rays need.
@@ -748,10 +771,18 @@ hipError_t launch_trace_closest(const TraceLaunch& L, hipStream_t s) {
     if (grid > want) grid = want;
     const DevScene& S5 = *L.scene;
     const int32_t cap5 = (L.cap > 0 && L.cap < kMaxLeafVisits) ? L.cap : kMaxLeafVisits;
-#define RE_LAUNCH_REFILL(C, O, ONE)                                                                            \
-  hipLaunchKernelGGL((trace_coop_refill_kernel<C, O, ONE>), dim3((unsigned)grid), dim3(block),                 \
+    const int hint = L.hit_leaf ? 2 : (L.start_leaf ? 1 : 0);
+#define RE_LAUNCH_REFILL_H(C, O, ONE, H)                                                                       \
+  hipLaunchKernelGGL((trace_coop_refill_kernel<C, O, ONE, H>), dim3((unsigned)grid), dim3(block),              \
                      (block / 64) * sizeof(CoopWave), s, S5.nodes, S5.leaves, S5.tris, S5.objs, S5.n_obj,        \
-                     L.ray_o, L.ray_d, L.n, cap5, L.secondary, L.prim, L.t, L.obj, L.counts, L.queue)
+                     L.ray_o, L.ray_d, L.n, cap5, L.secondary, L.prim, L.t, L.obj, L.counts, L.queue,            \
+                     L.start_leaf, L.hit_leaf, S5.n_leaves)
+#define RE_LAUNCH_REFILL(C, O, ONE)                                         \
+  do {                                                                      \
+    if (hint == 2) RE_LAUNCH_REFILL_H(C, O, ONE, 2);                        \
+    else if (hint == 1) RE_LAUNCH_REFILL_H(C, O, ONE, 1);                   \
+    else RE_LAUNCH_REFILL_H(C, O, ONE, 0);                                  \
+  } while (0)
     const bool one = S5.n_obj == 1;
     if (L.counts) {
       if (L.one_sided) { if (one) RE_LAUNCH_REFILL(true, true, true); else RE_LAUNCH_REFILL(true, true, false); }
@@ -761,6 +792,7 @@ hipError_t launch_trace_closest(const TraceLaunch& L, hipStream_t s) {
       else { if (one) RE_LAUNCH_REFILL(false, false, true); else RE_LAUNCH_REFILL(false, false, false); }
     }
 #undef RE_LAUNCH_REFILL
+#undef RE_LAUNCH_REFILL_H
     return hipGetLastError();
   }
   const DevScene& S = *L.scene;
@@ -801,11 +833,18 @@ hipError_t launch_trace_closest(const TraceLaunch& L, hipStream_t s) {
     }                                                                     \
   } while (0)
   if (L.variant == 4) {
-#define RE_LAUNCH_COOP(C, O, ONE)                                                                             \
-  hipLaunchKernelGGL((trace_coop_kernel<C, O, ONE>), dim3((unsigned)grid), dim3(block),                       \
+    const int hint = L.hit_leaf ? 2 : (L.start_leaf ? 1 : 0);
+#define RE_LAUNCH_COOP_H(C, O, ONE, H)                                                                        \
+  hipLaunchKernelGGL((trace_coop_kernel<C, O, ONE, H>), dim3((unsigned)grid), dim3(block),                    \
                      (block / 64) * sizeof(CoopWave), s, S.nodes, S.leaves,                                      \
                      S.tris, S.objs, S.n_obj, L.ray_o, L.ray_d, L.n, cap, L.secondary, L.prim, L.t, L.obj,        \
-                     L.counts)
+                     L.counts, L.start_leaf, L.hit_leaf, S.n_leaves)
+#define RE_LAUNCH_COOP(C, O, ONE)                                         \
+  do {                                                                    \
+    if (hint == 2) RE_LAUNCH_COOP_H(C, O, ONE, 2);                        \
+    else if (hint == 1) RE_LAUNCH_COOP_H(C, O, ONE, 1);                   \
+    else RE_LAUNCH_COOP_H(C, O, ONE, 0);                                  \
+  } while (0)
     const bool one = S.n_obj == 1;
     if (L.counts) {
       if (L.one_sided) { if (one) RE_LAUNCH_COOP(true, true, true); else RE_LAUNCH_COOP(true, true, false); }
@@ -815,6 +854,7 @@ hipError_t launch_trace_closest(const TraceLaunch& L, hipStream_t s) {
       else { if (one) RE_LAUNCH_COOP(false, false, true); else RE_LAUNCH_COOP(false, false, false); }
     }
 #undef RE_LAUNCH_COOP
+#undef RE_LAUNCH_COOP_H
   } else if (L.variant == 2) RE_LAUNCH_ALL(trace_closest_kernel, , true);
   else if (L.variant == 3 && S.n_obj == 1) RE_LAUNCH_ALL(trace_closest_o6_kernel, , true);
   else if (L.variant == 3) RE_LAUNCH_ALL(trace_closest_o6_kernel, , false);

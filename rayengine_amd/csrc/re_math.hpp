@@ -230,7 +230,7 @@ struct DevLeaf {
   uint32_t first;
   uint32_t count;
   int32_t rope[6];
-  uint32_t pad[2];
+  uint32_t pad[2];  // pad[0]: the leaf's object index in its scene (set at upload)
 };
 
 RE_HD DevLeaf dev_leaf(const LeafRec& l) {
