@@ -262,15 +262,32 @@ struct CoopWave {
   unsigned long long slot[64];   // (t bits << 32 | record) of the lane's best hit
 };
 
+// Inclusive sum over the 64 lanes of a wave (every lane active), on the
+// data-parallel primitives of the vector ALU: Hillis-Steele within each row
+// of 16 lanes (shifts by 1, 2, 4 and 8 lanes; a source outside the row gives
+// 0), then row 0's total added into row 1 and row 2's into row 3 (broadcast
+// of lane 15 of the row before), and rows 0-1's total into rows 2 and 3
+// (broadcast of lane 31).  Six vector adds instead of six cross-lane reads
+// through the LDS crossbar with their index arithmetic.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111 /* row_shr:1 */, 0xf, 0xf, false);
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112 /* row_shr:2 */, 0xf, 0xf, false);
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114 /* row_shr:4 */, 0xf, 0xf, false);
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118 /* row_shr:8 */, 0xf, 0xf, false);
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142 /* row_bcast:15 */, 0xa, 0xf, false);
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143 /* row_bcast:31 */, 0xc, 0xf, false);
+  return x;
+}
+
 template <bool ONE_SIDED>
 __device__ __forceinline__ void coop_leaf_test(const TriRec* __restrict__ tris, bool at_leaf, uint32_t first,
                                                uint32_t cnt, f3 ro, f3 rd, float& t_exit, bool& found, float& hit_t,
                                                int32_t& hit_prim, CoopWave& W) {
   const int lane = threadIdx.x & 63;
   const unsigned long long at_mask = __ballot(at_leaf);
-  const int lead = __ffsll((long long)at_mask) - 1;
-  const uint32_t lead_first = (uint32_t)__builtin_amdgcn_readfirstlane((int)__shfl(first, lead));
-  const uint32_t lc = (uint32_t)__builtin_amdgcn_readfirstlane((int)__shfl(cnt, lead));
+  const int lead = __ffsll((long long)at_mask) - 1;  // wave-uniform
+  const uint32_t lead_first = (uint32_t)__builtin_amdgcn_readlane((int)first, lead);
+  const uint32_t lc = (uint32_t)__builtin_amdgcn_readlane((int)cnt, lead);
   if (__ballot(at_leaf && (first != lead_first || cnt != lc)) == 0ull) {
     // Every visiting lane tests the same run of records (the same leaf; an
     // empty leaf may share its first index with another leaf, so the count is
@@ -307,17 +324,11 @@ __device__ __forceinline__ void coop_leaf_test(const TriRec* __restrict__ tris, 
     return;
   }
   const uint32_t c = at_leaf ? cnt : 0u;
-  // inclusive scan of the counts over the wave
-  uint32_t incl = c;
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const uint32_t v = __shfl_up(incl, off);
-    if (lane >= off) incl += v;
-  }
+  const uint32_t incl = wave_incl_scan(c);
   const uint32_t excl = incl - c;
   // in a scalar register, so the loops below exit on scalar branches taken by
   // the whole wave (their bodies hold wave barriers)
-  const uint32_t total = (uint32_t)__builtin_amdgcn_readfirstlane((int)__shfl(incl, 63));
+  const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
   W.slot[lane] = ~0ull;
   if (at_leaf) {
     W.ray[lane][0] = make_float4(ro.x, ro.y, ro.z, t_exit);
